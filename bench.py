@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Decoded-packings/s benchmark of the MI355X pallet-loading decoder.
+
+Workload (one GPU): the 20-type instance (587x233x220, dims U[25,115], stand
+flags p=0.7, ~1.2x container volume) decoded by K = 1024 device-resident
+decoders seeded 1..1024 with eps-greedy eps = 0.01, theta = 0.9; one step is
+P = 4 passes of every decoder (4096 packings) with D4' selection on the
+device.  Instance tables and decoder state are resident in HBM before the
+timed region; every step resets the decoders, runs, synchronises and checks
+the per-decoder error flags.
+
+Printed: one JSON line with the contract fields plus
+  roofline      dominant kernel (decode_kernel) against the VALU issue roof
+  cpu_baseline  the reference's g++ build (or the C oracle) on host cores
+  extras        plain one-step-per-launch GPU form, the 256-instance batch
+usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu] [--no-extras]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "plp-gpu_amd"))
+
+K_DEC = 1024
+PASSES = 4
+EPS, THETA = 0.01, 0.9
+# Lane-ops of one brute-force AABB test of the reference predicate
+# (plp_common.hpp:297-321): C1 6 compares + 5 ands, C3 1 compare + 2 min +
+# 2 max + 2 sub + 1 mul + 1 add.
+OPS_PER_TEST = 20
+# MI355X: 256 CUs x 64 lanes x 2.4 GHz VALU issue (= FP64 vector 78.6 TFLOP/s
+# counting an FMA as two flops).
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(text: str, threads: int) -> dict | None:
+    """The reference g++ build (oracle/_ref/ref_driver) or, when it was not
+    built, the C oracle: 256 decoders x 1 pass of the same workload."""
+    inst_path = os.path.join(REPO, "_bench_multi.txt")
+    with open(inst_path, "w") as f:
+        f.write(text)
+    ref = os.path.join(REPO, "oracle", "_ref", "ref_driver")
+    port = os.path.join(REPO, "oracle", "build", "plp_oracle")
+    n = 256
+    try:
+        if os.path.exists(ref):
+            cmd = [ref, "bench", inst_path, "--K", str(n), "--passes", "1", "--seed", "1",
+                   "--eps", str(EPS), "--theta", str(THETA), "--threads", str(threads)]
+            kind = "reference"
+        else:
+            if not os.path.exists(port):
+                subprocess.run(["make", "-C", os.path.join(REPO, "oracle")], check=True,
+                               capture_output=True)
+            cmd = [port, inst_path, "--K", str(n), "--passes", "1", "--seed", "1", "--eps",
+                   str(EPS), "--theta", str(THETA), "--threads", str(threads), "--bench"]
+            kind = "port"
+        out = json.loads(subprocess.run(cmd, check=True, capture_output=True, text=True,
+                                        timeout=600).stdout)
+    except Exception as e:  # noqa: BLE001 -- a missing CPU baseline is reported, not fatal
+        log(f"cpu_baseline unavailable: {e}")
+        return None
+    finally:
+        os.remove(inst_path)
+    return {"value": round(out["packings_per_s"], 3), "unit": "packings/s", "cores": threads,
+            "kind": kind,
+            "sample": f"{n} decoders (seeds 1..{n}) x 1 pass of the same 20-type instance, "
+                      f"eps {EPS}, theta {THETA}; {out['seconds']:.1f} s wall on {threads} threads"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: decoders never exchange data, so a multi-GPU run is "
+              "N independent copies of this job (see DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
+        return 2
+    import plp_amd
+    from plp_amd import synth
+
+    text = synth.multi_instance()
+    inst = plp_amd.Instance.from_text(text)
+    ctx = plp_amd.Ctx(0)
+    eng = plp_amd.Engine(ctx, use_graph=True)
+    prm = plp_amd.params(eps=EPS, theta=THETA)
+    eng.load([inst], [0] * K_DEC, list(range(1, K_DEC + 1)), prm)
+    log("tuning launch shape")
+    tuned, tune_ms = eng.autotune(1)
+    log(f"autotune: {tuned} ({tune_ms:.2f} ms/pass)")
+    for _ in range(a.warmup):
+        eng.reset()
+        eng.run(PASSES)
+        eng.sync()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        eng.reset()
+        eng.run(PASSES)
+        eng.sync()
+        kernel_ms.append(eng.last_ms())
+    elapsed = time.perf_counter() - t0
+    rows, _ = eng.fetch(placements=False)
+    assert all(r.err == 0 for r in rows)
+    work = sum(r.work_pair_box for r in rows)          # pair-box tests of one step
+    shape = eng.describe()
+    packings = K_DEC * PASSES * a.steps
+    value = packings / elapsed
+    ms_launch = sum(kernel_ms) / len(kernel_ms)
+    achieved = work * OPS_PER_TEST / (ms_launch * 1e-3) / 1e12
+    traffic = None
+    tfile = os.path.join(REPO, "profiles", "decode_kernel_hbm_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    line = {
+        "metric": "decoded packings/s at bit-parity vs plain GPU & g++ host; p50/p99 latency",
+        "value": round(value, 1),
+        "unit": "packings/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * elapsed / a.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+i32",
+        "data": "synthetic",
+        "config": {"workload": "20-type instance 587x233x220, K=1024 decoders seeds 1..1024, "
+                               "eps 0.01, theta 0.9, 4 passes per step",
+                   "decoders": K_DEC, "passes_per_step": PASSES, "parallelism": "none (1 GPU)",
+                   "launch": shape, "autotuned": tuned},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
+                     "unit": "Tlane-op/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
+                     "traffic": traffic,
+                     "kernel": "decode_kernel", "kernel_ms": round(ms_launch, 3),
+                     "pair_box_tests_per_launch": work, "ops_per_test": OPS_PER_TEST},
+    }
+    if not a.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads")
+        line["cpu_baseline"] = cpu_baseline(text, threads)
+    if not a.no_extras:
+        extras = {}
+        # Plain one-step-per-launch form, one decoder per launch sequence.
+        n_plain = 8
+        t0 = time.perf_counter()
+        for s in range(n_plain):
+            ctx.plain_run(inst, 1 + s, 1, prm)
+        extras["plain_gpu_packings_per_s"] = round(n_plain / (time.perf_counter() - t0), 2)
+        # 256 uneven instances x 16 seeds in one persistent launch.
+        texts = synth.mixed_batch(256)
+        insts = [plp_amd.Instance.from_text(t) for t in texts]
+        e2 = plp_amd.Engine(ctx, use_graph=True)
+        e2.load(insts, [i for i in range(256) for _ in range(16)],
+                [1 + s for _ in range(256) for s in range(16)], prm)
+        e2.run(1)
+        e2.sync()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            e2.reset()
+            e2.run(1)
+            e2.sync()
+        extras["batch_256x16_packings_per_s"] = round(3 * 4096 / (time.perf_counter() - t0), 1)
+        extras["batch_256x16_kernel_ms"] = round(e2.last_ms(), 3)
+        e2.close()
+        line["extras"] = extras
+    print(json.dumps(line), flush=True)
+    eng.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,228 @@
+/* plp_hip.h -- C ABI of the MI355X (gfx950) 3D pallet-loading decoder.
+ *
+ * Plain pointers, sizes and status codes only; no C++ or torch types cross
+ * this boundary.  Every entry point returns PLP_OK (0) or a PLP_E_* code;
+ * plp_last_error() gives the message of the last failure on the calling
+ * thread.  Buffers are caller-owned host memory unless stated otherwise.
+ *
+ * Conventions (same as the reference, gpu/plp_common.hpp):
+ *   coordinates and dimensions are int cm;
+ *   an extreme point (candidate) is 3 ints (x, y, z);
+ *   an orientation is 3 ints (w, d, h);
+ *   a placed-box AABB is 6 ints (x, y, z, x2, y2, z2);
+ *   a placement row is 7 ints (item_type, x, y, z, w, d, h), the JSON order
+ *   of plp_common.hpp:602-607;
+ *   an item type is 7 ints (l, lf, w, wf, h, hf, qty), file order.
+ */
+#ifndef PLP_HIP_H
+#define PLP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define PLP_OK 0
+#define PLP_E_ARG 1      /* invalid argument */
+#define PLP_E_HIP 2      /* HIP runtime failure (message has the HIP error) */
+#define PLP_E_NOMEM 3    /* device memory exhausted */
+#define PLP_E_EPCAP 4    /* extreme-point list exceeded its capacity (reference device err 2, plp_gpu.cu:969-990) */
+#define PLP_E_RANGE 5    /* container too large for packed keys (reference exit 2, plp_gpu.cu:1263-1268) */
+#define PLP_E_FORMAT 6   /* instance text violates the thpack format (reference InstanceFormatError) */
+#define PLP_E_BUSY 7     /* no free slot / memory for a serving request right now */
+
+const char *plp_status_string(int status);
+const char *plp_last_error(void);
+
+/* Number of HIP devices visible to the process. */
+int plp_device_count(int32_t *count);
+
+/* ------------------------------------------------------- instance parser */
+/* thpack "original" parser with the reference's error cases
+ * (gpu/plp_common.hpp:68-151, plp/parser.py:67-139).  Replaces the host
+ * call plp::load_instance (plp_common.hpp:143) for callers that hold text.
+ * Requires exactly one instance.  types_out receives n_types * 7 ints; when
+ * types_cap is too small PLP_E_ARG is returned and *n_types holds the need. */
+int plp_parse_instance(const char *text, int32_t *L, int32_t *W, int32_t *H,
+                       int32_t *n_types, int32_t *types_out, int32_t types_cap,
+                       char *err, int32_t errlen);
+
+/* --------------------------------------------------------------- context */
+/* One device, one HIP stream, grow-only device scratch.  Not reentrant: one
+ * host thread per context (the reference backends are single-threaded,
+ * plp_gpu.cu:415-422). */
+typedef struct plp_ctx plp_ctx;
+
+int plp_ctx_create(int32_t device, plp_ctx **out);
+void plp_ctx_destroy(plp_ctx *ctx);
+
+/* ----------------------------------------------- seam 1: feasibility step */
+/* Replaces FeasibilityBackend::check_all (gpu/plp_common.hpp:326-332), as
+ * implemented by CudaBackend::check_all (gpu/plp_gpu.cu:443-502):
+ * bits_out[ci * n_ors + oi] = 1 iff (cand ci, orientation oi) passes C2, C3
+ * and C1 (plp_common.hpp:297-321) against the n_aabb placed boxes.  One
+ * H2D + one launch + one D2H per call: the plain one-step-per-launch form. */
+int plp_feas_check_all(plp_ctx *ctx, const int32_t *cand_xyz, int32_t n_cand,
+                       const int32_t *ors_wdh, int32_t n_ors,
+                       const int32_t *aabb6, int32_t n_aabb,
+                       int32_t L, int32_t W, int32_t H, uint8_t *bits_out);
+
+/* ------------------------------------------ seam 2: batched per-step form */
+/* Replaces CudaBatchBackend::check_all_batch (gpu/plp_gpu.cu:174-255): K
+ * rows that share one orientation table, each with its own candidates and
+ * boxes; bits_out[k] gets n_cand[k] * n_ors bytes.  No K limit (the
+ * reference caps K at 16, plp_gpu.cu:106). */
+int plp_feas_check_batch(plp_ctx *ctx, int32_t K,
+                         const int32_t *const *cand_xyz, const int32_t *n_cand,
+                         const int32_t *ors_wdh, int32_t n_ors,
+                         const int32_t *const *aabb6, const int32_t *n_aabb,
+                         int32_t L, int32_t W, int32_t H, uint8_t *const *bits_out);
+
+/* ------------------------------------------------------------ FSR on GPU */
+/* Replaces plp::compute_fsr / plp::stability_score (plp_common.hpp:356-399)
+ * and the device stability_kernel (plp_gpu.cu:1121-1221) for n_packs
+ * packings given as placement rows (7 ints, JSON order) concatenated, with
+ * pack_off[n_packs + 1] row offsets.  fsr_out (may be NULL) receives one
+ * double per row; score_out one double per packing.  mu is the friction
+ * coefficient (reference plp::MU, default 0.2). */
+int plp_fsr_eval(plp_ctx *ctx, const int32_t *rows7, const int32_t *pack_off,
+                 int32_t n_packs, double mu, double *fsr_out, double *score_out);
+
+/* ------------------------------------------------------- device MT19937 */
+/* The --rngdump probe (gpu/plp_gpu.cu:1479-1536) generalised: for each of
+ * n_seeds seeds, run the op script on a device MT19937 seeded like CPython
+ * random.Random(seed) (plp_common.hpp:164-250).  ops[i] = 0 -> random()
+ * (double into out_f64), 1 -> getrandbits(arg[i]), 2 -> randrange(arg[i])
+ * (into out_u32).  Outputs are n_seeds * n_ops, seed-major; unused slots 0. */
+int plp_rng_script(plp_ctx *ctx, const uint64_t *seeds, int32_t n_seeds,
+                   const int32_t *ops, const int32_t *args, int32_t n_ops,
+                   double *out_f64, uint32_t *out_u32);
+
+/* ------------------------------ seam 3: device-resident lockstep decoders */
+/* Replaces CudaDeviceBackend init/run_pass/fetch (gpu/plp_gpu.cu:1225-1390)
+ * and the D4' selection of run_device (plp_gpu.cu:1395-1473), extended to
+ * many instances: decoder d decodes instance dec_inst[d] with
+ * random.Random(dec_seed[d]) (the reference seeds row k with seed + k,
+ * plp_gpu.cu:350).  Each decoder runs whole passes in one persistent
+ * wavefront; RNG, selection state and the best packings stay in device
+ * memory between calls. */
+
+typedef struct {
+    int32_t L, W, H;
+    int32_t n_types;
+    const int32_t *types;   /* n_types * 7: l, lf, w, wf, h, hf, qty */
+} plp_instance_desc;
+
+typedef struct {
+    double eps;          /* eps-greedy probability (reference --eps, default 0.3) */
+    double theta;        /* D4' acceptance threshold (--theta, default 0.9) */
+    double mu;           /* FSR friction coefficient (--mu / PLP_MU, default 0.2) */
+    int32_t use_theta;   /* 0 = --no-theta (accept every packing) */
+    int32_t first_fit;   /* --first-fit */
+    int32_t order;       /* 0 = volume, 1 = input (--order) */
+    int32_t reserved;
+} plp_params;
+
+typedef struct {
+    int32_t waves_per_decoder;   /* 0 = automatic */
+    int32_t ep_cap;              /* extreme-point capacity per decoder, 0 = automatic */
+    int32_t blocks_per_cu;       /* persistent blocks per CU, 0 = as many as fit */
+    int32_t use_graph;           /* capture the launch sequence in a hipGraph */
+} plp_engine_config;
+
+typedef struct {
+    int32_t n_boxes;
+    int32_t fallback;     /* D4': no packing reached theta */
+    int32_t n_passes;
+    int32_t err;          /* 0, or PLP_E_EPCAP for a decoder whose EP list overflowed */
+    int64_t volume;       /* volume_cm3 */
+    double score;         /* stability_score of the selected packing */
+    uint64_t seed;
+    uint64_t work_pair_box;  /* sum over steps of n_cand * n_ors * n_boxes (brute-force AABB tests) */
+    int32_t max_ep;          /* largest extreme-point list reached */
+    int32_t reserved;
+} plp_row_result;
+
+typedef struct plp_engine plp_engine;
+
+/* --------------------------------------------- plain one-step-per-launch */
+/* The reference's default GPU form (plp_common.hpp:471-523 driving
+ * CudaBackend, plp_gpu.cu:414-502): the decode loop, RNG and extreme-point
+ * update run on the host; every item step makes one plp_feas_check_all call
+ * (H2D + launch + D2H + sync); FSR runs through plp_fsr_eval.  Fixed pass
+ * count, D4' selection as in run().  placements gets n_boxes * 7 ints (cap:
+ * total item count * 7). */
+int plp_plain_run(plp_ctx *ctx, const plp_instance_desc *inst, uint64_t seed,
+                  int32_t n_passes, const plp_params *params, plp_row_result *row,
+                  int32_t *placements);
+
+
+int plp_engine_create(plp_ctx *ctx, const plp_engine_config *cfg, plp_engine **out);
+void plp_engine_destroy(plp_engine *eng);
+
+/* Upload instances and decoders; resets every decoder (RNG seeded, no
+ * packing yet).  Instances must satisfy L, W, H < 2^21 and
+ * bitlen(L) + bitlen(W) + bitlen(H) <= 31 (PLP_E_RANGE otherwise). */
+int plp_engine_load(plp_engine *eng, const plp_instance_desc *insts, int32_t n_inst,
+                    const int32_t *dec_inst, const uint64_t *dec_seed, int32_t n_dec,
+                    const plp_params *params);
+
+/* Return every decoder to its freshly loaded state (RNG reseeded on the next
+ * run, no packing kept) without re-uploading the tables. */
+int plp_engine_reset(plp_engine *eng);
+
+/* Enqueue n_passes more passes for every decoder (asynchronous). */
+int plp_engine_run(plp_engine *eng, int32_t n_passes);
+/* Wait for the engine's work; returns PLP_E_EPCAP if any decoder overflowed
+ * (the affected rows carry err; see plp_engine_config.ep_cap). */
+int plp_engine_sync(plp_engine *eng);
+/* Largest placement count of any decoder (row stride for fetch). */
+int plp_engine_max_items(plp_engine *eng, int32_t *max_items);
+/* Copy results: rows[n_dec]; placements (may be NULL) gets max_items * 7
+ * ints per decoder, the selected packing in JSON row order. */
+int plp_engine_fetch(plp_engine *eng, plp_row_result *rows, int32_t *placements);
+/* Milliseconds of device time of the last plp_engine_run (HIP events on the
+ * engine's stream). */
+int plp_engine_last_ms(plp_engine *eng, float *ms);
+/* Launch shape chosen for the loaded batch. */
+int plp_engine_describe(plp_engine *eng, int32_t *waves_per_decoder, int32_t *ep_cap,
+                        int32_t *lds_bytes, int32_t *grid_blocks);
+
+/* Autotuner: time candidate launch shapes (waves per decoder, EP capacity,
+ * blocks per CU) on the loaded batch with n_passes passes each, keep the
+ * fastest, reset the decoders and return its shape.  The decoders' results
+ * do not depend on the shape. */
+int plp_engine_autotune(plp_engine *eng, int32_t n_passes, plp_engine_config *best,
+                        float *best_ms);
+
+/* ------------------------------------------------------- serving loop */
+/* Continuous batching: requests (one instance, n_decoders seeds) are
+ * admitted while device memory and slots allow, run concurrently on their
+ * own streams, and complete independently.  No reference counterpart (the
+ * reference runs one instance per process). */
+typedef struct plp_server plp_server;
+
+int plp_server_create(int32_t device, int32_t max_inflight, int64_t mem_budget_bytes,
+                      plp_server **out);
+void plp_server_destroy(plp_server *srv);
+/* Admit a request; PLP_E_BUSY when no slot/memory is free right now.
+ * *req_id identifies it.  Seeds are seed0 .. seed0 + n_decoders - 1. */
+int plp_server_submit(plp_server *srv, const plp_instance_desc *inst, int32_t n_decoders,
+                      uint64_t seed0, int32_t n_passes, const plp_params *params,
+                      int64_t *req_id);
+/* Poll: *done = 1 when the request finished; then rows (n_decoders) and
+ * placements (may be NULL, max_items * 7 ints per row) are filled and the
+ * request's memory is released.  *max_items receives the row stride. */
+int plp_server_poll(plp_server *srv, int64_t req_id, int32_t *done, plp_row_result *rows,
+                    int32_t *placements, int32_t placements_cap_rows, int32_t *max_items);
+/* Requests currently admitted. */
+int plp_server_inflight(plp_server *srv, int32_t *n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

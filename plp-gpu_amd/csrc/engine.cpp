@@ -1,0 +1,509 @@
+// engine.cpp -- device-resident lockstep decoders (C ABI seam 3).
+//
+// Replaces CudaDeviceBackend (gpu/plp_gpu.cu:1225-1390) and run_device's
+// D4' selection (plp_gpu.cu:1395-1473) for many instances at once.  Layout in
+// HBM, all allocated once per load:
+//   instance headers (64 B each), step-type table, orientation table (20 ints
+//   per instance type), decoder -> instance map, decoder seeds, work order,
+//   per decoder: MT19937 state (625 words), DecState (64 B), two packings of
+//   max_items rows x 7 ints (best overall / best accepted).
+// A run is one fill kernel (queue head) + one persistent decode launch; it
+// can be captured once in a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "capi_internal.hpp"
+#include "kernels.hpp"
+
+using namespace plpi;
+using plpk::DecState;
+using plpk::InstHdr;
+
+struct plp_engine {
+    plp_ctx* ctx = nullptr;
+    plp_engine_config cfg{};
+    bool loaded = false;
+    int n_inst = 0, n_dec = 0, max_items = 1, box_cap = 1;
+    int ep_cap = 0, lds = 0, grid = 0, occ = 0;
+    plp_params params{};
+    int use_rng = 0;
+    int passes_done = 0;
+    bool need_reset = true;
+    bool states_fresh = false;
+    DevBuf d_inst, d_step, d_ors, d_dec_inst, d_dec_seed, d_order, d_retry, d_mt, d_state,
+        d_best_all, d_best_acc, d_counter;
+    std::vector<int32_t> h_order;
+    std::vector<int32_t> h_dec_inst;
+    std::vector<uint64_t> h_seed;
+    std::vector<DecState> h_state;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.0f;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int g_passes = -1, g_reset = -1, g_grid = -1, g_cap = -1;
+};
+
+namespace {
+
+int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+void drop_graph(plp_engine* g) {
+    if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    g->gexec = nullptr;
+    g->graph = nullptr;
+    g->g_passes = g->g_reset = g->g_grid = g->g_cap = -1;
+}
+
+// Launch shape for an EP capacity: LDS bytes, resident blocks per CU, grid.
+int shape(plp_engine* g, int ep_cap, int blocks_per_cu_req, int n_work) {
+    const plpk::LdsLayout lay = plpk::lds_layout(ep_cap, g->box_cap, g->use_rng);
+    if (lay.total + 256 > g->ctx->max_lds)
+        return fail(PLP_E_EPCAP, "decoder state (" + std::to_string(lay.total) +
+                                     " B of LDS for " + std::to_string(ep_cap) +
+                                     " extreme points) exceeds the block LDS limit");
+    int occ = 0;
+    hipError_t e = plpk::decode_occupancy(lay.total, &occ);
+    if (e != hipSuccess) return hip_fail(e, "decode_kernel occupancy");
+    if (occ < 1) return fail(PLP_E_EPCAP, "decode_kernel cannot be resident with this LDS size");
+    int bpc = blocks_per_cu_req > 0 ? std::min(blocks_per_cu_req, occ) : occ;
+    g->ep_cap = ep_cap;
+    g->lds = lay.total;
+    g->occ = occ;
+    g->grid = std::max(1, std::min(n_work, bpc * g->ctx->n_cu));
+    return PLP_OK;
+}
+
+plpk::DecodeArgs make_args(plp_engine* g, int n_passes, int reset, const int32_t* order,
+                           int n_work) {
+    plpk::DecodeArgs a;
+    a.inst = (const InstHdr*)g->d_inst.p;
+    a.step_type = (const int32_t*)g->d_step.p;
+    a.ors_tab = (const int32_t*)g->d_ors.p;
+    a.dec_inst = (const int32_t*)g->d_dec_inst.p;
+    a.dec_seed = (const uint64_t*)g->d_dec_seed.p;
+    a.work_order = order;
+    a.n_work = n_work;
+    a.n_passes = n_passes;
+    a.mt = (uint32_t*)g->d_mt.p;
+    a.state = (DecState*)g->d_state.p;
+    a.best_all = (int32_t*)g->d_best_all.p;
+    a.best_acc = (int32_t*)g->d_best_acc.p;
+    a.max_items = g->max_items;
+    a.ep_cap = g->ep_cap;
+    a.box_cap = g->box_cap;
+    a.use_rng = g->use_rng;
+    a.use_theta = g->params.use_theta;
+    a.reset = reset;
+    a.eps = g->params.eps;
+    a.theta = g->params.theta;
+    a.mu = g->params.mu;
+    a.work_counter = (uint32_t*)g->d_counter.p;
+    return a;
+}
+
+int launch_run(plp_engine* g, int n_passes, int reset) {
+    plp_ctx* c = g->ctx;
+    plpk::DecodeArgs a = make_args(g, n_passes, reset, (const int32_t*)g->d_order.p, g->n_dec);
+    hipError_t e;
+    if (g->cfg.use_graph) {
+        if (!g->gexec || g->g_passes != n_passes || g->g_reset != reset || g->g_grid != g->grid ||
+            g->g_cap != g->ep_cap) {
+            drop_graph(g);
+            e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+            hipError_t le = plpk::launch_decode(a, g->grid, g->lds, c->stream);
+            e = hipStreamEndCapture(c->stream, &g->graph);
+            if (le != hipSuccess) return hip_fail(le, "decode capture");
+            if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+            e = hipGraphInstantiate(&g->gexec, g->graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+            g->g_passes = n_passes;
+            g->g_reset = reset;
+            g->g_grid = g->grid;
+            g->g_cap = g->ep_cap;
+        }
+        e = hipGraphLaunch(g->gexec, c->stream);
+        if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+        return PLP_OK;
+    }
+    e = plpk::launch_decode(a, g->grid, g->lds, c->stream);
+    if (e != hipSuccess) return hip_fail(e, "decode_kernel launch");
+    return PLP_OK;
+}
+
+int read_states(plp_engine* g) {
+    g->h_state.resize((size_t)g->n_dec);
+    hipError_t e = hipMemcpyAsync(g->h_state.data(), g->d_state.p,
+                                  sizeof(DecState) * (size_t)g->n_dec, hipMemcpyDeviceToHost,
+                                  g->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "D2H decoder states");
+    g->states_fresh = true;
+    return PLP_OK;
+}
+
+}  // namespace
+
+extern "C" int plp_engine_create(plp_ctx* ctx, const plp_engine_config* cfg, plp_engine** out) {
+    if (!ctx || !out) return fail(PLP_E_ARG, "NULL argument");
+    *out = nullptr;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    plp_engine* g = new plp_engine();
+    g->ctx = ctx;
+    if (cfg) g->cfg = *cfg;
+    if (g->cfg.waves_per_decoder < 0 || g->cfg.waves_per_decoder > 1 || g->cfg.ep_cap < 0 ||
+        g->cfg.blocks_per_cu < 0) {
+        delete g;
+        return fail(PLP_E_ARG, "config: waves_per_decoder must be 0 or 1; caps >= 0");
+    }
+    if ((e = hipEventCreate(&g->ev0)) != hipSuccess || (e = hipEventCreate(&g->ev1)) != hipSuccess) {
+        delete g;
+        return hip_fail(e, "hipEventCreate");
+    }
+    int rc = g->d_counter.reserve(64, "queue head");
+    if (rc != PLP_OK) {
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return PLP_OK;
+}
+
+extern "C" void plp_engine_destroy(plp_engine* g) {
+    if (!g) return;
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    drop_graph(g);
+    for (DevBuf* b : {&g->d_inst, &g->d_step, &g->d_ors, &g->d_dec_inst, &g->d_dec_seed,
+                      &g->d_order, &g->d_retry, &g->d_mt, &g->d_state, &g->d_best_all,
+                      &g->d_best_acc, &g->d_counter})
+        b->release();
+    if (g->ev0) (void)hipEventDestroy(g->ev0);
+    if (g->ev1) (void)hipEventDestroy(g->ev1);
+    delete g;
+}
+
+extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
+                               const int32_t* dec_inst, const uint64_t* dec_seed, int32_t n_dec,
+                               const plp_params* params) {
+    if (!g || !insts || !dec_inst || !dec_seed || !params) return fail(PLP_E_ARG, "NULL argument");
+    if (n_inst < 1 || n_dec < 1) return fail(PLP_E_ARG, "need at least one instance and decoder");
+    if (params->order != 0 && params->order != 1) return fail(PLP_E_ARG, "order must be 0 or 1");
+    if (!(params->eps >= 0.0) || params->eps > 1.0) return fail(PLP_E_ARG, "eps must be in [0, 1]");
+    hipError_t e = hipSetDevice(g->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    e = hipStreamSynchronize(g->ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "engine sync");
+    drop_graph(g);
+    g->loaded = false;
+    std::vector<InstHdr> hdr((size_t)n_inst);
+    std::vector<int32_t> steps;
+    std::vector<int32_t> ors;
+    std::vector<int64_t> cost((size_t)n_inst, 0);
+    int max_items = 1;
+    for (int i = 0; i < n_inst; i++) {
+        plp::Instance in;
+        int rc = to_instance(&insts[i], &in);
+        if (rc != PLP_OK) return fail(rc, "instance " + std::to_string(i) + ": " + plp_last_error());
+        std::vector<int> seq = plp::item_sequence(in, params->order == 0 ? "volume" : "input");
+        InstHdr& h = hdr[(size_t)i];
+        std::memset(&h, 0, sizeof h);
+        h.L = in.L;
+        h.W = in.W;
+        h.H = in.H;
+        h.n_steps = (int32_t)seq.size();
+        h.step_off = (int32_t)steps.size();
+        h.ors_off = (int32_t)(ors.size() / plpk::kOrsRow);
+        h.n_types = (int32_t)in.items.size();
+        h.xbits = bitlen(in.L);
+        h.ybits = bitlen(in.W);
+        steps.insert(steps.end(), seq.begin(), seq.end());
+        for (const plp::Item& it : in.items) {
+            auto o = plp::orientations(it);
+            int32_t row[plpk::kOrsRow] = {0};
+            row[0] = (int32_t)o.size();
+            for (size_t k = 0; k < o.size(); k++) {
+                row[1 + 3 * k] = o[k][0];
+                row[2 + 3 * k] = o[k][1];
+                row[3 + 3 * k] = o[k][2];
+            }
+            ors.insert(ors.end(), row, row + plpk::kOrsRow);
+        }
+        max_items = std::max(max_items, (int)seq.size());
+        cost[(size_t)i] = (int64_t)seq.size() * (int64_t)seq.size();
+    }
+    if (ors.empty()) ors.assign(plpk::kOrsRow, 0);
+    if (steps.empty()) steps.push_back(0);
+    for (int d = 0; d < n_dec; d++)
+        if (dec_inst[d] < 0 || dec_inst[d] >= n_inst)
+            return fail(PLP_E_ARG, "dec_inst out of range");
+    g->n_inst = n_inst;
+    g->n_dec = n_dec;
+    g->max_items = max_items;
+    g->box_cap = max_items;
+    g->params = *params;
+    g->use_rng = (params->first_fit || params->eps == 0.0) ? 0 : 1;
+    g->h_dec_inst.assign(dec_inst, dec_inst + n_dec);
+    g->h_seed.assign(dec_seed, dec_seed + n_dec);
+    // Work order: most expensive instances first (longest-processing-time
+    // first keeps the persistent grid's tail short); stable by decoder index.
+    g->h_order.resize((size_t)n_dec);
+    for (int d = 0; d < n_dec; d++) g->h_order[(size_t)d] = d;
+    std::stable_sort(g->h_order.begin(), g->h_order.end(), [&](int a, int b) {
+        return cost[(size_t)dec_inst[a]] > cost[(size_t)dec_inst[b]];
+    });
+    const size_t rows_bytes = (size_t)n_dec * (size_t)max_items * 7 * sizeof(int32_t);
+    int rc;
+    if ((rc = g->d_inst.reserve(sizeof(InstHdr) * hdr.size(), "instance headers")) ||
+        (rc = g->d_step.reserve(4 * steps.size(), "step table")) ||
+        (rc = g->d_ors.reserve(4 * ors.size(), "orientation table")) ||
+        (rc = g->d_dec_inst.reserve(4 * (size_t)n_dec, "decoder map")) ||
+        (rc = g->d_dec_seed.reserve(8 * (size_t)n_dec, "decoder seeds")) ||
+        (rc = g->d_order.reserve(4 * (size_t)n_dec, "work order")) ||
+        (rc = g->d_retry.reserve(4 * (size_t)n_dec, "retry order")) ||
+        (rc = g->d_mt.reserve(4 * plpk::kMtWords * (size_t)n_dec, "RNG states")) ||
+        (rc = g->d_state.reserve(sizeof(DecState) * (size_t)n_dec, "decoder states")) ||
+        (rc = g->d_best_all.reserve(rows_bytes, "best packings")) ||
+        (rc = g->d_best_acc.reserve(rows_bytes, "accepted packings")))
+        return rc;
+    struct Up {
+        void* dst;
+        const void* src;
+        size_t n;
+    } ups[] = {
+        {g->d_inst.p, hdr.data(), sizeof(InstHdr) * hdr.size()},
+        {g->d_step.p, steps.data(), 4 * steps.size()},
+        {g->d_ors.p, ors.data(), 4 * ors.size()},
+        {g->d_dec_inst.p, dec_inst, 4 * (size_t)n_dec},
+        {g->d_dec_seed.p, dec_seed, 8 * (size_t)n_dec},
+        {g->d_order.p, g->h_order.data(), 4 * (size_t)n_dec},
+    };
+    for (const Up& u : ups) {
+        e = hipMemcpyAsync(u.dst, u.src, u.n, hipMemcpyHostToDevice, g->ctx->stream);
+        if (e != hipSuccess) return hip_fail(e, "H2D engine tables");
+    }
+    if ((e = hipStreamSynchronize(g->ctx->stream)) != hipSuccess) return hip_fail(e, "table upload sync");
+    int cap = g->cfg.ep_cap > 0 ? g->cfg.ep_cap : round_up(4 * max_items + 64, 64);
+    // Shrink an automatic capacity until one block fits the LDS limit.
+    rc = shape(g, cap, g->cfg.blocks_per_cu, n_dec);
+    while (rc == PLP_E_EPCAP && g->cfg.ep_cap == 0 && cap > 256) {
+        cap = round_up(cap / 2, 64);
+        rc = shape(g, cap, g->cfg.blocks_per_cu, n_dec);
+    }
+    if (rc != PLP_OK) return rc;
+    g->passes_done = 0;
+    g->need_reset = true;
+    g->states_fresh = false;
+    g->loaded = true;
+    return PLP_OK;
+}
+
+extern "C" int plp_engine_reset(plp_engine* g) {
+    if (!g || !g->loaded) return fail(PLP_E_ARG, "engine not loaded");
+    g->need_reset = true;
+    g->passes_done = 0;
+    g->states_fresh = false;
+    return PLP_OK;
+}
+
+extern "C" int plp_engine_run(plp_engine* g, int32_t n_passes) {
+    if (!g || !g->loaded) return fail(PLP_E_ARG, "engine not loaded");
+    if (n_passes < 1) return fail(PLP_E_ARG, "n_passes must be >= 1");
+    hipError_t e = hipSetDevice(g->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const int reset = g->need_reset ? 1 : 0;
+    if ((e = hipEventRecord(g->ev0, g->ctx->stream)) != hipSuccess) return hip_fail(e, "event");
+    int rc = launch_run(g, n_passes, reset);
+    if (rc != PLP_OK) return rc;
+    if ((e = hipEventRecord(g->ev1, g->ctx->stream)) != hipSuccess) return hip_fail(e, "event");
+    g->passes_done = reset ? n_passes : g->passes_done + n_passes;
+    g->need_reset = false;
+    g->states_fresh = false;
+    return PLP_OK;
+}
+
+extern "C" int plp_engine_sync(plp_engine* g) {
+    if (!g || !g->loaded) return fail(PLP_E_ARG, "engine not loaded");
+    plp_ctx* c = g->ctx;
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "decode sync");
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, g->ev0, g->ev1) != hipSuccess) {
+        (void)hipGetLastError();
+        ms = 0.0f;
+    }
+    g->last_ms = ms;
+    if (g->passes_done == 0) return PLP_OK;
+    int rc = read_states(g);
+    if (rc != PLP_OK) return rc;
+    // Capacity escalation: a decoder whose extreme-point list outgrew the
+    // LDS capacity stopped with err set; rerun exactly those decoders from
+    // their seed with a doubled capacity.  Results do not depend on the
+    // capacity, so the rerun reproduces what an unlimited run would give.
+    for (;;) {
+        std::vector<int32_t> failed;
+        int need = 0;
+        for (int d = 0; d < g->n_dec; d++)
+            if (g->h_state[(size_t)d].err != 0) {
+                failed.push_back(d);
+                need = std::max(need, (int)g->h_state[(size_t)d].max_ep);
+            }
+        if (failed.empty()) return PLP_OK;
+        int cap = std::max(round_up(2 * g->ep_cap, 64), round_up(need + need / 4, 64));
+        rc = shape(g, cap, g->cfg.blocks_per_cu, g->n_dec);
+        if (rc != PLP_OK) {
+            fail(PLP_E_EPCAP, std::to_string(failed.size()) +
+                                  " decoder(s) exceeded the extreme-point capacity: " + plp_last_error());
+            return PLP_E_EPCAP;
+        }
+        drop_graph(g);
+        e = hipMemcpyAsync(g->d_retry.p, failed.data(), 4 * failed.size(), hipMemcpyHostToDevice,
+                           c->stream);
+        if (e != hipSuccess) return hip_fail(e, "H2D retry order");
+        const int grid_all = g->grid;
+        g->grid = std::max(1, std::min((int)failed.size(), grid_all));
+        plpk::DecodeArgs a = make_args(g, g->passes_done, 1, (const int32_t*)g->d_retry.p,
+                                       (int)failed.size());
+        if ((e = hipEventRecord(g->ev0, c->stream)) != hipSuccess) return hip_fail(e, "event");
+        e = plpk::launch_decode(a, g->grid, g->lds, c->stream);
+        g->grid = std::max(1, std::min(g->n_dec, (g->cfg.blocks_per_cu > 0
+                                                      ? std::min(g->cfg.blocks_per_cu, g->occ)
+                                                      : g->occ) * c->n_cu));
+        if (e != hipSuccess) return hip_fail(e, "decode retry launch");
+        if ((e = hipEventRecord(g->ev1, c->stream)) != hipSuccess) return hip_fail(e, "event");
+        if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "retry sync");
+        if (hipEventElapsedTime(&ms, g->ev0, g->ev1) == hipSuccess) g->last_ms += ms;
+        else (void)hipGetLastError();
+        if ((rc = read_states(g)) != PLP_OK) return rc;
+    }
+}
+
+extern "C" int plp_engine_max_items(plp_engine* g, int32_t* max_items) {
+    if (!g || !g->loaded || !max_items) return fail(PLP_E_ARG, "engine not loaded");
+    *max_items = g->max_items;
+    return PLP_OK;
+}
+
+extern "C" int plp_engine_fetch(plp_engine* g, plp_row_result* rows, int32_t* placements) {
+    if (!g || !g->loaded || !rows) return fail(PLP_E_ARG, "engine not loaded");
+    hipError_t e = hipSetDevice(g->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    if ((e = hipStreamSynchronize(g->ctx->stream)) != hipSuccess) return hip_fail(e, "fetch sync");
+    int rc;
+    if (!g->states_fresh && (rc = read_states(g)) != PLP_OK) return rc;
+    const size_t stride = (size_t)g->max_items * 7;
+    std::vector<int32_t> all, acc;
+    if (placements) {
+        all.resize(stride * (size_t)g->n_dec);
+        acc.resize(stride * (size_t)g->n_dec);
+        e = hipMemcpyAsync(all.data(), g->d_best_all.p, 4 * all.size(), hipMemcpyDeviceToHost,
+                           g->ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(acc.data(), g->d_best_acc.p, 4 * acc.size(), hipMemcpyDeviceToHost,
+                               g->ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(g->ctx->stream);
+        if (e != hipSuccess) return hip_fail(e, "D2H packings");
+    }
+    for (int d = 0; d < g->n_dec; d++) {
+        const DecState& s = g->h_state[(size_t)d];
+        plp_row_result& r = rows[d];
+        std::memset(&r, 0, sizeof r);
+        const bool have_acc = s.n_acc >= 0;
+        r.n_boxes = have_acc ? s.n_acc : std::max(0, s.n_all);
+        r.fallback = have_acc ? 0 : 1;
+        r.n_passes = s.passes;
+        r.err = s.err;
+        r.volume = have_acc ? s.v_acc : s.v_all;
+        r.score = have_acc ? s.s_acc : s.s_all;
+        r.seed = g->h_seed[(size_t)d];
+        r.work_pair_box = s.work_pair_box;
+        r.max_ep = (int32_t)s.max_ep;
+        if (placements) {
+            const int32_t* src = (have_acc ? acc.data() : all.data()) + stride * (size_t)d;
+            std::memcpy(placements + stride * (size_t)d, src, 4 * 7 * (size_t)r.n_boxes);
+        }
+    }
+    return PLP_OK;
+}
+
+extern "C" int plp_engine_last_ms(plp_engine* g, float* ms) {
+    if (!g || !ms) return fail(PLP_E_ARG, "NULL argument");
+    *ms = g->last_ms;
+    return PLP_OK;
+}
+
+extern "C" int plp_engine_describe(plp_engine* g, int32_t* wpd, int32_t* ep_cap, int32_t* lds,
+                                   int32_t* grid) {
+    if (!g || !g->loaded) return fail(PLP_E_ARG, "engine not loaded");
+    if (wpd) *wpd = 1;
+    if (ep_cap) *ep_cap = g->ep_cap;
+    if (lds) *lds = g->lds;
+    if (grid) *grid = g->grid;
+    return PLP_OK;
+}
+
+extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_config* best,
+                                   float* best_ms) {
+    if (!g || !g->loaded) return fail(PLP_E_ARG, "engine not loaded");
+    if (n_passes < 1) return fail(PLP_E_ARG, "n_passes must be >= 1");
+    const plp_engine_config saved = g->cfg;
+    // Probe run: learn the largest extreme-point list of this batch.
+    g->need_reset = true;
+    int rc = plp_engine_run(g, n_passes);
+    if (rc == PLP_OK) rc = plp_engine_sync(g);
+    if (rc != PLP_OK) return rc;
+    int max_ep = 1;
+    for (const DecState& s : g->h_state) max_ep = std::max(max_ep, (int)s.max_ep);
+    std::vector<int> caps;
+    for (int num : {4, 5, 6})   // 1.0x, 1.25x, 1.5x the observed peak
+        caps.push_back(std::max(64, round_up(max_ep * num / 4, 32)));
+    caps.push_back(g->ep_cap);
+    std::sort(caps.begin(), caps.end());
+    caps.erase(std::unique(caps.begin(), caps.end()), caps.end());
+    float bms = -1.0f;
+    plp_engine_config bcfg = saved;
+    for (int cap : caps) {
+        g->cfg.ep_cap = cap;
+        g->cfg.blocks_per_cu = 0;
+        if (shape(g, cap, 0, g->n_dec) != PLP_OK) continue;
+        const int occ = g->occ;
+        std::vector<int> bpcs = {occ, std::max(1, occ * 3 / 4), std::max(1, occ / 2)};
+        bpcs.erase(std::unique(bpcs.begin(), bpcs.end()), bpcs.end());
+        for (int bpc : bpcs) {
+            g->cfg.blocks_per_cu = bpc;
+            if (shape(g, cap, bpc, g->n_dec) != PLP_OK) continue;
+            drop_graph(g);
+            g->need_reset = true;
+            rc = plp_engine_run(g, n_passes);
+            if (rc == PLP_OK) rc = plp_engine_sync(g);
+            if (rc != PLP_OK) return rc;
+            if (bms < 0.0f || g->last_ms < bms) {
+                bms = g->last_ms;
+                bcfg = g->cfg;
+                bcfg.ep_cap = g->ep_cap;   // escalation may have raised it
+            }
+        }
+    }
+    g->cfg = saved;
+    g->cfg.ep_cap = bcfg.ep_cap;
+    g->cfg.blocks_per_cu = bcfg.blocks_per_cu;
+    rc = shape(g, bcfg.ep_cap, bcfg.blocks_per_cu, g->n_dec);
+    if (rc != PLP_OK) return rc;
+    drop_graph(g);
+    g->need_reset = true;
+    g->passes_done = 0;
+    g->states_fresh = false;
+    if (best) {
+        *best = g->cfg;
+        best->waves_per_decoder = 1;
+    }
+    if (best_ms) *best_ms = bms;
+    return PLP_OK;
+}

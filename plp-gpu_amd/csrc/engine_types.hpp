@@ -1,0 +1,102 @@
+// engine_types.hpp -- device-resident layouts shared by host and kernels.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+namespace plpk {
+
+// Per-instance header in device memory (16 ints).
+struct InstHdr {
+    int32_t L, W, H, n_steps;
+    int32_t step_off;   // first entry of this instance in the step-type table
+    int32_t ors_off;    // first (instance, type) row of the orientation table
+    int32_t n_types;
+    int32_t xbits;      // bit length of L: packed key = z << (xbits+ybits) | y << xbits | x
+    int32_t ybits;      // bit length of W
+    int32_t pad[7];
+};
+static_assert(sizeof(InstHdr) == 64, "InstHdr layout");
+
+// Orientation table row: n, then up to 6 (w, d, h) triples, padded to 20 ints.
+constexpr int kOrsRow = 20;
+
+// Per-decoder state kept in device memory between launches.
+struct DecState {
+    int32_t passes;          // passes completed
+    int32_t err;             // 0 or PLP_E_EPCAP
+    int32_t n_all, n_acc;    // D4' best overall / best accepted packing size; -1 = none
+    int64_t v_all, v_acc;    // their volumes
+    double s_all, s_acc;     // their stability scores
+    uint64_t work_pair_box;  // brute-force AABB tests so far (sum of n_cand * n_ors * n_box)
+    uint32_t work_steps;     // steps that evaluated feasibility
+    uint32_t max_ep;         // largest extreme-point list reached
+};
+static_assert(sizeof(DecState) == 64, "DecState layout");
+
+constexpr int kMtWords = 625;      // 624 state words + index
+
+// Extreme-point key flag: bit 31 marks a point no box can ever occupy again
+// (outside the container's far walls, or inside the half-open extent of a
+// placed box); such a point keeps its place in the sorted list but is never
+// offered as a candidate.
+constexpr uint32_t kDead = 0x80000000u;
+constexpr uint32_t kKeyMask = 0x7fffffffu;
+
+struct DecodeArgs {
+    const InstHdr* inst;
+    const int32_t* step_type;
+    const int32_t* ors_tab;
+    const int32_t* dec_inst;
+    const uint64_t* dec_seed;
+    const int32_t* work_order;   // queue position -> decoder index
+    int32_t n_work;              // queue length
+    int32_t n_passes;
+    uint32_t* mt;            // n_dec * kMtWords
+    DecState* state;         // n_dec
+    int32_t* best_all;       // n_dec * max_items * 7 (JSON row order)
+    int32_t* best_acc;
+    int32_t max_items;
+    int32_t ep_cap;
+    int32_t box_cap;
+    int32_t use_rng;
+    int32_t use_theta;
+    int32_t reset;           // 1: seed RNG and clear selection state first
+    double eps, theta, mu;
+    uint32_t* work_counter;  // persistent-grid work queue head
+};
+
+// Dynamic LDS layout of one decoder block.  The pass-time region (extreme
+// points, candidate list, pair masks) is reused by FSR after the pass.
+struct LdsLayout {
+    int32_t off_ep, off_tmp, off_act, off_mask, n_mask;
+    int32_t off_load, off_key, n_key;       // FSR view of the first region
+    int32_t off_box, off_mt, total;
+};
+
+__host__ __device__ inline int32_t next_pow2(int32_t v) {
+    int32_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+__host__ __device__ inline LdsLayout lds_layout(int32_t ep_cap, int32_t box_cap, int32_t use_rng) {
+    LdsLayout l;
+    l.off_ep = 0;
+    l.off_tmp = ep_cap * 4;
+    l.off_act = 2 * ep_cap * 4;
+    l.off_mask = (3 * ep_cap * 4 + 7) & ~7;
+    l.n_mask = (ep_cap * 6 + 63) / 64;
+    int32_t pass_end = l.off_mask + l.n_mask * 8;
+    l.n_key = next_pow2(box_cap > 0 ? box_cap : 1);
+    l.off_load = 0;
+    l.off_key = box_cap * 8;
+    int32_t fsr_end = l.off_key + l.n_key * 8;
+    int32_t region = pass_end > fsr_end ? pass_end : fsr_end;
+    l.off_box = (region + 15) & ~15;
+    l.off_mt = l.off_box + box_cap * 32;
+    l.total = l.off_mt + (use_rng ? 624 * 4 : 0);
+    return l;
+}
+
+}  // namespace plpk

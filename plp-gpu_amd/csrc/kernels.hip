@@ -1,0 +1,612 @@
+// kernels.hip -- gfx950 kernels of the pallet-loading decoder.
+//
+// Arithmetic contract (gpu/Makefile:6-10, gpu/plp_common.hpp:9-15): integer
+// geometry stays integer; the only floating point is IEEE-754 binary64 in the
+// reference's operand order (C3 ratio, random(), FSR), built with
+// -ffp-contract=off and no fast-math, so no multiply-add is fused.
+//
+// Kernels:
+//   feas_kernel        one lane per (candidate, orientation) pair of one step
+//                      (plain form, plp_gpu.cu:74-88)
+//   feas_batch_kernel  the same for K rows (plp_gpu.cu:123-144)
+//   decode_kernel      persistent: one wavefront decodes one (instance, seed)
+//                      for whole passes, then takes the next from a work queue
+//                      (plp_gpu.cu:701-1095 + stability_kernel 1121-1221 +
+//                      the D4' selection of run_device 1438-1473)
+//   fsr_kernel         FSR + stability score of given packings
+//   rng_script_kernel  device MT19937 probe (plp_gpu.cu:642-661)
+#include "kernels.hpp"
+
+#include "engine_types.hpp"
+#include "mt19937.hpp"
+#include "wave.hpp"
+
+namespace plpk {
+
+// Placed box in LDS: AABB + item type (32 bytes; the hot loop reads 24).
+struct Box {
+    int32_t x, y, z, x2, y2, z2, type, pad;
+};
+
+__device__ __forceinline__ void key_xyz(uint32_t key, int xb, int yb, int& x, int& y, int& z) {
+    x = (int)(key & ((1u << xb) - 1u));
+    y = (int)((key >> xb) & ((1u << yb) - 1u));
+    z = (int)(key >> (xb + yb));
+}
+
+__device__ __forceinline__ uint32_t xyz_key(int x, int y, int z, int xb, int yb) {
+    return ((uint32_t)z << (xb + yb)) | ((uint32_t)y << xb) | (uint32_t)x;
+}
+
+// check_feasible_one (plp_common.hpp:297-321) for this lane's pair:
+//   C2  x+w <= L, y+d <= W, z+h <= H
+//   C3  z > 0: s = sum of top-face overlaps of boxes whose top is at z;
+//       infeasible iff (double)s / (double)(w*d) < 0.5
+//   C1  no strict-interior overlap with any box
+// The verdict does not depend on the order the boxes are visited, so boxes
+// are scanned newest first (nearest the packing front, where overlaps are
+// found soonest) and the wave leaves the scan once every lane has failed
+// (ballot every 8 boxes).  Areas fit 32 bits: L*W < 2^30 is a load-time
+// precondition, and ox, oy < 2^21 so the 24-bit multiply is exact.
+__device__ __forceinline__ bool pair_feasible(int x, int y, int z, int w, int d, int h, bool valid,
+                                              const Box* __restrict__ boxes, int n_box,
+                                              int L, int W, int H) {
+    const int x2 = x + w, y2 = y + d, z2 = z + h;
+    bool alive = valid && (x2 <= L) && (y2 <= W) && (z2 <= H);
+    uint32_t s = 0;
+    int b = n_box - 1;
+    while (b >= 0) {
+        if (ballot(alive) == 0) break;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (b - u >= 0) {
+                const Box B = boxes[b - u];
+                const bool ov = (x < B.x2) & (B.x < x2) & (y < B.y2) & (B.y < y2) &
+                                (z < B.z2) & (B.z < z2);
+                alive = alive & !ov;
+                const int ox = min(x2, B.x2) - max(x, B.x);
+                const int oy = min(y2, B.y2) - max(y, B.y);
+                const bool sup = (B.z2 == z) & (ox > 0) & (oy > 0);
+                s += sup ? __umul24((uint32_t)ox, (uint32_t)oy) : 0u;
+            }
+        }
+        b -= 8;
+    }
+    if (alive && z > 0) alive = !((double)s / (double)((int64_t)w * d) < 0.5);
+    return alive;
+}
+
+// ---------------------------------------------------------------- FSR ----
+// compute_fsr + stability_score (plp_common.hpp:356-399) for the n boxes in
+// LDS, on one wave.  The reference adds load[k] * a / total into load[j]
+// while visiting k in (z desc, index asc) order; that order is produced by a
+// bitonic sort of 64-bit keys ((2^31-1-z) << 32 | index) in LDS, and the k
+// loop stays serial so every load[j] receives its additions in the
+// reference's order (bit-exact FP64).  Returns the score; per-box FSR goes to
+// fsr_out when non-null.
+__device__ double fsr_score(const Box* __restrict__ box, int n, double mu, double* load,
+                            uint64_t* skey, int n_key, double* fsr_out) {
+    const int lane = lane_id();
+    if (n <= 0) return 0.0;
+    const int n2 = next_pow2(n);
+    for (int i = lane; i < n2; i += kWave)
+        skey[i] = i < n ? (((uint64_t)(0x7fffffffu - (uint32_t)box[i].z) << 32) | (uint32_t)i)
+                        : ~0ull;
+    for (int i = lane; i < n; i += kWave) {
+        const int64_t v = (int64_t)(box[i].x2 - box[i].x) * (box[i].y2 - box[i].y) *
+                          (box[i].z2 - box[i].z);
+        load[i] = 50.0 * (double)v / 1.0e6;   // m = RHO * V / 1e6
+    }
+    block_sync();
+    // Bitonic sort, ascending.
+    for (int k = 2; k <= n2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = lane; i < n2; i += kWave) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const uint64_t p = skey[i], q = skey[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((p > q) == up) {
+                        skey[i] = q;
+                        skey[ixj] = p;
+                    }
+                }
+            }
+            block_sync();
+        }
+    }
+    (void)n_key;
+    for (int t = 0; t < n; t++) {
+        const int k = (int)(uint32_t)uni(skey[t]);
+        const Box Bk = box[k];
+        const int bkz = uni(Bk.z);
+        if (bkz == 0) break;   // the rest are floor boxes: nothing supports them
+        const int bkx = uni(Bk.x), bky = uni(Bk.y), bkx2 = uni(Bk.x2), bky2 = uni(Bk.y2);
+        uint64_t part = 0;
+        for (int j = lane; j < n; j += kWave) {
+            const Box Bj = box[j];
+            if (j != k && Bj.z2 == bkz) {
+                const int ox = min(bkx2, Bj.x2) - max(bkx, Bj.x);
+                const int oy = min(bky2, Bj.y2) - max(bky, Bj.y);
+                if (ox > 0 && oy > 0) part += (uint64_t)ox * (uint64_t)oy;
+            }
+        }
+        const uint64_t total = wave_sum_u64(part);
+        if (total > 0) {
+            const double lk = load[k];
+            for (int j = lane; j < n; j += kWave) {
+                const Box Bj = box[j];
+                if (j != k && Bj.z2 == bkz) {
+                    const int ox = min(bkx2, Bj.x2) - max(bkx, Bj.x);
+                    const int oy = min(bky2, Bj.y2) - max(bky, Bj.y);
+                    if (ox > 0 && oy > 0) {
+                        const uint64_t a = (uint64_t)ox * (uint64_t)oy;
+                        load[j] = load[j] + lk * (double)a / (double)total;
+                    }
+                }
+            }
+        }
+        block_sync();
+    }
+    uint32_t cnt = 0;
+    for (int i = lane; i < n; i += kWave) {
+        const int64_t v = (int64_t)(box[i].x2 - box[i].x) * (box[i].y2 - box[i].y) *
+                          (box[i].z2 - box[i].z);
+        const double m = 50.0 * (double)v / 1.0e6;
+        const double f = (mu / 0.3) * load[i] / m;
+        if (fsr_out) fsr_out[i] = f;
+        cnt += (f >= 1.0) ? 1u : 0u;
+    }
+    const uint64_t c = wave_sum_u64(cnt);
+    block_sync();
+    return (double)c / (double)n;
+}
+
+// ------------------------------------------------------------ decoder ----
+
+struct PassOut {
+    int n_box;
+    int err;
+    int max_ep;
+    uint64_t work;
+    uint64_t steps;
+};
+
+// One decode pass (plp_common.hpp:404-456) by one wave.  Extreme points are
+// packed (z, y, x) keys, sorted and unique in ep[] (flag kDead on points no
+// pair can ever use); act[] holds the live ones, so a pair index
+// p = e * n_ors + o enumerates exactly the reference's candidate-major feas
+// order restricted to points that can be feasible.  Each 64-pair chunk's
+// verdicts become one ballot mask in LDS; the pick is the idx-th set bit.
+__device__ PassOut decode_pass_wave(const DecodeArgs& a, const InstHdr& I, uint32_t* ep,
+                                    uint32_t* tmp, uint32_t* act, uint64_t* mask, Box* box,
+                                    uint32_t* mt, int& mti, int* s_ors) {
+    const int lane = lane_id();
+    const int L = I.L, W = I.W, H = I.H, xb = I.xbits, yb = I.ybits;
+    PassOut out = {0, 0, 1, 0, 0};
+    int n_ep = 1, n_act = 1, n_box = 0;
+    if (lane == 0) {
+        ep[0] = 0u;    // origin; L, W, H > 0 so it is live
+        act[0] = 0u;
+    }
+    block_sync();
+    const int32_t* steps = a.step_type + I.step_off;
+    for (int s = 0; s < I.n_steps; s++) {
+        const int ti = uni(steps[s]);
+        const int32_t* orow = a.ors_tab + (size_t)(I.ors_off + ti) * kOrsRow;
+        const int no = uni(orow[0]);
+        if (no == 0) continue;     // no allowed orientation: skipped, no RNG use
+        if (lane < 3 * no) s_ors[lane] = orow[1 + lane];
+        block_sync();
+        out.work += (uint64_t)n_ep * (uint64_t)no * (uint64_t)n_box;
+        out.steps++;
+        const int n_pairs = n_act * no;
+        const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)no - 1) / (uint64_t)no);
+        const int n_chunks = (n_pairs + kWave - 1) / kWave;
+        int cnt = 0;
+        for (int c = 0; c < n_chunks; c++) {
+            const int p = c * kWave + lane;
+            const bool valid = p < n_pairs;
+            const int e = no == 1 ? p : (int)__umulhi((uint32_t)p, magic);
+            const int o = p - e * no;
+            int x = 0, y = 0, z = 0, w = 1, d = 1, h = 1;
+            if (valid) {
+                key_xyz(act[e], xb, yb, x, y, z);
+                w = s_ors[3 * o];
+                d = s_ors[3 * o + 1];
+                h = s_ors[3 * o + 2];
+            }
+            const bool f = pair_feasible(x, y, z, w, d, h, valid, box, n_box, L, W, H);
+            const uint64_t m = ballot(f);
+            if (lane == 0) mask[c] = m;
+            cnt += __popcll(m);
+        }
+        cnt = uni(cnt);
+        if (cnt == 0) continue;    // this copy is skipped; no RNG use
+        block_sync();
+        // feas[0], or with eps-greedy feas[randrange(|feas|)] when
+        // random() >= 1 - eps (plp_common.hpp:427-435).
+        int idx = 0;
+        if (a.use_rng) {
+            if (mt_random(mt, mti) >= 1.0 - a.eps) idx = mt_randrange(mt, mti, cnt);
+        }
+        int c = 0;
+        uint64_t m = uni(mask[0]);
+        for (;;) {
+            const int pc = __popcll(m);
+            if (idx < pc) break;
+            idx -= pc;
+            c++;
+            m = uni(mask[c]);
+        }
+        const bool hit = ((m >> lane) & 1ull) && rank_below(m) == idx;
+        const int bit = __ffsll((unsigned long long)ballot(hit)) - 1;
+        const int p = c * kWave + bit;
+        const int e = p / no, o = p - (p / no) * no;
+        int x, y, z;
+        key_xyz(uni(act[e]), xb, yb, x, y, z);
+        x = uni(x);
+        y = uni(y);
+        z = uni(z);
+        const int w = uni(s_ors[3 * o]), d = uni(s_ors[3 * o + 1]), h = uni(s_ors[3 * o + 2]);
+        const int X2 = x + w, Y2 = y + d, Z2 = z + h;
+        if (lane == 0) {
+            Box nb;
+            nb.x = x; nb.y = y; nb.z = z; nb.x2 = X2; nb.y2 = Y2; nb.z2 = Z2;
+            nb.type = ti; nb.pad = 0;
+            box[n_box] = nb;
+        }
+        n_box++;
+        block_sync();
+
+        // Extreme-point update (plp_common.hpp:439-453): drop points strictly
+        // inside the new box (mark the ones in its half-open extent dead),
+        // then merge the 8 corners that are not already present.
+        int base = 0;
+        for (int i0 = 0; i0 < n_ep; i0 += kWave) {
+            const int i = i0 + lane;
+            const bool valid = i < n_ep;
+            const uint32_t v = valid ? ep[i] : 0u;
+            int px, py, pz;
+            key_xyz(v & kKeyMask, xb, yb, px, py, pz);
+            const bool strict = (x < px) & (px < X2) & (y < py) & (py < Y2) & (z < pz) & (pz < Z2);
+            const bool half = (x <= px) & (px < X2) & (y <= py) & (py < Y2) & (z <= pz) & (pz < Z2);
+            const bool keep = valid && !strict;
+            const uint64_t km = ballot(keep);
+            if (keep) tmp[base + rank_below(km)] = half ? (v | kDead) : v;
+            base += __popcll(km);
+        }
+        const int n_surv = uni(base);
+        // Corner j (lanes 0..7): bit 0 -> x side, bit 1 -> y side, bit 2 -> z side.
+        const int cx = (lane & 1) ? X2 : x;
+        const int cy = (lane & 2) ? Y2 : y;
+        const int cz = (lane & 4) ? Z2 : z;
+        const uint32_t ck = xyz_key(cx, cy, cz, xb, yb);
+        int jx[8], jy[8], jz[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            jx[j] = (j & 1) ? X2 : x;
+            jy[j] = (j & 2) ? Y2 : y;
+            jz[j] = (j & 4) ? Z2 : z;
+        }
+        uint32_t dead_bits = 0;
+        for (int b0 = 0; b0 < n_box; b0 += kWave) {
+            const int bi = b0 + lane;
+            const bool bv = bi < n_box;
+            Box B;
+            if (bv) B = box[bi];
+            else { B.x = B.y = B.z = 1; B.x2 = B.y2 = B.z2 = 0; }
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const bool in = (B.x <= jx[j]) & (jx[j] < B.x2) & (B.y <= jy[j]) & (jy[j] < B.y2) &
+                                (B.z <= jz[j]) & (jz[j] < B.z2);
+                if (ballot(in)) dead_bits |= 1u << j;
+            }
+        }
+        block_sync();
+        bool cdead = (cx >= L) | (cy >= W) | (cz >= H) | (((dead_bits >> (lane & 7)) & 1u) != 0);
+        int lo = 0;
+        bool kept = false;
+        if (lane < 8) {
+            int hi = n_surv;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if ((tmp[mid] & kKeyMask) < ck) lo = mid + 1;
+                else hi = mid;
+            }
+            kept = !(lo < n_surv && (tmp[lo] & kKeyMask) == ck);
+        }
+        const uint64_t kmask = ballot(kept);
+        const int n_kept = __popcll(kmask);
+        if (n_surv + n_kept > a.ep_cap) {
+            out.err = 4;   // PLP_E_EPCAP
+            out.max_ep = n_surv + n_kept;
+            out.n_box = n_box;
+            return out;
+        }
+        uint32_t kk[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) kk[j] = ((kmask >> j) & 1ull) ? read_lane(ck, j) : 0xffffffffu;
+        if (kept) {
+            int r = 0;
+#pragma unroll
+            for (int j = 0; j < 8; j++) r += (kk[j] < ck) ? 1 : 0;
+            ep[lo + r] = ck | (cdead ? kDead : 0u);
+        }
+        for (int i = lane; i < n_surv; i += kWave) {
+            const uint32_t v = tmp[i];
+            const uint32_t k = v & kKeyMask;
+            int r = 0;
+#pragma unroll
+            for (int j = 0; j < 8; j++) r += (kk[j] < k) ? 1 : 0;
+            ep[i + r] = v;
+        }
+        n_ep = n_surv + n_kept;
+        out.max_ep = max(out.max_ep, n_ep);
+        block_sync();
+        base = 0;
+        for (int i0 = 0; i0 < n_ep; i0 += kWave) {
+            const int i = i0 + lane;
+            const uint32_t v = i < n_ep ? ep[i] : kDead;
+            const bool live = (v & kDead) == 0u;
+            const uint64_t lm = ballot(live);
+            if (live) act[base + rank_below(lm)] = v;
+            base += __popcll(lm);
+        }
+        n_act = uni(base);
+        block_sync();
+    }
+    out.n_box = n_box;
+    return out;
+}
+
+__device__ __forceinline__ void write_rows(int32_t* dst, const Box* box, int n) {
+    for (int t = lane_id(); t < 7 * n; t += kWave) {
+        const int i = t / 7, c = t - (t / 7) * 7;
+        const Box& B = box[i];
+        int v;
+        switch (c) {
+            case 0: v = B.type; break;
+            case 1: v = B.x; break;
+            case 2: v = B.y; break;
+            case 3: v = B.z; break;
+            case 4: v = B.x2 - B.x; break;
+            case 5: v = B.y2 - B.y; break;
+            default: v = B.z2 - B.z; break;
+        }
+        dst[t] = v;
+    }
+}
+
+__global__ void __launch_bounds__(64) decode_kernel(DecodeArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int s_ors[18];
+    const LdsLayout lay = lds_layout(a.ep_cap, a.box_cap, a.use_rng);
+    uint32_t* ep = (uint32_t*)(lds + lay.off_ep);
+    uint32_t* tmp = (uint32_t*)(lds + lay.off_tmp);
+    uint32_t* act = (uint32_t*)(lds + lay.off_act);
+    uint64_t* mask = (uint64_t*)(lds + lay.off_mask);
+    double* load = (double*)(lds + lay.off_load);
+    uint64_t* skey = (uint64_t*)(lds + lay.off_key);
+    Box* box = (Box*)(lds + lay.off_box);
+    uint32_t* mt = (uint32_t*)(lds + lay.off_mt);
+    const int lane = lane_id();
+
+    for (;;) {
+        int item = 0;
+        if (lane == 0) item = (int)atomicAdd(a.work_counter, 1u);
+        item = read_lane(item, 0);
+        if (item >= a.n_work) break;
+        const int d = uni(a.work_order[item]);
+        const InstHdr I = a.inst[uni(a.dec_inst[d])];
+        DecState st;
+        if (a.reset) {
+            st.passes = 0; st.err = 0; st.n_all = -1; st.n_acc = -1;
+            st.v_all = 0; st.v_acc = 0; st.s_all = 0.0; st.s_acc = 0.0;
+            st.work_pair_box = 0; st.work_steps = 0; st.max_ep = 0;
+        } else {
+            st = a.state[d];
+            st.passes = uni(st.passes); st.err = uni(st.err);
+            st.n_all = uni(st.n_all); st.n_acc = uni(st.n_acc);
+            st.v_all = uni(st.v_all); st.v_acc = uni(st.v_acc);
+            st.work_pair_box = uni(st.work_pair_box); st.work_steps = uni(st.work_steps);
+            st.max_ep = uni(st.max_ep);
+        }
+        if (st.err != 0) continue;
+        int mti = 624;
+        uint32_t* gmt = a.mt + (size_t)d * kMtWords;
+        if (a.use_rng) {
+            if (a.reset) {
+                mt_seed_lane0(mt, a.dec_seed[d]);
+            } else {
+                for (int i = lane; i < 624; i += kWave) mt[i] = gmt[i];
+                mti = uni((int)gmt[624]);
+            }
+            block_sync();
+        }
+        int32_t* out_all = a.best_all + (size_t)d * a.max_items * 7;
+        int32_t* out_acc = a.best_acc + (size_t)d * a.max_items * 7;
+        for (int pass = 0; pass < a.n_passes; pass++) {
+            PassOut po = decode_pass_wave(a, I, (uint32_t*)(lds + lay.off_ep), tmp, act, mask,
+                                          box, mt, mti, s_ors);
+            st.work_pair_box += po.work;
+            st.work_steps += (uint32_t)po.steps;
+            st.max_ep = max(st.max_ep, (uint32_t)po.max_ep);
+            if (po.err) {
+                st.err = po.err;
+                break;
+            }
+            const int n = po.n_box;
+            int64_t vpart = 0;
+            for (int i = lane; i < n; i += kWave)
+                vpart += (int64_t)(box[i].x2 - box[i].x) * (box[i].y2 - box[i].y) *
+                         (box[i].z2 - box[i].z);
+            const int64_t vol = wave_sum_i64(vpart);
+            const double score = fsr_score(box, n, a.mu, load, skey, lay.n_key, nullptr);
+            // D4' (plp_common.hpp:493-511), kept incrementally: the best of
+            // all packings and the best accepted one, strict > on
+            // (volume, n_boxes) so the earliest winner stays.
+            if (st.n_all < 0 || vol > st.v_all || (vol == st.v_all && n > st.n_all)) {
+                st.n_all = n; st.v_all = vol; st.s_all = score;
+                write_rows(out_all, box, n);
+            }
+            const bool accepted = !a.use_theta || score >= a.theta;
+            if (accepted && (st.n_acc < 0 || vol > st.v_acc || (vol == st.v_acc && n > st.n_acc))) {
+                st.n_acc = n; st.v_acc = vol; st.s_acc = score;
+                write_rows(out_acc, box, n);
+            }
+            st.passes++;
+            block_sync();
+        }
+        if (a.use_rng) {
+            for (int i = lane; i < 624; i += kWave) gmt[i] = mt[i];
+            if (lane == 0) gmt[624] = (uint32_t)mti;
+        }
+        if (lane == 0) a.state[d] = st;
+        block_sync();
+    }
+}
+
+// -------------------------------------------------------- plain forms ----
+
+// One lane per (candidate, orientation) pair of one row; the row's boxes are
+// staged in LDS in tiles of kFeasTile.  bits[t] with t = ci * n_ors + oi
+// (plp_gpu.cpp:23-29 order).
+constexpr int kFeasTile = 1024;
+constexpr int kFeasBlock = 256;
+
+__global__ void __launch_bounds__(kFeasBlock) feas_kernel(FeasRowArgs r) {
+    __shared__ Box sbox[kFeasTile];
+    const int row = r.block_row[blockIdx.x];
+    const int first_block = r.row_block0[row];
+    const int n_cand = r.n_cand[row];
+    const int n_box = r.n_box[row];
+    const int n_ors = r.n_ors;
+    const int n_pairs = n_cand * n_ors;
+    const int p = (blockIdx.x - first_block) * kFeasBlock + threadIdx.x;
+    const bool valid = p < n_pairs;
+    const int ci = valid ? p / n_ors : 0;
+    const int oi = valid ? p - ci * n_ors : 0;
+    const int32_t* cand = r.cand + (size_t)r.cand_off[row] * 3;
+    const int32_t* aabb = r.aabb + (size_t)r.box_off[row] * 6;
+    int x = 0, y = 0, z = 0, w = 1, d = 1, h = 1;
+    if (valid) {
+        x = cand[3 * ci]; y = cand[3 * ci + 1]; z = cand[3 * ci + 2];
+        w = r.ors[3 * oi]; d = r.ors[3 * oi + 1]; h = r.ors[3 * oi + 2];
+    }
+    const int x2 = x + w, y2 = y + d, z2 = z + h;
+    bool alive = valid && x2 <= r.L && y2 <= r.W && z2 <= r.H;
+    uint32_t s = 0;
+    for (int t0 = 0; t0 < n_box; t0 += kFeasTile) {
+        const int nt = min(kFeasTile, n_box - t0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < nt; i += kFeasBlock) {
+            const int32_t* q = aabb + (size_t)(t0 + i) * 6;
+            Box B;
+            B.x = q[0]; B.y = q[1]; B.z = q[2]; B.x2 = q[3]; B.y2 = q[4]; B.z2 = q[5];
+            B.type = 0; B.pad = 0;
+            sbox[i] = B;
+        }
+        __syncthreads();
+        for (int b = 0; b < nt; b++) {
+            if ((b & 7) == 0 && ballot(alive) == 0) break;
+            const Box B = sbox[b];
+            const bool ov = (x < B.x2) & (B.x < x2) & (y < B.y2) & (B.y < y2) & (z < B.z2) & (B.z < z2);
+            alive = alive & !ov;
+            const int ox = min(x2, B.x2) - max(x, B.x);
+            const int oy = min(y2, B.y2) - max(y, B.y);
+            const bool sup = (B.z2 == z) & (ox > 0) & (oy > 0);
+            s += sup ? __umul24((uint32_t)ox, (uint32_t)oy) : 0u;
+        }
+    }
+    if (alive && z > 0) alive = !((double)s / (double)((int64_t)w * d) < 0.5);
+    if (valid) r.bits[(size_t)r.bits_off[row] + p] = alive ? 1 : 0;
+}
+
+// FSR of given packings: one wave per packing.
+__global__ void __launch_bounds__(64) fsr_kernel(FsrArgs f) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int pk = blockIdx.x;
+    const int r0 = f.pack_off[pk], n = f.pack_off[pk + 1] - r0;
+    const LdsLayout lay = lds_layout(0, f.box_cap, 0);
+    double* load = (double*)(lds + lay.off_load);
+    uint64_t* skey = (uint64_t*)(lds + lay.off_key);
+    Box* box = (Box*)(lds + lay.off_box);
+    for (int i = lane_id(); i < n; i += kWave) {
+        const int32_t* q = f.rows7 + (size_t)(r0 + i) * 7;
+        Box B;
+        B.type = q[0]; B.x = q[1]; B.y = q[2]; B.z = q[3];
+        B.x2 = q[1] + q[4]; B.y2 = q[2] + q[5]; B.z2 = q[3] + q[6]; B.pad = 0;
+        box[i] = B;
+    }
+    block_sync();
+    const double sc = fsr_score(box, n, f.mu, load, skey, lay.n_key,
+                                f.fsr_out ? f.fsr_out + r0 : nullptr);
+    if (lane_id() == 0) f.score_out[pk] = sc;
+}
+
+// Device MT19937 probe: one wave per seed runs the op script.
+__global__ void __launch_bounds__(64) rng_script_kernel(RngArgs g) {
+    __shared__ uint32_t mt[624];
+    const int sd = blockIdx.x;
+    mt_seed_lane0(mt, g.seeds[sd]);
+    block_sync();
+    int mti = 624;
+    for (int i = 0; i < g.n_ops; i++) {
+        const int op = g.ops[i], arg = g.args[i];
+        double f = 0.0;
+        uint32_t u = 0;
+        if (op == 0) f = mt_random(mt, mti);
+        else if (op == 1) u = mt_getrandbits(mt, mti, arg);
+        else u = (uint32_t)mt_randrange(mt, mti, arg);
+        if (lane_id() == 0) {
+            g.out_f64[(size_t)sd * g.n_ops + i] = f;
+            g.out_u32[(size_t)sd * g.n_ops + i] = u;
+        }
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------ launchers --
+
+hipError_t launch_decode(const DecodeArgs& a, int grid, int lds_bytes, hipStream_t st) {
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, a.work_counter, 0u, 1);
+    hipLaunchKernelGGL(decode_kernel, dim3(grid), dim3(64), (size_t)lds_bytes, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_feas(const FeasRowArgs& r, int n_blocks, hipStream_t st) {
+    hipLaunchKernelGGL(feas_kernel, dim3(n_blocks), dim3(kFeasBlock), 0, st, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_fsr(const FsrArgs& f, int n_packs, int lds_bytes, hipStream_t st) {
+    hipLaunchKernelGGL(fsr_kernel, dim3(n_packs), dim3(64), (size_t)lds_bytes, st, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_rng_script(const RngArgs& g, int n_seeds, hipStream_t st) {
+    hipLaunchKernelGGL(rng_script_kernel, dim3(n_seeds), dim3(64), 0, st, g);
+    return hipGetLastError();
+}
+
+hipError_t decode_occupancy(int lds_bytes, int* blocks_per_cu) {
+    hipError_t e = hipFuncSetAttribute((const void*)decode_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, decode_kernel, 64,
+                                                        (size_t)lds_bytes);
+}
+
+hipError_t fsr_prepare(int lds_bytes) {
+    return hipFuncSetAttribute((const void*)fsr_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+}
+
+int feas_block_size() { return kFeasBlock; }
+
+}  // namespace plpk

@@ -1,0 +1,172 @@
+// mt19937.hpp -- CPython random.Random(int) on one wavefront.
+//
+// Contract (gpu/plp_common.hpp:164-250, CPython Modules/_randommodule.c):
+//   seeding   init_genrand(19650218) then init_by_array(key), key = the
+//             little-endian 32-bit chunks of the seed, mt[0] = 0x80000000,
+//             index 624 (plp_common.hpp:170-232);
+//   next()    regenerate the 624-word block when exhausted, then temper
+//             (plp_common.hpp:234-249);
+//   random()  (a * 2^26 + b) * 2^-53 with a = next() >> 5, b = next() >> 6
+//             (plp_common.hpp:182-186);
+//   getrandbits(k) = next() >> (32 - k), 1 <= k <= 32;
+//   randrange(n)   k = bit_length(n), redraw getrandbits(k) while >= n
+//             (plp_common.hpp:189-194).
+//
+// The 624-word state lives in LDS.  The block regeneration is split into the
+// four ranges where the recurrence has no intra-range dependence:
+//   [0,227)    reads old mt[i], mt[i+1], mt[i+397]
+//   [227,454)  reads new mt[i-227]
+//   [454,623)  reads new mt[i-227]
+//   623        reads new mt[0] and mt[396]
+// so 64 lanes regenerate a block in 4 load/store rounds instead of 624 serial
+// steps.  Every lane computes identical draws (broadcast LDS reads), so the
+// RNG's data-dependent branches (randrange rejection) stay wave-uniform.
+#pragma once
+
+#include "wave.hpp"
+
+namespace plpk {
+
+constexpr uint32_t kMtUpper = 0x80000000u;
+constexpr uint32_t kMtLower = 0x7fffffffu;
+constexpr uint32_t kMtMatrix = 0x9908b0dfu;
+
+__device__ __forceinline__ uint32_t mt_mix(uint32_t cur, uint32_t nxt, uint32_t far) {
+    uint32_t y = (cur & kMtUpper) | (nxt & kMtLower);
+    return far ^ (y >> 1) ^ ((y & 1u) ? kMtMatrix : 0u);
+}
+
+// Serial CPython seeding by lane 0 (the recurrence is inherently serial);
+// the previous word is carried in a register so LDS reads stay off the
+// dependence chain.  Caller syncs afterwards.
+__device__ inline void mt_seed_lane0(uint32_t* mt, uint64_t seed) {
+    if (lane_id() != 0) return;
+    uint32_t key[2];
+    int keyused;
+    if (seed == 0) {
+        key[0] = 0u;
+        key[1] = 0u;
+        keyused = 1;
+    } else {
+        key[0] = (uint32_t)seed;
+        key[1] = (uint32_t)(seed >> 32);
+        keyused = key[1] ? 2 : 1;
+    }
+    uint32_t prev = 19650218u;
+    mt[0] = prev;
+    for (int i = 1; i < 624; i++) {
+        prev = 1812433253u * (prev ^ (prev >> 30)) + (uint32_t)i;
+        mt[i] = prev;
+    }
+    prev = 19650218u;   // mt[i - 1] for i = 1
+    int i = 1, j = 0;
+    for (int k = 624; k; k--) {
+        prev = (mt[i] ^ ((prev ^ (prev >> 30)) * 1664525u)) + key[j] + (uint32_t)j;
+        mt[i] = prev;
+        i++;
+        j++;
+        if (i >= 624) {
+            mt[0] = prev;
+            i = 1;
+        }
+        if (j >= keyused) j = 0;
+    }
+    for (int k = 623; k; k--) {
+        prev = (mt[i] ^ ((prev ^ (prev >> 30)) * 1566083941u)) - (uint32_t)i;
+        mt[i] = prev;
+        i++;
+        if (i >= 624) {
+            mt[0] = prev;
+            i = 1;
+        }
+    }
+    mt[0] = 0x80000000u;
+}
+
+// Regenerate the block: all lanes of a one-wave block.
+__device__ inline void mt_twist(uint32_t* mt) {
+    const int lane = lane_id();
+    uint32_t v[4];
+    // [0, 227): every input is an old word.
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        int i = lane + r * kWave;
+        if (i < 227) v[r] = mt_mix(mt[i], mt[i + 1], mt[i + 397]);
+    }
+    block_sync();
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        int i = lane + r * kWave;
+        if (i < 227) mt[i] = v[r];
+    }
+    block_sync();
+    // [227, 454): mt[i-227] is new.
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        int i = 227 + lane + r * kWave;
+        if (i < 454) v[r] = mt_mix(mt[i], mt[i + 1], mt[i - 227]);
+    }
+    block_sync();
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        int i = 227 + lane + r * kWave;
+        if (i < 454) mt[i] = v[r];
+    }
+    block_sync();
+    // [454, 623): mt[i-227] is new.
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        int i = 454 + lane + r * kWave;
+        if (i < 623) v[r] = mt_mix(mt[i], mt[i + 1], mt[i - 227]);
+    }
+    block_sync();
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        int i = 454 + lane + r * kWave;
+        if (i < 623) mt[i] = v[r];
+    }
+    block_sync();
+    // 623: wraps to the new mt[0].
+    uint32_t last = mt_mix(mt[623], mt[0], mt[396]);
+    block_sync();
+    if (lane == 0) mt[623] = last;
+    block_sync();
+}
+
+__device__ __forceinline__ uint32_t mt_temper(uint32_t y) {
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+// One 32-bit output; mti is wave-uniform state held by the caller.
+__device__ __forceinline__ uint32_t mt_next(uint32_t* mt, int& mti) {
+    if (mti >= 624) {
+        mt_twist(mt);
+        mti = 0;
+    }
+    uint32_t y = uni(mt[mti]);
+    mti++;
+    return mt_temper(y);
+}
+
+__device__ __forceinline__ double mt_random(uint32_t* mt, int& mti) {
+    uint32_t a = mt_next(mt, mti) >> 5;
+    uint32_t b = mt_next(mt, mti) >> 6;
+    return ((double)a * 67108864.0 + (double)b) * (1.0 / 9007199254740992.0);
+}
+
+__device__ __forceinline__ uint32_t mt_getrandbits(uint32_t* mt, int& mti, int k) {
+    return mt_next(mt, mti) >> (32 - k);
+}
+
+__device__ __forceinline__ int mt_randrange(uint32_t* mt, int& mti, int n) {
+    int k = 32 - __clz((unsigned)n);
+    int r = (int)mt_getrandbits(mt, mti, k);
+    while (r >= n) r = (int)mt_getrandbits(mt, mti, k);
+    return r;
+}
+
+}  // namespace plpk

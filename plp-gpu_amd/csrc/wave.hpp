@@ -1,0 +1,61 @@
+// wave.hpp -- wave64 primitives for gfx950 (CDNA4).
+//
+// Every helper here is called by all 64 lanes of a wavefront with uniform
+// control flow; results that are the same in every lane are moved to SGPRs
+// with readfirstlane so loop bounds and branches stay scalar.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace plpk {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// 64-bit lane mask of lanes where p holds.
+__device__ __forceinline__ uint64_t ballot(bool p) { return (uint64_t)__ballot(p ? 1 : 0); }
+
+// Number of set bits of m below this lane (mbcnt).
+__device__ __forceinline__ int rank_below(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t uni(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+__device__ __forceinline__ uint64_t uni(uint64_t v) {
+    uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ int64_t uni(int64_t v) { return (int64_t)uni((uint64_t)v); }
+
+// Value of v in lane src (src uniform).
+__device__ __forceinline__ int read_lane(int v, int src) {
+    return __builtin_amdgcn_readlane(v, src);
+}
+__device__ __forceinline__ uint32_t read_lane(uint32_t v, int src) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, src);
+}
+
+// Wave-wide integer sums (exact; order-independent).
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return uni(v);
+}
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return uni(v);
+}
+
+// LDS ordering point between phases.  Blocks here are one or a few waves;
+// the barrier also orders this wave's own LDS traffic across lanes.
+__device__ __forceinline__ void block_sync() { __syncthreads(); }
+
+}  // namespace plpk

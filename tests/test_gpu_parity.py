@@ -1,0 +1,315 @@
+"""HIP path vs the oracle and the reference's fixtures, through the C ABI.
+
+Bit-exact throughout: placements, volumes, n_boxes, fallback and every FP64
+value (stability scores, per-box FSR, random()) are compared as exact values.
+"""
+import json
+import random
+import struct
+
+import numpy as np
+import pytest
+
+from conftest import golden_path
+from oracle_lib import Oracle, load_text
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def plp():
+    import plp_amd
+    assert plp_amd.device_count() >= 1, "no HIP device"
+    return plp_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(plp):
+    c = plp.Ctx(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle()
+
+
+def hexd(x):
+    return format(struct.unpack("<Q", struct.pack("<d", x))[0], "016x")
+
+
+def inst_from(plp, rel):
+    return plp.Instance.from_text(load_text(rel))
+
+
+def prm(plp, a):
+    return plp.params(eps=a["eps"], theta=0.9 if a["theta"] is None else a["theta"], mu=a["mu"],
+                      use_theta=a["theta"] is not None, first_fit=a["first_fit"], order=a["order"])
+
+
+def row_fields(plp, row, pl):
+    d = plp.to_json_dict(row, pl)
+    return d
+
+
+def ref_fields(res):
+    return {k: res[k] for k in ("n_boxes", "volume_cm3", "stability_score", "fallback",
+                                "n_passes", "seed", "placements")}
+
+
+# ------------------------------------------------------------- RNG --------
+
+def test_device_rng_selftest_protocol(ctx, rng_vectors):
+    for blk in rng_vectors["selftest"]:
+        ops = [0] * 16 + [1] * 16
+        args = [0] * 16 + [13] * 16
+        f, u = ctx.rng_script([blk["seed"]], ops, args)
+        assert [hexd(v) for v in f[0, :16]] == blk["random_hex"]
+        assert list(map(int, u[0, 16:])) == blk["getrandbits13"]
+
+
+def test_device_rng_scripts(ctx, rng_vectors):
+    for sc in rng_vectors["scripts"]:
+        f, u = ctx.rng_script([sc["seed"]], sc["ops"], sc["args"])
+        for i, (op, want) in enumerate(zip(sc["ops"], sc["vals"])):
+            got = hexd(f[0, i]) if op == 0 else int(u[0, i])
+            assert got == want, (sc["seed"], i, op)
+
+
+def test_device_rng_many_seeds_vs_oracle(ctx, orc):
+    seeds = list(range(0, 300)) + [2**32 + 7, 2**45 + 1]
+    ops = [0] * 400 + [2] * 300 + [1] * 50
+    rr = random.Random(1)
+    args = [0] * 400 + [rr.randint(1, 5000) for _ in range(300)] + [rr.randint(1, 32) for _ in range(50)]
+    f, u = ctx.rng_script(seeds, ops, args)
+    for si, s in enumerate(seeds):
+        r = orc.rng(s)
+        for i, (op, arg) in enumerate(zip(ops, args)):
+            if op == 0:
+                assert f[si, i] == orc.L.orc_rng_random(r)
+            elif op == 1:
+                assert u[si, i] == orc.L.orc_rng_getrandbits(r, arg)
+            else:
+                assert u[si, i] == orc.L.orc_rng_randrange(r, arg)
+
+
+# ------------------------------------------------------- feasibility ------
+
+def _random_state(rnd, L, W, H, n_box, n_cand):
+    boxes = []
+    tries = 0
+    while len(boxes) < n_box and tries < n_box * 50:
+        tries += 1
+        w, d, h = rnd.randint(5, 60), rnd.randint(5, 60), rnd.randint(5, 60)
+        x, y, z = rnd.randint(0, L - w), rnd.randint(0, W - d), rnd.randint(0, H - h)
+        c = (x, y, z, x + w, y + d, z + h)
+        if all(not (c[0] < b[3] and b[0] < c[3] and c[1] < b[4] and b[1] < c[4] and c[2] < b[5]
+                    and b[2] < c[5]) for b in boxes):
+            boxes.append(c)
+    pts = set()
+    for b in boxes:
+        for cx in (b[0], b[3]):
+            for cy in (b[1], b[4]):
+                for cz in (b[2], b[5]):
+                    pts.add((cx, cy, cz))
+    pts = sorted(pts, key=lambda p: (p[2], p[1], p[0]))[:n_cand] or [(0, 0, 0)]
+    return pts, boxes
+
+
+@pytest.mark.parametrize("n_box,n_cand", [(0, 1), (1, 8), (30, 200), (200, 1500), (1500, 3000)])
+def test_feas_check_all_vs_oracle(ctx, orc, n_box, n_cand):
+    rnd = random.Random(n_box * 7 + n_cand)
+    L, W, H = 587, 233, 220
+    pts, boxes = _random_state(rnd, L, W, H, n_box, n_cand)
+    for ors in ([(10, 20, 30)], [(40, 30, 20), (30, 40, 20), (40, 20, 30), (20, 40, 30),
+                                 (30, 20, 40), (20, 30, 40)], [(1, 1, 1), (587, 233, 220)]):
+        got = ctx.feas_check_all(pts, ors, boxes, L, W, H)
+        want = orc.check_all(pts, ors, boxes, L, W, H)
+        assert np.array_equal(got, want)
+
+
+def test_feas_check_batch_vs_rows(ctx, orc):
+    rnd = random.Random(99)
+    L, W, H = 587, 233, 220
+    rows = [_random_state(rnd, L, W, H, nb, nc) for nb, nc in ((0, 1), (5, 40), (60, 400), (3, 0))]
+    ors = [(55, 45, 35), (45, 55, 35)]
+    outs = ctx.feas_check_batch([r[0] if r[0] else np.zeros((0, 3)) for r in rows], ors,
+                                [r[1] for r in rows], L, W, H)
+    for (pts, boxes), got in zip(rows, outs):
+        assert np.array_equal(got, orc.check_all(pts, ors, boxes, L, W, H) if pts else
+                              np.zeros(0, np.uint8))
+
+
+# ------------------------------------------------------------- FSR --------
+
+def test_fsr_eval_bits_vs_reference(ctx, ref_runs):
+    packs, want, mus = [], [], []
+    for c in ref_runs:
+        packs.append(np.asarray(c["result"]["placements"], np.int32).reshape(-1, 7))
+        want.append((c["result"]["fsr_hex"], c["result"]["stability_score"], c["args"]["mu"]))
+    for mu in sorted({w[2] for w in want}):
+        idx = [i for i, w in enumerate(want) if w[2] == mu]
+        fsr, score = ctx.fsr_eval([packs[i] for i in idx], mu=mu)
+        for j, i in enumerate(idx):
+            assert [hexd(v) for v in fsr[j]] == want[i][0], ref_runs[i]["case"]
+            assert score[j] == want[i][1], ref_runs[i]["case"]
+
+
+def test_fsr_eval_hand_kats(ctx):
+    tower = [[0, 0, 0, 10 * i, 10, 10, 10] for i in range(10)]
+    packs = [[[0, 0, 0, 0, 10, 10, 10]],
+             [[0, 0, 0, 0, 14, 10, 10], [0, 14, 0, 0, 14, 10, 10], [0, 5, 0, 10, 14, 10, 10]],
+             tower, np.zeros((0, 7))]
+    fsr, score = ctx.fsr_eval(packs)
+    assert fsr[0][0] == pytest.approx(2 / 3)
+    assert list(fsr[1]) == pytest.approx([23 / 21, 19 / 21, 2 / 3])
+    assert score[2] == 0.9 and score[3] == 0.0
+
+
+# ------------------------------------------------ device-resident engine --
+
+def test_engine_every_reference_case(plp, ctx, ref_runs):
+    """Each reference run as its own decoder in one multi-instance batch."""
+    by_params = {}
+    for c in ref_runs:
+        a = c["args"]
+        key = (a["passes"], a["eps"], a["theta"], a["mu"], a["order"], a["first_fit"])
+        by_params.setdefault(key, []).append(c)
+    for key, cases in by_params.items():
+        insts = [inst_from(plp, c["instance"]) for c in cases]
+        eng = plp.Engine(ctx)
+        eng.load(insts, list(range(len(cases))), [c["args"]["seed"] for c in cases],
+                 prm(plp, cases[0]["args"]))
+        eng.run(key[0])
+        eng.sync()
+        rows, pl = eng.fetch()
+        for i, c in enumerate(cases):
+            assert rows[i].err == 0, c["case"]
+            assert plp.to_json_dict(rows[i], pl[i]) == ref_fields(c["result"]), c["case"]
+            assert rows[i].work_pair_box == sum(c["result"]["pass_pair_box"]), c["case"]
+            assert rows[i].max_ep == c["result"]["max_cand"], c["case"]
+        eng.close()
+
+
+def test_engine_passes_in_pieces_equal_one_call(plp, ctx, ref_runs):
+    """RNG and D4' state persist on the device across runs (run(1) x 4 ==
+    run(4)), as the reference's time-budgeted loop needs."""
+    c = next(c for c in ref_runs if c["case"] == "BR1_p4_theta")
+    inst = inst_from(plp, c["instance"])
+    eng = plp.Engine(ctx)
+    eng.load([inst], [0], [c["args"]["seed"]], prm(plp, c["args"]))
+    for _ in range(4):
+        eng.run(1)
+        eng.sync()
+    rows, pl = eng.fetch()
+    assert plp.to_json_dict(rows[0], pl[0]) == ref_fields(c["result"])
+
+
+def test_engine_capacity_escalation(plp, ctx, ref_runs):
+    """A tiny EP capacity overflows; the engine reruns those decoders from
+    their seeds with a larger capacity and the results are unchanged."""
+    cases = [c for c in ref_runs if c["case"] in ("BR1_eps03_s1", "BR15_eps0", "plain_c1")]
+    for c in cases:
+        eng = plp.Engine(ctx, ep_cap=64)
+        eng.load([inst_from(plp, c["instance"])], [0], [c["args"]["seed"]], prm(plp, c["args"]))
+        eng.run(c["args"]["passes"])
+        eng.sync()
+        rows, pl = eng.fetch()
+        assert rows[0].err == 0
+        assert eng.describe()["ep_cap"] >= c["result"]["max_cand"]
+        assert plp.to_json_dict(rows[0], pl[0]) == ref_fields(c["result"]), c["case"]
+
+
+def test_engine_graph_replay_matches_eager(plp, ctx, ref_runs):
+    cases = [c for c in ref_runs if c["case"].startswith("multi_c2_s")]
+    insts = [inst_from(plp, cases[0]["instance"])]
+    seeds = [c["args"]["seed"] for c in cases]
+    outs = []
+    for use_graph in (False, True):
+        eng = plp.Engine(ctx, use_graph=use_graph)
+        eng.load(insts, [0] * len(seeds), seeds, prm(plp, cases[0]["args"]))
+        for rep in range(3):   # replays with eager work and syncs between them
+            eng.reset()
+            eng.run(2)
+            eng.sync()
+            rows, pl = eng.fetch()
+            got = [plp.to_json_dict(r, p) for r, p in zip(rows, pl)]
+            assert got == [ref_fields(c["result"]) for c in cases], (use_graph, rep)
+            outs.append(got)
+        eng.close()
+
+
+def test_engine_many_decoders_vs_oracle(plp, ctx, orc):
+    """K = 256 decoders of the 20-type instance (seeds 1..256, eps 0.01,
+    theta 0.9, 2 passes) against the oracle, decoder by decoder."""
+    text = load_text("synth/multi.txt")
+    inst = plp.Instance.from_text(text)
+    o_inst = orc.parse(text)[0]
+    K = 256
+    eng = plp.Engine(ctx)
+    eng.load([inst], [0] * K, list(range(1, K + 1)), plp.params(eps=0.01, theta=0.9))
+    eng.run(2)
+    eng.sync()
+    rows, pl = eng.fetch()
+    for k in range(K):
+        want = orc.run(o_inst, seed=k + 1, passes=2, eps=0.01, theta=0.9)
+        got = plp.to_json_dict(rows[k], pl[k])
+        assert got == {x: want[x] for x in got}, k
+        assert rows[k].work_pair_box == want["pair_box"], k
+
+
+def test_engine_mixed_batch_vs_oracle(plp, ctx, orc):
+    """Uneven instances (3..100 types) x 4 seeds in one persistent launch."""
+    from plp_amd import synth
+    texts = synth.mixed_batch(24, seed0=777_000)
+    insts = [plp.Instance.from_text(t) for t in texts]
+    dec_inst = [i for i in range(len(texts)) for _ in range(4)]
+    seeds = [100 + 3 * i + s for i in range(len(texts)) for s in range(4)]
+    eng = plp.Engine(ctx)
+    eng.load(insts, dec_inst, seeds, plp.params(eps=0.3, theta=0.9))
+    eng.run(1)
+    eng.sync()
+    rows, pl = eng.fetch()
+    for d, (i, s) in enumerate(zip(dec_inst, seeds)):
+        want = orc.run(orc.parse(texts[i])[0], seed=s, passes=1, eps=0.3, theta=0.9)
+        assert plp.to_json_dict(rows[d], pl[d]) == {k: want[k] for k in
+                                                     ("n_boxes", "volume_cm3", "stability_score",
+                                                      "fallback", "n_passes", "seed",
+                                                      "placements")}, d
+
+
+def test_engine_autotune_keeps_results(plp, ctx, ref_runs):
+    cases = [c for c in ref_runs if c["case"].startswith("multi_c2_s")]
+    inst = inst_from(plp, cases[0]["instance"])
+    seeds = [c["args"]["seed"] for c in cases]
+    eng = plp.Engine(ctx)
+    eng.load([inst], [0] * len(seeds), seeds, prm(plp, cases[0]["args"]))
+    cfg, ms = eng.autotune(1)
+    assert cfg["ep_cap"] > 0 and ms > 0
+    eng.run(2)
+    eng.sync()
+    rows, pl = eng.fetch()
+    assert [plp.to_json_dict(r, p) for r, p in zip(rows, pl)] == \
+        [ref_fields(c["result"]) for c in cases]
+
+
+def test_engine_rejects_out_of_range_container(plp, ctx):
+    eng = plp.Engine(ctx)
+    big = plp.Instance(2000, 2000, 2000, [(10, 1, 10, 1, 10, 1, 1)])
+    with pytest.raises(plp.PlpError) as ei:
+        eng.load([big], [0], [0], plp.params())
+    assert ei.value.code == plp.PLP_E_RANGE
+
+
+# ----------------------------------------------------------- plain form ---
+
+def test_plain_form_reference_cases(plp, ctx, ref_runs):
+    names = {"BR1_eps0", "BR2_eps0", "BR3_eps0", "SYN1_eps0", "BR1_eps03_s7", "BR2_p4_theta",
+             "tower10_eps0", "tower9_eps0", "no_orient_eps0", "empty_qty_eps0", "too_big_eps0"}
+    for c in ref_runs:
+        if c["case"] not in names:
+            continue
+        a = c["args"]
+        row, pl = ctx.plain_run(inst_from(plp, c["instance"]), a["seed"], a["passes"], prm(plp, a))
+        assert plp.to_json_dict(row, pl) == ref_fields(c["result"]), c["case"]
