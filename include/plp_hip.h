@@ -57,6 +57,7 @@ int plp_parse_instance(const char *text, int32_t *L, int32_t *W, int32_t *H,
 typedef struct plp_ctx plp_ctx;
 
 int plp_ctx_create(int32_t device, plp_ctx **out);
+/* Destroy every engine created on the context first: engines use its stream. */
 void plp_ctx_destroy(plp_ctx *ctx);
 
 /* ----------------------------------------------- seam 1: feasibility step */
@@ -127,7 +128,7 @@ typedef struct {
 } plp_params;
 
 typedef struct {
-    int32_t waves_per_decoder;   /* 0 = automatic */
+    int32_t waves_per_decoder;   /* wavefronts per decoder block: 1, 2, 4; 0 = automatic (1) */
     int32_t ep_cap;              /* extreme-point capacity per decoder, 0 = automatic */
     int32_t blocks_per_cu;       /* persistent blocks per CU, 0 = as many as fit */
     int32_t use_graph;           /* capture the launch sequence in a hipGraph */
@@ -142,6 +143,7 @@ typedef struct {
     double score;         /* stability_score of the selected packing */
     uint64_t seed;
     uint64_t work_pair_box;  /* sum over steps of n_cand * n_ors * n_boxes (brute-force AABB tests) */
+    uint64_t work_live_pair_box;  /* the same over the steps and live points the kernel evaluates */
     int32_t max_ep;          /* largest extreme-point list reached */
     int32_t reserved;
 } plp_row_result;

@@ -28,7 +28,7 @@ struct plp_engine {
     plp_engine_config cfg{};
     bool loaded = false;
     int n_inst = 0, n_dec = 0, max_items = 1, box_cap = 1;
-    int ep_cap = 0, lds = 0, grid = 0, occ = 0;
+    int ep_cap = 0, lds = 0, grid = 0, occ = 0, wpd = 1;
     plp_params params{};
     int use_rng = 0;
     int passes_done = 0;
@@ -44,7 +44,7 @@ struct plp_engine {
     float last_ms = 0.0f;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int g_passes = -1, g_reset = -1, g_grid = -1, g_cap = -1;
+    int g_passes = -1, g_reset = -1, g_grid = -1, g_cap = -1, g_wpd = -1;
 };
 
 namespace {
@@ -56,18 +56,18 @@ void drop_graph(plp_engine* g) {
     if (g->graph) (void)hipGraphDestroy(g->graph);
     g->gexec = nullptr;
     g->graph = nullptr;
-    g->g_passes = g->g_reset = g->g_grid = g->g_cap = -1;
+    g->g_passes = g->g_reset = g->g_grid = g->g_cap = g->g_wpd = -1;
 }
 
 // Launch shape for an EP capacity: LDS bytes, resident blocks per CU, grid.
 int shape(plp_engine* g, int ep_cap, int blocks_per_cu_req, int n_work) {
-    const plpk::LdsLayout lay = plpk::lds_layout(ep_cap, g->box_cap, g->use_rng);
+    const plpk::LdsLayout lay = plpk::lds_layout(ep_cap, g->box_cap, g->use_rng, g->wpd);
     if (lay.total + 256 > g->ctx->max_lds)
         return fail(PLP_E_EPCAP, "decoder state (" + std::to_string(lay.total) +
                                      " B of LDS for " + std::to_string(ep_cap) +
                                      " extreme points) exceeds the block LDS limit");
     int occ = 0;
-    hipError_t e = plpk::decode_occupancy(lay.total, &occ);
+    hipError_t e = plpk::decode_occupancy(g->wpd, lay.total, &occ);
     if (e != hipSuccess) return hip_fail(e, "decode_kernel occupancy");
     if (occ < 1) return fail(PLP_E_EPCAP, "decode_kernel cannot be resident with this LDS size");
     int bpc = blocks_per_cu_req > 0 ? std::min(blocks_per_cu_req, occ) : occ;
@@ -112,11 +112,11 @@ int launch_run(plp_engine* g, int n_passes, int reset) {
     hipError_t e;
     if (g->cfg.use_graph) {
         if (!g->gexec || g->g_passes != n_passes || g->g_reset != reset || g->g_grid != g->grid ||
-            g->g_cap != g->ep_cap) {
+            g->g_cap != g->ep_cap || g->g_wpd != g->wpd) {
             drop_graph(g);
             e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
             if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-            hipError_t le = plpk::launch_decode(a, g->grid, g->lds, c->stream);
+            hipError_t le = plpk::launch_decode(a, g->wpd, g->grid, g->lds, c->stream);
             e = hipStreamEndCapture(c->stream, &g->graph);
             if (le != hipSuccess) return hip_fail(le, "decode capture");
             if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
@@ -126,12 +126,13 @@ int launch_run(plp_engine* g, int n_passes, int reset) {
             g->g_reset = reset;
             g->g_grid = g->grid;
             g->g_cap = g->ep_cap;
+            g->g_wpd = g->wpd;
         }
         e = hipGraphLaunch(g->gexec, c->stream);
         if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
         return PLP_OK;
     }
-    e = plpk::launch_decode(a, g->grid, g->lds, c->stream);
+    e = plpk::launch_decode(a, g->wpd, g->grid, g->lds, c->stream);
     if (e != hipSuccess) return hip_fail(e, "decode_kernel launch");
     return PLP_OK;
 }
@@ -157,11 +158,12 @@ extern "C" int plp_engine_create(plp_ctx* ctx, const plp_engine_config* cfg, plp
     plp_engine* g = new plp_engine();
     g->ctx = ctx;
     if (cfg) g->cfg = *cfg;
-    if (g->cfg.waves_per_decoder < 0 || g->cfg.waves_per_decoder > 1 || g->cfg.ep_cap < 0 ||
-        g->cfg.blocks_per_cu < 0) {
+    const int wq = g->cfg.waves_per_decoder;
+    if ((wq != 0 && wq != 1 && wq != 2 && wq != 4) || g->cfg.ep_cap < 0 || g->cfg.blocks_per_cu < 0) {
         delete g;
-        return fail(PLP_E_ARG, "config: waves_per_decoder must be 0 or 1; caps >= 0");
+        return fail(PLP_E_ARG, "config: waves_per_decoder must be 0, 1, 2 or 4; caps >= 0");
     }
+    g->wpd = wq > 0 ? wq : 1;
     if ((e = hipEventCreate(&g->ev0)) != hipSuccess || (e = hipEventCreate(&g->ev1)) != hipSuccess) {
         delete g;
         return hip_fail(e, "hipEventCreate");
@@ -246,7 +248,7 @@ extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, in
     g->n_inst = n_inst;
     g->n_dec = n_dec;
     g->max_items = max_items;
-    g->box_cap = max_items;
+    g->box_cap = (max_items + 7) / 8 * 8;   // sentinel-padded groups of 8
     g->params = *params;
     g->use_rng = (params->first_fit || params->eps == 0.0) ? 0 : 1;
     g->h_dec_inst.assign(dec_inst, dec_inst + n_dec);
@@ -372,7 +374,7 @@ extern "C" int plp_engine_sync(plp_engine* g) {
         plpk::DecodeArgs a = make_args(g, g->passes_done, 1, (const int32_t*)g->d_retry.p,
                                        (int)failed.size());
         if ((e = hipEventRecord(g->ev0, c->stream)) != hipSuccess) return hip_fail(e, "event");
-        e = plpk::launch_decode(a, g->grid, g->lds, c->stream);
+        e = plpk::launch_decode(a, g->wpd, g->grid, g->lds, c->stream);
         g->grid = std::max(1, std::min(g->n_dec, (g->cfg.blocks_per_cu > 0
                                                       ? std::min(g->cfg.blocks_per_cu, g->occ)
                                                       : g->occ) * c->n_cu));
@@ -424,6 +426,7 @@ extern "C" int plp_engine_fetch(plp_engine* g, plp_row_result* rows, int32_t* pl
         r.score = have_acc ? s.s_acc : s.s_all;
         r.seed = g->h_seed[(size_t)d];
         r.work_pair_box = s.work_pair_box;
+        r.work_live_pair_box = s.work_live;
         r.max_ep = (int32_t)s.max_ep;
         if (placements) {
             const int32_t* src = (have_acc ? acc.data() : all.data()) + stride * (size_t)d;
@@ -442,7 +445,7 @@ extern "C" int plp_engine_last_ms(plp_engine* g, float* ms) {
 extern "C" int plp_engine_describe(plp_engine* g, int32_t* wpd, int32_t* ep_cap, int32_t* lds,
                                    int32_t* grid) {
     if (!g || !g->loaded) return fail(PLP_E_ARG, "engine not loaded");
-    if (wpd) *wpd = 1;
+    if (wpd) *wpd = g->wpd;
     if (ep_cap) *ep_cap = g->ep_cap;
     if (lds) *lds = g->lds;
     if (grid) *grid = g->grid;
@@ -469,31 +472,38 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
     caps.erase(std::unique(caps.begin(), caps.end()), caps.end());
     float bms = -1.0f;
     plp_engine_config bcfg = saved;
-    for (int cap : caps) {
-        g->cfg.ep_cap = cap;
-        g->cfg.blocks_per_cu = 0;
-        if (shape(g, cap, 0, g->n_dec) != PLP_OK) continue;
-        const int occ = g->occ;
-        std::vector<int> bpcs = {occ, std::max(1, occ * 3 / 4), std::max(1, occ / 2)};
-        bpcs.erase(std::unique(bpcs.begin(), bpcs.end()), bpcs.end());
-        for (int bpc : bpcs) {
-            g->cfg.blocks_per_cu = bpc;
-            if (shape(g, cap, bpc, g->n_dec) != PLP_OK) continue;
-            drop_graph(g);
-            g->need_reset = true;
-            rc = plp_engine_run(g, n_passes);
-            if (rc == PLP_OK) rc = plp_engine_sync(g);
-            if (rc != PLP_OK) return rc;
-            if (bms < 0.0f || g->last_ms < bms) {
-                bms = g->last_ms;
-                bcfg = g->cfg;
-                bcfg.ep_cap = g->ep_cap;   // escalation may have raised it
+    const int saved_wpd = g->wpd;
+    for (int wpd : {1, 2, 4}) {
+        if (saved.waves_per_decoder != 0 && wpd != saved.waves_per_decoder) continue;
+        g->wpd = wpd;
+        for (int cap : caps) {
+            g->cfg.ep_cap = cap;
+            g->cfg.blocks_per_cu = 0;
+            if (shape(g, cap, 0, g->n_dec) != PLP_OK) continue;
+            const int occ = g->occ;
+            std::vector<int> bpcs = {occ, std::max(1, occ * 3 / 4), std::max(1, occ / 2)};
+            bpcs.erase(std::unique(bpcs.begin(), bpcs.end()), bpcs.end());
+            for (int bpc : bpcs) {
+                g->cfg.blocks_per_cu = bpc;
+                if (shape(g, cap, bpc, g->n_dec) != PLP_OK) continue;
+                drop_graph(g);
+                g->need_reset = true;
+                rc = plp_engine_run(g, n_passes);
+                if (rc == PLP_OK) rc = plp_engine_sync(g);
+                if (rc != PLP_OK) return rc;
+                if (bms < 0.0f || g->last_ms < bms) {
+                    bms = g->last_ms;
+                    bcfg = g->cfg;
+                    bcfg.ep_cap = g->ep_cap;   // escalation may have raised it
+                    bcfg.waves_per_decoder = wpd;
+                }
             }
         }
     }
     g->cfg = saved;
     g->cfg.ep_cap = bcfg.ep_cap;
     g->cfg.blocks_per_cu = bcfg.blocks_per_cu;
+    g->wpd = bcfg.waves_per_decoder > 0 ? bcfg.waves_per_decoder : saved_wpd;
     rc = shape(g, bcfg.ep_cap, bcfg.blocks_per_cu, g->n_dec);
     if (rc != PLP_OK) return rc;
     drop_graph(g);
@@ -502,7 +512,7 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
     g->states_fresh = false;
     if (best) {
         *best = g->cfg;
-        best->waves_per_decoder = 1;
+        best->waves_per_decoder = g->wpd;
     }
     if (best_ms) *best_ms = bms;
     return PLP_OK;

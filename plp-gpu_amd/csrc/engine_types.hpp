@@ -29,10 +29,12 @@ struct DecState {
     int64_t v_all, v_acc;    // their volumes
     double s_all, s_acc;     // their stability scores
     uint64_t work_pair_box;  // brute-force AABB tests so far (sum of n_cand * n_ors * n_box)
+    uint64_t work_live;      // the same over the steps and live points this kernel evaluates
     uint32_t work_steps;     // steps that evaluated feasibility
     uint32_t max_ep;         // largest extreme-point list reached
+    uint32_t pad[2];
 };
-static_assert(sizeof(DecState) == 64, "DecState layout");
+static_assert(sizeof(DecState) == 80, "DecState layout");
 
 constexpr int kMtWords = 625;      // 624 state words + index
 
@@ -67,9 +69,10 @@ struct DecodeArgs {
 };
 
 // Dynamic LDS layout of one decoder block.  The pass-time region (extreme
-// points, candidate list, pair masks) is reused by FSR after the pass.
+// points, candidate list, pair masks, per-wave culled box lists) is reused by
+// FSR after the pass.  The box array has box_cap slots plus one sentinel.
 struct LdsLayout {
-    int32_t off_ep, off_tmp, off_act, off_mask, n_mask;
+    int32_t off_ep, off_tmp, off_act, off_mask, n_mask, off_rel;
     int32_t off_load, off_key, n_key;       // FSR view of the first region
     int32_t off_box, off_mt, total;
 };
@@ -80,21 +83,23 @@ __host__ __device__ inline int32_t next_pow2(int32_t v) {
     return p;
 }
 
-__host__ __device__ inline LdsLayout lds_layout(int32_t ep_cap, int32_t box_cap, int32_t use_rng) {
+__host__ __device__ inline LdsLayout lds_layout(int32_t ep_cap, int32_t box_cap, int32_t use_rng,
+                                                int32_t n_waves = 1) {
     LdsLayout l;
     l.off_ep = 0;
     l.off_tmp = ep_cap * 4;
     l.off_act = 2 * ep_cap * 4;
     l.off_mask = (3 * ep_cap * 4 + 7) & ~7;
     l.n_mask = (ep_cap * 6 + 63) / 64;
-    int32_t pass_end = l.off_mask + l.n_mask * 8;
+    l.off_rel = l.off_mask + l.n_mask * 8;
+    int32_t pass_end = l.off_rel + n_waves * (box_cap + 8) * 4;
     l.n_key = next_pow2(box_cap > 0 ? box_cap : 1);
     l.off_load = 0;
     l.off_key = box_cap * 8;
     int32_t fsr_end = l.off_key + l.n_key * 8;
     int32_t region = pass_end > fsr_end ? pass_end : fsr_end;
     l.off_box = (region + 15) & ~15;
-    l.off_mt = l.off_box + box_cap * 32;
+    l.off_mt = l.off_box + (box_cap + 8) * 32;
     l.total = l.off_mt + (use_rng ? 624 * 4 : 0);
     return l;
 }

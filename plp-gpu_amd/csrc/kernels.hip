@@ -43,37 +43,70 @@ __device__ __forceinline__ uint32_t xyz_key(int x, int y, int z, int xb, int yb)
 //   C3  z > 0: s = sum of top-face overlaps of boxes whose top is at z;
 //       infeasible iff (double)s / (double)(w*d) < 0.5
 //   C1  no strict-interior overlap with any box
-// The verdict does not depend on the order the boxes are visited, so boxes
-// are scanned newest first (nearest the packing front, where overlaps are
-// found soonest) and the wave leaves the scan once every lane has failed
-// (ballot every 8 boxes).  Areas fit 32 bits: L*W < 2^30 is a load-time
-// precondition, and ox, oy < 2^21 so the 24-bit multiply is exact.
-__device__ __forceinline__ bool pair_feasible(int x, int y, int z, int w, int d, int h, bool valid,
-                                              const Box* __restrict__ boxes, int n_box,
-                                              int L, int W, int H) {
+// The verdict does not depend on which boxes are visited in what order as
+// long as every box that can overlap or support the pair is visited, so the
+// wave visits only the boxes in rel[] (the chunk's culled list, newest first,
+// padded with the sentinel slot to a multiple of kBoxGroup).  A group's LDS
+// reads issue together; the wave leaves the scan once every lane has failed,
+// and skips the support arithmetic of a box whose top is at no live lane's
+// level (both wave-uniform branches).  Areas fit 32 bits: L*W < 2^30 is a
+// load-time precondition, and ox, oy < 2^21 so the 24-bit multiply is exact.
+constexpr int kBoxGroup = 8;
+
+__device__ __forceinline__ bool pair_feasible(int x, int y, int z, int w, int d, int h,
+                                              bool alive, const Box* __restrict__ boxes,
+                                              const uint32_t* __restrict__ rel, int n_rel) {
     const int x2 = x + w, y2 = y + d, z2 = z + h;
-    bool alive = valid && (x2 <= L) && (y2 <= W) && (z2 <= H);
     uint32_t s = 0;
-    int b = n_box - 1;
-    while (b >= 0) {
+    for (int g = 0; g < n_rel; g += kBoxGroup) {
         if (ballot(alive) == 0) break;
+        Box B[kBoxGroup];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            if (b - u >= 0) {
-                const Box B = boxes[b - u];
-                const bool ov = (x < B.x2) & (B.x < x2) & (y < B.y2) & (B.y < y2) &
-                                (z < B.z2) & (B.z < z2);
-                alive = alive & !ov;
-                const int ox = min(x2, B.x2) - max(x, B.x);
-                const int oy = min(y2, B.y2) - max(y, B.y);
-                const bool sup = (B.z2 == z) & (ox > 0) & (oy > 0);
+        for (int u = 0; u < kBoxGroup; u++) B[u] = boxes[rel[g + u]];
+#pragma unroll
+        for (int u = 0; u < kBoxGroup; u++) {
+            const bool ov = (x < B[u].x2) & (B[u].x < x2) & (y < B[u].y2) & (B[u].y < y2) &
+                            (z < B[u].z2) & (B[u].z < z2);
+            alive = alive & !ov;
+            if (ballot(alive & (B[u].z2 == z))) {
+                const int ox = min(x2, B[u].x2) - max(x, B[u].x);
+                const int oy = min(y2, B[u].y2) - max(y, B[u].y);
+                const bool sup = (B[u].z2 == z) & (ox > 0) & (oy > 0);
                 s += sup ? __umul24((uint32_t)ox, (uint32_t)oy) : 0u;
             }
         }
-        b -= 8;
     }
     if (alive && z > 0) alive = !((double)s / (double)((int64_t)w * d) < 0.5);
     return alive;
+}
+
+// Culled box list of one 64-pair chunk: every placed box that intersects the
+// region [xlo,xhi) x [ylo,yhi) x [zlo,zhi) spanned by the chunk's live pairs
+// or whose top lies in [zlo, zhi) over that footprint (a possible support).
+// A box outside it can neither overlap nor support any pair of the chunk.
+// Written newest first into rel[], padded with box index `sentinel` to a
+// multiple of kBoxGroup; returns the padded length.
+__device__ __forceinline__ int cull_boxes(const Box* __restrict__ boxes, int n_box, int sentinel,
+                                          int xlo, int xhi, int ylo, int yhi, int zlo, int zhi,
+                                          uint32_t* __restrict__ rel) {
+    const int lane = lane_id();
+    int n_rel = 0;
+    for (int top = n_box - 1; top >= 0; top -= kWave) {
+        const int b = top - lane;
+        bool r = false;
+        if (b >= 0) {
+            const Box B = boxes[b];
+            r = (B.x < xhi) & (B.x2 > xlo) & (B.y < yhi) & (B.y2 > ylo) & (B.z < zhi) &
+                (B.z2 >= zlo);
+        }
+        const uint64_t m = ballot(r);
+        if (r) rel[n_rel + rank_below(m)] = (uint32_t)b;
+        n_rel += __popcll(m);
+    }
+    const int n_pad = (n_rel + kBoxGroup - 1) / kBoxGroup * kBoxGroup;
+    if (lane < n_pad - n_rel) rel[n_rel + lane] = (uint32_t)sentinel;
+    wave_sync();
+    return n_pad;
 }
 
 // ---------------------------------------------------------------- FSR ----
@@ -97,7 +130,7 @@ __device__ double fsr_score(const Box* __restrict__ box, int n, double mu, doubl
                           (box[i].z2 - box[i].z);
         load[i] = 50.0 * (double)v / 1.0e6;   // m = RHO * V / 1e6
     }
-    block_sync();
+    wave_sync();
     // Bitonic sort, ascending.
     for (int k = 2; k <= n2; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
@@ -112,7 +145,7 @@ __device__ double fsr_score(const Box* __restrict__ box, int n, double mu, doubl
                     }
                 }
             }
-            block_sync();
+            wave_sync();
         }
     }
     (void)n_key;
@@ -146,7 +179,7 @@ __device__ double fsr_score(const Box* __restrict__ box, int n, double mu, doubl
                 }
             }
         }
-        block_sync();
+        wave_sync();
     }
     uint32_t cnt = 0;
     for (int i = lane; i < n; i += kWave) {
@@ -158,207 +191,23 @@ __device__ double fsr_score(const Box* __restrict__ box, int n, double mu, doubl
         cnt += (f >= 1.0) ? 1u : 0u;
     }
     const uint64_t c = wave_sum_u64(cnt);
-    block_sync();
+    wave_sync();
     return (double)c / (double)n;
 }
 
 // ------------------------------------------------------------ decoder ----
 
-struct PassOut {
-    int n_box;
-    int err;
-    int max_ep;
-    uint64_t work;
-    uint64_t steps;
+// Block-wide scalars of the step in flight.  Wave 0 writes them between
+// barriers; the per-step fields are double-buffered by step parity so a
+// wave still reading step s never sees step s+1's values.
+struct StepShared {
+    int item;       // queue item of the block
+    int quit;       // the decoder stopped (capacity error): leave the pass loop
+    int no[2];      // orientations to evaluate this step (0: nothing to evaluate)
+    int n_act[2];   // live extreme points
+    int n_box[2];   // placed boxes
+    int cnt[4];     // feasible pairs found by each wave
 };
-
-// One decode pass (plp_common.hpp:404-456) by one wave.  Extreme points are
-// packed (z, y, x) keys, sorted and unique in ep[] (flag kDead on points no
-// pair can ever use); act[] holds the live ones, so a pair index
-// p = e * n_ors + o enumerates exactly the reference's candidate-major feas
-// order restricted to points that can be feasible.  Each 64-pair chunk's
-// verdicts become one ballot mask in LDS; the pick is the idx-th set bit.
-__device__ PassOut decode_pass_wave(const DecodeArgs& a, const InstHdr& I, uint32_t* ep,
-                                    uint32_t* tmp, uint32_t* act, uint64_t* mask, Box* box,
-                                    uint32_t* mt, int& mti, int* s_ors) {
-    const int lane = lane_id();
-    const int L = I.L, W = I.W, H = I.H, xb = I.xbits, yb = I.ybits;
-    PassOut out = {0, 0, 1, 0, 0};
-    int n_ep = 1, n_act = 1, n_box = 0;
-    if (lane == 0) {
-        ep[0] = 0u;    // origin; L, W, H > 0 so it is live
-        act[0] = 0u;
-    }
-    block_sync();
-    const int32_t* steps = a.step_type + I.step_off;
-    for (int s = 0; s < I.n_steps; s++) {
-        const int ti = uni(steps[s]);
-        const int32_t* orow = a.ors_tab + (size_t)(I.ors_off + ti) * kOrsRow;
-        const int no = uni(orow[0]);
-        if (no == 0) continue;     // no allowed orientation: skipped, no RNG use
-        if (lane < 3 * no) s_ors[lane] = orow[1 + lane];
-        block_sync();
-        out.work += (uint64_t)n_ep * (uint64_t)no * (uint64_t)n_box;
-        out.steps++;
-        const int n_pairs = n_act * no;
-        const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)no - 1) / (uint64_t)no);
-        const int n_chunks = (n_pairs + kWave - 1) / kWave;
-        int cnt = 0;
-        for (int c = 0; c < n_chunks; c++) {
-            const int p = c * kWave + lane;
-            const bool valid = p < n_pairs;
-            const int e = no == 1 ? p : (int)__umulhi((uint32_t)p, magic);
-            const int o = p - e * no;
-            int x = 0, y = 0, z = 0, w = 1, d = 1, h = 1;
-            if (valid) {
-                key_xyz(act[e], xb, yb, x, y, z);
-                w = s_ors[3 * o];
-                d = s_ors[3 * o + 1];
-                h = s_ors[3 * o + 2];
-            }
-            const bool f = pair_feasible(x, y, z, w, d, h, valid, box, n_box, L, W, H);
-            const uint64_t m = ballot(f);
-            if (lane == 0) mask[c] = m;
-            cnt += __popcll(m);
-        }
-        cnt = uni(cnt);
-        if (cnt == 0) continue;    // this copy is skipped; no RNG use
-        block_sync();
-        // feas[0], or with eps-greedy feas[randrange(|feas|)] when
-        // random() >= 1 - eps (plp_common.hpp:427-435).
-        int idx = 0;
-        if (a.use_rng) {
-            if (mt_random(mt, mti) >= 1.0 - a.eps) idx = mt_randrange(mt, mti, cnt);
-        }
-        int c = 0;
-        uint64_t m = uni(mask[0]);
-        for (;;) {
-            const int pc = __popcll(m);
-            if (idx < pc) break;
-            idx -= pc;
-            c++;
-            m = uni(mask[c]);
-        }
-        const bool hit = ((m >> lane) & 1ull) && rank_below(m) == idx;
-        const int bit = __ffsll((unsigned long long)ballot(hit)) - 1;
-        const int p = c * kWave + bit;
-        const int e = p / no, o = p - (p / no) * no;
-        int x, y, z;
-        key_xyz(uni(act[e]), xb, yb, x, y, z);
-        x = uni(x);
-        y = uni(y);
-        z = uni(z);
-        const int w = uni(s_ors[3 * o]), d = uni(s_ors[3 * o + 1]), h = uni(s_ors[3 * o + 2]);
-        const int X2 = x + w, Y2 = y + d, Z2 = z + h;
-        if (lane == 0) {
-            Box nb;
-            nb.x = x; nb.y = y; nb.z = z; nb.x2 = X2; nb.y2 = Y2; nb.z2 = Z2;
-            nb.type = ti; nb.pad = 0;
-            box[n_box] = nb;
-        }
-        n_box++;
-        block_sync();
-
-        // Extreme-point update (plp_common.hpp:439-453): drop points strictly
-        // inside the new box (mark the ones in its half-open extent dead),
-        // then merge the 8 corners that are not already present.
-        int base = 0;
-        for (int i0 = 0; i0 < n_ep; i0 += kWave) {
-            const int i = i0 + lane;
-            const bool valid = i < n_ep;
-            const uint32_t v = valid ? ep[i] : 0u;
-            int px, py, pz;
-            key_xyz(v & kKeyMask, xb, yb, px, py, pz);
-            const bool strict = (x < px) & (px < X2) & (y < py) & (py < Y2) & (z < pz) & (pz < Z2);
-            const bool half = (x <= px) & (px < X2) & (y <= py) & (py < Y2) & (z <= pz) & (pz < Z2);
-            const bool keep = valid && !strict;
-            const uint64_t km = ballot(keep);
-            if (keep) tmp[base + rank_below(km)] = half ? (v | kDead) : v;
-            base += __popcll(km);
-        }
-        const int n_surv = uni(base);
-        // Corner j (lanes 0..7): bit 0 -> x side, bit 1 -> y side, bit 2 -> z side.
-        const int cx = (lane & 1) ? X2 : x;
-        const int cy = (lane & 2) ? Y2 : y;
-        const int cz = (lane & 4) ? Z2 : z;
-        const uint32_t ck = xyz_key(cx, cy, cz, xb, yb);
-        int jx[8], jy[8], jz[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            jx[j] = (j & 1) ? X2 : x;
-            jy[j] = (j & 2) ? Y2 : y;
-            jz[j] = (j & 4) ? Z2 : z;
-        }
-        uint32_t dead_bits = 0;
-        for (int b0 = 0; b0 < n_box; b0 += kWave) {
-            const int bi = b0 + lane;
-            const bool bv = bi < n_box;
-            Box B;
-            if (bv) B = box[bi];
-            else { B.x = B.y = B.z = 1; B.x2 = B.y2 = B.z2 = 0; }
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const bool in = (B.x <= jx[j]) & (jx[j] < B.x2) & (B.y <= jy[j]) & (jy[j] < B.y2) &
-                                (B.z <= jz[j]) & (jz[j] < B.z2);
-                if (ballot(in)) dead_bits |= 1u << j;
-            }
-        }
-        block_sync();
-        bool cdead = (cx >= L) | (cy >= W) | (cz >= H) | (((dead_bits >> (lane & 7)) & 1u) != 0);
-        int lo = 0;
-        bool kept = false;
-        if (lane < 8) {
-            int hi = n_surv;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if ((tmp[mid] & kKeyMask) < ck) lo = mid + 1;
-                else hi = mid;
-            }
-            kept = !(lo < n_surv && (tmp[lo] & kKeyMask) == ck);
-        }
-        const uint64_t kmask = ballot(kept);
-        const int n_kept = __popcll(kmask);
-        if (n_surv + n_kept > a.ep_cap) {
-            out.err = 4;   // PLP_E_EPCAP
-            out.max_ep = n_surv + n_kept;
-            out.n_box = n_box;
-            return out;
-        }
-        uint32_t kk[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) kk[j] = ((kmask >> j) & 1ull) ? read_lane(ck, j) : 0xffffffffu;
-        if (kept) {
-            int r = 0;
-#pragma unroll
-            for (int j = 0; j < 8; j++) r += (kk[j] < ck) ? 1 : 0;
-            ep[lo + r] = ck | (cdead ? kDead : 0u);
-        }
-        for (int i = lane; i < n_surv; i += kWave) {
-            const uint32_t v = tmp[i];
-            const uint32_t k = v & kKeyMask;
-            int r = 0;
-#pragma unroll
-            for (int j = 0; j < 8; j++) r += (kk[j] < k) ? 1 : 0;
-            ep[i + r] = v;
-        }
-        n_ep = n_surv + n_kept;
-        out.max_ep = max(out.max_ep, n_ep);
-        block_sync();
-        base = 0;
-        for (int i0 = 0; i0 < n_ep; i0 += kWave) {
-            const int i = i0 + lane;
-            const uint32_t v = i < n_ep ? ep[i] : kDead;
-            const bool live = (v & kDead) == 0u;
-            const uint64_t lm = ballot(live);
-            if (live) act[base + rank_below(lm)] = v;
-            base += __popcll(lm);
-        }
-        n_act = uni(base);
-        block_sync();
-    }
-    out.n_box = n_box;
-    return out;
-}
 
 __device__ __forceinline__ void write_rows(int32_t* dst, const Box* box, int n) {
     for (int t = lane_id(); t < 7 * n; t += kWave) {
@@ -378,10 +227,25 @@ __device__ __forceinline__ void write_rows(int32_t* dst, const Box* box, int n) 
     }
 }
 
-__global__ void __launch_bounds__(64) decode_kernel(DecodeArgs a) {
+// Persistent decoder: one block of NW waves decodes one (instance, seed)
+// for n_passes passes (plp_common.hpp:404-456 per pass, FSR and the D4'
+// selection after each), then takes the next queue item.
+//
+// Extreme points are packed (z, y, x) keys, sorted and unique in ep[]
+// (flag kDead on points no pair can ever use again); act[] holds the live
+// ones, so pair p = e * n_ors + o enumerates exactly the reference's
+// candidate-major feas order restricted to points that can be feasible.
+// Every wave evaluates 64-pair chunks c = wave, wave + NW, ...; each chunk's
+// verdicts become one ballot mask in LDS and the pick is the idx-th set bit
+// (plp_common.hpp:419-435).  Wave 0 alone runs the serial part of a step
+// (RNG, pick, place, extreme-point merge), FSR and the selection; the other
+// waves wait at the block barrier.
+template <int NW>
+__global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int s_ors[18];
-    const LdsLayout lay = lds_layout(a.ep_cap, a.box_cap, a.use_rng);
+    __shared__ StepShared sh;
+    const LdsLayout lay = lds_layout(a.ep_cap, a.box_cap, a.use_rng, NW);
     uint32_t* ep = (uint32_t*)(lds + lay.off_ep);
     uint32_t* tmp = (uint32_t*)(lds + lay.off_tmp);
     uint32_t* act = (uint32_t*)(lds + lay.off_act);
@@ -391,78 +255,334 @@ __global__ void __launch_bounds__(64) decode_kernel(DecodeArgs a) {
     Box* box = (Box*)(lds + lay.off_box);
     uint32_t* mt = (uint32_t*)(lds + lay.off_mt);
     const int lane = lane_id();
+    const int wid = (int)(threadIdx.x >> 6);
+    uint32_t* rel = (uint32_t*)(lds + lay.off_rel) + wid * (a.box_cap + 8);
+    const int sentinel = a.box_cap;    // box slot that always holds a sentinel
 
     for (;;) {
-        int item = 0;
-        if (lane == 0) item = (int)atomicAdd(a.work_counter, 1u);
-        item = read_lane(item, 0);
+        if (threadIdx.x == 0) sh.item = (int)atomicAdd(a.work_counter, 1u);
+        block_sync();
+        const int item = uni(sh.item);
+        block_sync();
         if (item >= a.n_work) break;
         const int d = uni(a.work_order[item]);
         const InstHdr I = a.inst[uni(a.dec_inst[d])];
+        const int L = I.L, W = I.W, H = I.H, xb = I.xbits, yb = I.ybits;
+        const int32_t* steps = a.step_type + I.step_off;
+
+        // ---- decoder state (wave 0)
         DecState st;
-        if (a.reset) {
-            st.passes = 0; st.err = 0; st.n_all = -1; st.n_acc = -1;
-            st.v_all = 0; st.v_acc = 0; st.s_all = 0.0; st.s_acc = 0.0;
-            st.work_pair_box = 0; st.work_steps = 0; st.max_ep = 0;
-        } else {
-            st = a.state[d];
-            st.passes = uni(st.passes); st.err = uni(st.err);
-            st.n_all = uni(st.n_all); st.n_acc = uni(st.n_acc);
-            st.v_all = uni(st.v_all); st.v_acc = uni(st.v_acc);
-            st.work_pair_box = uni(st.work_pair_box); st.work_steps = uni(st.work_steps);
-            st.max_ep = uni(st.max_ep);
-        }
-        if (st.err != 0) continue;
         int mti = 624;
         uint32_t* gmt = a.mt + (size_t)d * kMtWords;
-        if (a.use_rng) {
+        if (wid == 0) {
             if (a.reset) {
-                mt_seed_lane0(mt, a.dec_seed[d]);
+                st.passes = 0; st.err = 0; st.n_all = -1; st.n_acc = -1;
+                st.v_all = 0; st.v_acc = 0; st.s_all = 0.0; st.s_acc = 0.0;
+                st.work_pair_box = 0; st.work_live = 0; st.work_steps = 0; st.max_ep = 0;
+                st.pad[0] = st.pad[1] = 0;
             } else {
-                for (int i = lane; i < 624; i += kWave) mt[i] = gmt[i];
-                mti = uni((int)gmt[624]);
+                st = a.state[d];
+                st.passes = uni(st.passes); st.err = uni(st.err);
+                st.n_all = uni(st.n_all); st.n_acc = uni(st.n_acc);
+                st.v_all = uni(st.v_all); st.v_acc = uni(st.v_acc);
+                st.work_pair_box = uni(st.work_pair_box); st.work_live = uni(st.work_live);
+                st.work_steps = uni(st.work_steps); st.max_ep = uni(st.max_ep);
             }
-            block_sync();
+            if (st.err == 0 && a.use_rng) {
+                if (a.reset) {
+                    mt_seed_lane0(mt, a.dec_seed[d]);
+                } else {
+                    for (int i = lane; i < 624; i += kWave) mt[i] = gmt[i];
+                    mti = uni((int)gmt[624]);
+                }
+                wave_sync();
+            }
+            if (lane == 0) sh.quit = st.err != 0;
         }
+        block_sync();
+        const bool skip_decoder = uni(sh.quit) != 0;
         int32_t* out_all = a.best_all + (size_t)d * a.max_items * 7;
         int32_t* out_acc = a.best_acc + (size_t)d * a.max_items * 7;
-        for (int pass = 0; pass < a.n_passes; pass++) {
-            PassOut po = decode_pass_wave(a, I, (uint32_t*)(lds + lay.off_ep), tmp, act, mask,
-                                          box, mt, mti, s_ors);
-            st.work_pair_box += po.work;
-            st.work_steps += (uint32_t)po.steps;
-            st.max_ep = max(st.max_ep, (uint32_t)po.max_ep);
-            if (po.err) {
-                st.err = po.err;
-                break;
+
+        for (int pass = 0; pass < a.n_passes && !skip_decoder; pass++) {
+            // ---- pass start (wave 0): origin only, no boxes
+            int n_ep = 1, n_act = 1, n_box = 0;
+            int fail_ti = -1;   // type whose last copy found nothing since the last placement
+            uint64_t work = 0, live = 0;
+            uint32_t n_steps_eval = 0, max_ep = 1;
+            if (wid == 0) {
+                if (lane == 0) {
+                    ep[0] = 0u;     // origin; L, W, H > 0 so it is live
+                    act[0] = 0u;
+                }
+                for (int i = lane; i < a.box_cap + 8; i += kWave) {
+                    Box sb;
+                    sb.x = sb.y = sb.z = sb.x2 = sb.y2 = sb.z2 = -1;
+                    sb.type = 0;
+                    sb.pad = 0;
+                    box[i] = sb;
+                }
             }
-            const int n = po.n_box;
-            int64_t vpart = 0;
-            for (int i = lane; i < n; i += kWave)
-                vpart += (int64_t)(box[i].x2 - box[i].x) * (box[i].y2 - box[i].y) *
-                         (box[i].z2 - box[i].z);
-            const int64_t vol = wave_sum_i64(vpart);
-            const double score = fsr_score(box, n, a.mu, load, skey, lay.n_key, nullptr);
-            // D4' (plp_common.hpp:493-511), kept incrementally: the best of
-            // all packings and the best accepted one, strict > on
-            // (volume, n_boxes) so the earliest winner stays.
-            if (st.n_all < 0 || vol > st.v_all || (vol == st.v_all && n > st.n_all)) {
-                st.n_all = n; st.v_all = vol; st.s_all = score;
-                write_rows(out_all, box, n);
-            }
-            const bool accepted = !a.use_theta || score >= a.theta;
-            if (accepted && (st.n_acc < 0 || vol > st.v_acc || (vol == st.v_acc && n > st.n_acc))) {
-                st.n_acc = n; st.v_acc = vol; st.s_acc = score;
-                write_rows(out_acc, box, n);
-            }
-            st.passes++;
             block_sync();
+            for (int s = 0; s < I.n_steps; s++) {
+                const int par = s & 1;
+                const int ti = uni(steps[s]);
+                // ---- step prologue (wave 0)
+                if (wid == 0) {
+                    const int32_t* orow = a.ors_tab + (size_t)(I.ors_off + ti) * kOrsRow;
+                    const int no = uni(orow[0]);
+                    int eval = 0;
+                    if (no > 0) {
+                        work += (uint64_t)n_ep * (uint64_t)no * (uint64_t)n_box;
+                        n_steps_eval++;
+                        // Same type, no placement since its last copy found
+                        // nothing: the verdicts repeat, nothing is drawn.
+                        if (ti != fail_ti) {
+                            eval = no;
+                            live += (uint64_t)n_act * (uint64_t)no * (uint64_t)n_box;
+                            if (lane < 3 * no) s_ors[lane] = orow[1 + lane];
+                        }
+                    }
+                    if (lane == 0) {
+                        sh.no[par] = eval;
+                        sh.n_act[par] = n_act;
+                        sh.n_box[par] = n_box;
+                    }
+                }
+                block_sync();
+                const int no = uni(sh.no[par]);
+                if (no == 0) continue;
+                const int na = uni(sh.n_act[par]);
+                const int nb = uni(sh.n_box[par]);
+
+                // ---- pair verdicts (every wave)
+                const int n_pairs = na * no;
+                const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)no - 1) / (uint64_t)no);
+                const int n_chunks = (n_pairs + kWave - 1) / kWave;
+                int cnt_w = 0;
+                for (int c = wid; c < n_chunks; c += NW) {
+                    const int p = c * kWave + lane;
+                    const bool valid = p < n_pairs;
+                    const int e = no == 1 ? p : (int)__umulhi((uint32_t)p, magic);
+                    const int o = p - e * no;
+                    int x = 0, y = 0, z = 0, w = 1, dd = 1, h = 1;
+                    if (valid) {
+                        key_xyz(act[e], xb, yb, x, y, z);
+                        w = s_ors[3 * o];
+                        dd = s_ors[3 * o + 1];
+                        h = s_ors[3 * o + 2];
+                    }
+                    const bool alive = valid && (x + w <= L) && (y + dd <= W) && (z + h <= H);
+                    uint64_t m = 0;
+                    if (ballot(alive) != 0) {
+                        const int xlo = wave_min(alive ? x : 0x7fffffff);
+                        const int xhi = wave_max(alive ? x + w : -1);
+                        const int ylo = wave_min(alive ? y : 0x7fffffff);
+                        const int yhi = wave_max(alive ? y + dd : -1);
+                        const int zlo = wave_min(alive ? z : 0x7fffffff);
+                        const int zhi = wave_max(alive ? z + h : -1);
+                        const int n_rel = cull_boxes(box, nb, sentinel, xlo, xhi, ylo, yhi, zlo,
+                                                     zhi, rel);
+                        m = ballot(pair_feasible(x, y, z, w, dd, h, alive, box, rel, n_rel));
+                    }
+                    if (lane == 0) mask[c] = m;
+                    cnt_w += __popcll(m);
+                }
+                if (lane == 0) sh.cnt[wid] = cnt_w;
+                block_sync();
+
+                // ---- pick, place, extreme-point merge (wave 0)
+                if (wid == 0) {
+                    int cnt = 0;
+#pragma unroll
+                    for (int w2 = 0; w2 < NW; w2++) cnt += sh.cnt[w2];
+                    cnt = uni(cnt);
+                    if (cnt == 0) {
+                        fail_ti = ti;     // this copy is skipped; no RNG use
+                    } else {
+                        fail_ti = -1;
+                        // feas[0], or feas[randrange(|feas|)] when random()
+                        // >= 1 - eps (plp_common.hpp:427-435).
+                        int idx = 0;
+                        if (a.use_rng) {
+                            if (mt_random(mt, mti) >= 1.0 - a.eps) idx = mt_randrange(mt, mti, cnt);
+                        }
+                        int c = 0;
+                        uint64_t m = uni(mask[0]);
+                        for (;;) {
+                            const int pc = __popcll(m);
+                            if (idx < pc) break;
+                            idx -= pc;
+                            c++;
+                            m = uni(mask[c]);
+                        }
+                        const bool hit = ((m >> lane) & 1ull) && rank_below(m) == idx;
+                        const int bit = __ffsll((unsigned long long)ballot(hit)) - 1;
+                        const int p = c * kWave + bit;
+                        const int e = p / no, o = p - (p / no) * no;
+                        int x, y, z;
+                        key_xyz(uni(act[e]), xb, yb, x, y, z);
+                        x = uni(x);
+                        y = uni(y);
+                        z = uni(z);
+                        const int w = uni(s_ors[3 * o]), dd = uni(s_ors[3 * o + 1]),
+                                  h = uni(s_ors[3 * o + 2]);
+                        const int X2 = x + w, Y2 = y + dd, Z2 = z + h;
+                        if (lane == 0) {
+                            Box nbx;
+                            nbx.x = x; nbx.y = y; nbx.z = z; nbx.x2 = X2; nbx.y2 = Y2; nbx.z2 = Z2;
+                            nbx.type = ti; nbx.pad = 0;
+                            box[n_box] = nbx;
+                        }
+                        n_box++;
+                        wave_sync();
+
+                        // Drop points strictly inside the new box (mark the
+                        // ones in its half-open extent dead), then merge the
+                        // 8 corners not already present (plp_common.hpp:439-453).
+                        int base = 0;
+                        for (int i0 = 0; i0 < n_ep; i0 += kWave) {
+                            const int i = i0 + lane;
+                            const bool v_ok = i < n_ep;
+                            const uint32_t v = v_ok ? ep[i] : 0u;
+                            int px, py, pz;
+                            key_xyz(v & kKeyMask, xb, yb, px, py, pz);
+                            const bool strict = (x < px) & (px < X2) & (y < py) & (py < Y2) &
+                                                (z < pz) & (pz < Z2);
+                            const bool half = (x <= px) & (px < X2) & (y <= py) & (py < Y2) &
+                                              (z <= pz) & (pz < Z2);
+                            const bool keep = v_ok && !strict;
+                            const uint64_t km = ballot(keep);
+                            if (keep) tmp[base + rank_below(km)] = half ? (v | kDead) : v;
+                            base += __popcll(km);
+                        }
+                        const int n_surv = uni(base);
+                        // Corner j (lanes 0..7): bit 0 -> x side, bit 1 -> y, bit 2 -> z.
+                        const int cx = (lane & 1) ? X2 : x;
+                        const int cy = (lane & 2) ? Y2 : y;
+                        const int cz = (lane & 4) ? Z2 : z;
+                        const uint32_t ck = xyz_key(cx, cy, cz, xb, yb);
+                        int jx[8], jy[8], jz[8];
+#pragma unroll
+                        for (int j = 0; j < 8; j++) {
+                            jx[j] = (j & 1) ? X2 : x;
+                            jy[j] = (j & 2) ? Y2 : y;
+                            jz[j] = (j & 4) ? Z2 : z;
+                        }
+                        uint32_t dead_bits = 0;
+                        for (int b0 = 0; b0 < n_box; b0 += kWave) {
+                            const int bi = b0 + lane;
+                            const bool bv = bi < n_box;
+                            Box B;
+                            if (bv) B = box[bi];
+                            else { B.x = B.y = B.z = 1; B.x2 = B.y2 = B.z2 = 0; }
+#pragma unroll
+                            for (int j = 0; j < 8; j++) {
+                                const bool in = (B.x <= jx[j]) & (jx[j] < B.x2) & (B.y <= jy[j]) &
+                                                (jy[j] < B.y2) & (B.z <= jz[j]) & (jz[j] < B.z2);
+                                if (ballot(in)) dead_bits |= 1u << j;
+                            }
+                        }
+                        wave_sync();
+                        const bool cdead = (cx >= L) | (cy >= W) | (cz >= H) |
+                                           (((dead_bits >> (lane & 7)) & 1u) != 0);
+                        int lo = 0;
+                        bool kept = false;
+                        if (lane < 8) {
+                            int hi = n_surv;
+                            while (lo < hi) {
+                                const int mid = (lo + hi) >> 1;
+                                if ((tmp[mid] & kKeyMask) < ck) lo = mid + 1;
+                                else hi = mid;
+                            }
+                            kept = !(lo < n_surv && (tmp[lo] & kKeyMask) == ck);
+                        }
+                        const uint64_t kmask = ballot(kept);
+                        const int n_kept = __popcll(kmask);
+                        if (n_surv + n_kept > a.ep_cap) {
+                            st.err = 4;   // PLP_E_EPCAP
+                            max_ep = max(max_ep, (uint32_t)(n_surv + n_kept));
+                        } else {
+                            uint32_t kk[8];
+#pragma unroll
+                            for (int j = 0; j < 8; j++)
+                                kk[j] = ((kmask >> j) & 1ull) ? read_lane(ck, j) : 0xffffffffu;
+                            if (kept) {
+                                int r = 0;
+#pragma unroll
+                                for (int j = 0; j < 8; j++) r += (kk[j] < ck) ? 1 : 0;
+                                ep[lo + r] = ck | (cdead ? kDead : 0u);
+                            }
+                            for (int i = lane; i < n_surv; i += kWave) {
+                                const uint32_t v = tmp[i];
+                                const uint32_t k = v & kKeyMask;
+                                int r = 0;
+#pragma unroll
+                                for (int j = 0; j < 8; j++) r += (kk[j] < k) ? 1 : 0;
+                                ep[i + r] = v;
+                            }
+                            n_ep = n_surv + n_kept;
+                            max_ep = max(max_ep, (uint32_t)n_ep);
+                            wave_sync();
+                            base = 0;
+                            for (int i0 = 0; i0 < n_ep; i0 += kWave) {
+                                const int i = i0 + lane;
+                                const uint32_t v = i < n_ep ? ep[i] : kDead;
+                                const bool lv = (v & kDead) == 0u;
+                                const uint64_t lm = ballot(lv);
+                                if (lv) act[base + rank_below(lm)] = v;
+                                base += __popcll(lm);
+                            }
+                            n_act = uni(base);
+                            wave_sync();
+                        }
+                        if (lane == 0) sh.quit = st.err != 0;
+                    }
+                }
+                block_sync();
+                if (uni(sh.quit) != 0) break;
+            }
+
+            // ---- pass end (wave 0): stability score and D4' selection
+            if (wid == 0) {
+                st.work_pair_box += work;
+                st.work_live += live;
+                st.work_steps += n_steps_eval;
+                st.max_ep = max(st.max_ep, max_ep);
+                if (st.err == 0) {
+                    const int n = n_box;
+                    int64_t vpart = 0;
+                    for (int i = lane; i < n; i += kWave)
+                        vpart += (int64_t)(box[i].x2 - box[i].x) * (box[i].y2 - box[i].y) *
+                                 (box[i].z2 - box[i].z);
+                    const int64_t vol = wave_sum_i64(vpart);
+                    const double score = fsr_score(box, n, a.mu, load, skey, lay.n_key, nullptr);
+                    // D4' (plp_common.hpp:493-511), kept incrementally: the best
+                    // of all packings and the best accepted one, strict > on
+                    // (volume, n_boxes) so the earliest winner stays.
+                    if (st.n_all < 0 || vol > st.v_all || (vol == st.v_all && n > st.n_all)) {
+                        st.n_all = n; st.v_all = vol; st.s_all = score;
+                        write_rows(out_all, box, n);
+                    }
+                    const bool accepted = !a.use_theta || score >= a.theta;
+                    if (accepted &&
+                        (st.n_acc < 0 || vol > st.v_acc || (vol == st.v_acc && n > st.n_acc))) {
+                        st.n_acc = n; st.v_acc = vol; st.s_acc = score;
+                        write_rows(out_acc, box, n);
+                    }
+                    st.passes++;
+                }
+            }
+            block_sync();
+            if (uni(sh.quit) != 0) break;
         }
-        if (a.use_rng) {
-            for (int i = lane; i < 624; i += kWave) gmt[i] = mt[i];
-            if (lane == 0) gmt[624] = (uint32_t)mti;
+        if (wid == 0 && !skip_decoder) {
+            if (a.use_rng) {
+                for (int i = lane; i < 624; i += kWave) gmt[i] = mt[i];
+                if (lane == 0) gmt[624] = (uint32_t)mti;
+            }
+            if (lane == 0) a.state[d] = st;
         }
-        if (lane == 0) a.state[d] = st;
         block_sync();
     }
 }
@@ -573,9 +693,15 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int n) {
 
 // ------------------------------------------------------------ launchers --
 
-hipError_t launch_decode(const DecodeArgs& a, int grid, int lds_bytes, hipStream_t st) {
+hipError_t launch_decode(const DecodeArgs& a, int waves, int grid, int lds_bytes,
+                         hipStream_t st) {
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, a.work_counter, 0u, 1);
-    hipLaunchKernelGGL(decode_kernel, dim3(grid), dim3(64), (size_t)lds_bytes, st, a);
+    switch (waves) {
+        case 1: hipLaunchKernelGGL(decode_kernel<1>, dim3(grid), dim3(64), (size_t)lds_bytes, st, a); break;
+        case 2: hipLaunchKernelGGL(decode_kernel<2>, dim3(grid), dim3(128), (size_t)lds_bytes, st, a); break;
+        case 4: hipLaunchKernelGGL(decode_kernel<4>, dim3(grid), dim3(256), (size_t)lds_bytes, st, a); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -594,12 +720,22 @@ hipError_t launch_rng_script(const RngArgs& g, int n_seeds, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t decode_occupancy(int lds_bytes, int* blocks_per_cu) {
-    hipError_t e = hipFuncSetAttribute((const void*)decode_kernel,
+template <int NW>
+static hipError_t occupancy_of(int lds_bytes, int* blocks_per_cu) {
+    hipError_t e = hipFuncSetAttribute((const void*)decode_kernel<NW>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
     if (e != hipSuccess) return e;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, decode_kernel, 64,
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, decode_kernel<NW>, 64 * NW,
                                                         (size_t)lds_bytes);
+}
+
+hipError_t decode_occupancy(int waves, int lds_bytes, int* blocks_per_cu) {
+    switch (waves) {
+        case 1: return occupancy_of<1>(lds_bytes, blocks_per_cu);
+        case 2: return occupancy_of<2>(lds_bytes, blocks_per_cu);
+        case 4: return occupancy_of<4>(lds_bytes, blocks_per_cu);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t fsr_prepare(int lds_bytes) {

@@ -38,7 +38,7 @@ __device__ __forceinline__ uint32_t mt_mix(uint32_t cur, uint32_t nxt, uint32_t 
 
 // Serial CPython seeding by lane 0 (the recurrence is inherently serial);
 // the previous word is carried in a register so LDS reads stay off the
-// dependence chain.  Caller syncs afterwards.
+// dependence chain.  Caller syncs the wave afterwards.
 __device__ inline void mt_seed_lane0(uint32_t* mt, uint64_t seed) {
     if (lane_id() != 0) return;
     uint32_t key[2];
@@ -83,7 +83,7 @@ __device__ inline void mt_seed_lane0(uint32_t* mt, uint64_t seed) {
     mt[0] = 0x80000000u;
 }
 
-// Regenerate the block: all lanes of a one-wave block.
+// Regenerate the block: all lanes of one wave.
 __device__ inline void mt_twist(uint32_t* mt) {
     const int lane = lane_id();
     uint32_t v[4];
@@ -93,44 +93,44 @@ __device__ inline void mt_twist(uint32_t* mt) {
         int i = lane + r * kWave;
         if (i < 227) v[r] = mt_mix(mt[i], mt[i + 1], mt[i + 397]);
     }
-    block_sync();
+    wave_sync();
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         int i = lane + r * kWave;
         if (i < 227) mt[i] = v[r];
     }
-    block_sync();
+    wave_sync();
     // [227, 454): mt[i-227] is new.
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         int i = 227 + lane + r * kWave;
         if (i < 454) v[r] = mt_mix(mt[i], mt[i + 1], mt[i - 227]);
     }
-    block_sync();
+    wave_sync();
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         int i = 227 + lane + r * kWave;
         if (i < 454) mt[i] = v[r];
     }
-    block_sync();
+    wave_sync();
     // [454, 623): mt[i-227] is new.
 #pragma unroll
     for (int r = 0; r < 3; r++) {
         int i = 454 + lane + r * kWave;
         if (i < 623) v[r] = mt_mix(mt[i], mt[i + 1], mt[i - 227]);
     }
-    block_sync();
+    wave_sync();
 #pragma unroll
     for (int r = 0; r < 3; r++) {
         int i = 454 + lane + r * kWave;
         if (i < 623) mt[i] = v[r];
     }
-    block_sync();
+    wave_sync();
     // 623: wraps to the new mt[0].
     uint32_t last = mt_mix(mt[623], mt[0], mt[396]);
-    block_sync();
+    wave_sync();
     if (lane == 0) mt[623] = last;
-    block_sync();
+    wave_sync();
 }
 
 __device__ __forceinline__ uint32_t mt_temper(uint32_t y) {

@@ -33,6 +33,7 @@ int PlainRunner::run_pass() {
         r.cand.assign(1, {0, 0, 0});
         r.aabb.clear();
         r.placed.clear();
+        r.fail_ti = -1;
     }
     std::vector<const int32_t*> cp((size_t)K), bp((size_t)K);
     std::vector<int32_t> nc((size_t)K), nb((size_t)K);
@@ -41,26 +42,39 @@ int PlainRunner::run_pass() {
     for (int ti : seq_) {
         const auto& ors = orients_[(size_t)ti];
         if (ors.empty()) continue;
+        // A row whose previous copy of this type found nothing, with no
+        // placement since, would get the same all-zero verdicts: it sits out.
+        int n_live = 0;
+        std::vector<int> live((size_t)K);
         for (int k = 0; k < K; k++) {
             Row& r = rows_[(size_t)k];
-            bits[(size_t)k].assign(r.cand.size() * ors.size(), 0);
-            cp[(size_t)k] = &r.cand[0][0];
-            nc[(size_t)k] = (int32_t)r.cand.size();
-            bp[(size_t)k] = r.aabb.empty() ? nullptr : &r.aabb[0][0];
-            nb[(size_t)k] = (int32_t)r.aabb.size();
-            op[(size_t)k] = bits[(size_t)k].data();
             r.work += (uint64_t)r.cand.size() * ors.size() * r.aabb.size();
+            if (r.fail_ti == ti) continue;
+            const size_t j = (size_t)n_live++;
+            live[j] = k;
+            bits[(size_t)k].assign(r.cand.size() * ors.size(), 0);
+            cp[j] = &r.cand[0][0];
+            nc[j] = (int32_t)r.cand.size();
+            bp[j] = r.aabb.empty() ? nullptr : &r.aabb[0][0];
+            nb[j] = (int32_t)r.aabb.size();
+            op[j] = bits[(size_t)k].data();
         }
-        int rc = plp_feas_check_batch(ctx_, K, cp.data(), nc.data(), &ors[0][0], (int32_t)ors.size(),
-                                      bp.data(), nb.data(), inst_.L, inst_.W, inst_.H, op.data());
+        if (n_live == 0) continue;
+        int rc = plp_feas_check_batch(ctx_, n_live, cp.data(), nc.data(), &ors[0][0],
+                                      (int32_t)ors.size(), bp.data(), nb.data(), inst_.L, inst_.W,
+                                      inst_.H, op.data());
         if (rc != PLP_OK) return rc;
-        for (int k = 0; k < K; k++) {
-            Row& r = rows_[(size_t)k];
-            const std::vector<uint8_t>& b = bits[(size_t)k];
+        for (int j = 0; j < n_live; j++) {
+            Row& r = rows_[(size_t)live[(size_t)j]];
+            const std::vector<uint8_t>& b = bits[(size_t)live[(size_t)j]];
             std::vector<int> feas;   // candidate-major pair indices
             for (size_t t = 0; t < b.size(); t++)
                 if (b[t]) feas.push_back((int)t);
-            if (feas.empty()) continue;
+            if (feas.empty()) {
+                r.fail_ti = ti;
+                continue;
+            }
+            r.fail_ti = -1;
             int pick = feas[0];
             if (use_rng_ && r.rng.random() >= 1.0 - p_.eps)
                 pick = feas[(size_t)r.rng.randrange((int)feas.size())];

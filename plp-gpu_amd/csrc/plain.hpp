@@ -34,6 +34,7 @@ private:
         std::vector<std::array<int32_t, 7>> placed;   // (type, x, y, z, w, d, h)
         Pack all, acc;
         int passes = 0;
+        int fail_ti = -1;   // type whose last copy failed since the last placement
         int max_ep = 1;
         uint64_t work = 0;
     };

@@ -54,8 +54,28 @@ __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
     return uni(v);
 }
 
-// LDS ordering point between phases.  Blocks here are one or a few waves;
-// the barrier also orders this wave's own LDS traffic across lanes.
+// LDS ordering point between phases of a block (one or a few waves).
 __device__ __forceinline__ void block_sync() { __syncthreads(); }
+
+// LDS ordering point between lanes of one wave.  A wave's LDS instructions
+// execute in issue order, so a store seen before a load in program order is
+// visible to that load; the fences and the wave barrier stop the compiler
+// from moving LDS accesses across this point.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, kWave));
+    return uni(v);
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
+    return uni(v);
+}
 
 }  // namespace plpk

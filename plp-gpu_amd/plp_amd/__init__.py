@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -58,8 +59,8 @@ class EngineConfig(C.Structure):
 class RowResult(C.Structure):
     _fields_ = [("n_boxes", C.c_int32), ("fallback", C.c_int32), ("n_passes", C.c_int32),
                 ("err", C.c_int32), ("volume", C.c_int64), ("score", C.c_double),
-                ("seed", C.c_uint64), ("work_pair_box", C.c_uint64), ("max_ep", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("seed", C.c_uint64), ("work_pair_box", C.c_uint64),
+                ("work_live_pair_box", C.c_uint64), ("max_ep", C.c_int32), ("reserved", C.c_int32)]
 
 
 _P = C.c_void_p
@@ -168,9 +169,13 @@ def params(eps=0.3, theta=0.9, mu=0.2, use_theta=True, first_fit=False, order="v
 class Ctx:
     def __init__(self, device: int = 0):
         self.h = C.c_void_p()
+        self._children = weakref.WeakSet()
         _check(_lib.plp_ctx_create(device, C.byref(self.h)))
 
     def close(self):
+        # Engines borrow the context's stream: destroy them first.
+        for child in list(self._children):
+            child.close()
         if self.h:
             _lib.plp_ctx_destroy(self.h)
             self.h = C.c_void_p()
@@ -236,11 +241,13 @@ class Ctx:
 class Engine:
     """Device-resident decoders (seam 3)."""
 
-    def __init__(self, ctx: Ctx, ep_cap=0, blocks_per_cu=0, use_graph=False):
+    def __init__(self, ctx: Ctx, ep_cap=0, blocks_per_cu=0, use_graph=False, waves_per_decoder=0):
         self.ctx = ctx
         self.h = C.c_void_p()
-        cfg = EngineConfig(0, int(ep_cap), int(blocks_per_cu), int(bool(use_graph)))
+        cfg = EngineConfig(int(waves_per_decoder), int(ep_cap), int(blocks_per_cu),
+                           int(bool(use_graph)))
         _check(_lib.plp_engine_create(ctx.h, C.byref(cfg), C.byref(self.h)))
+        ctx._children.add(self)
         self.n_dec = 0
 
     def close(self):
