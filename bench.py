@@ -43,25 +43,25 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(text: str, threads: int) -> dict | None:
+def cpu_baseline(text: str, threads: int, n: int = 1024, passes: int = 3) -> dict | None:
     """The reference g++ build (oracle/_ref/ref_driver) or, when it was not
-    built, the C oracle: 256 decoders x 1 pass of the same workload."""
-    inst_path = os.path.join(REPO, "_bench_multi.txt")
+    built, the C oracle: n decoders x `passes` passes of the same workload
+    (about 20 CPU-seconds at the default size)."""
+    inst_path = os.path.join(REPO, f"_bench_multi_{os.getpid()}.txt")
     with open(inst_path, "w") as f:
         f.write(text)
     ref = os.path.join(REPO, "oracle", "_ref", "ref_driver")
     port = os.path.join(REPO, "oracle", "build", "plp_oracle")
-    n = 256
     try:
         if os.path.exists(ref):
-            cmd = [ref, "bench", inst_path, "--K", str(n), "--passes", "1", "--seed", "1",
+            cmd = [ref, "bench", inst_path, "--K", str(n), "--passes", str(passes), "--seed", "1",
                    "--eps", str(EPS), "--theta", str(THETA), "--threads", str(threads)]
             kind = "reference"
         else:
             if not os.path.exists(port):
                 subprocess.run(["make", "-C", os.path.join(REPO, "oracle")], check=True,
                                capture_output=True)
-            cmd = [port, inst_path, "--K", str(n), "--passes", "1", "--seed", "1", "--eps",
+            cmd = [port, inst_path, "--K", str(n), "--passes", str(passes), "--seed", "1", "--eps",
                    str(EPS), "--theta", str(THETA), "--threads", str(threads), "--bench"]
             kind = "port"
         out = json.loads(subprocess.run(cmd, check=True, capture_output=True, text=True,
@@ -73,8 +73,71 @@ def cpu_baseline(text: str, threads: int) -> dict | None:
         os.remove(inst_path)
     return {"value": round(out["packings_per_s"], 3), "unit": "packings/s", "cores": threads,
             "kind": kind,
-            "sample": f"{n} decoders (seeds 1..{n}) x 1 pass of the same 20-type instance, "
-                      f"eps {EPS}, theta {THETA}; {out['seconds']:.1f} s wall on {threads} threads"}
+            "sample": f"{n} decoders (seeds 1..{n}) x {passes} passes of the same 20-type "
+                      f"instance, eps {EPS}, theta {THETA}; {out['seconds']:.2f} s wall on "
+                      f"{threads} threads"}
+
+
+def serve(plp_amd, synth, seconds: float, rate: float, passes: int, max_inflight: int,
+          seed: int = 2031) -> dict:
+    """Poisson arrivals at `rate` req/s for `seconds`; each request is one
+    instance (types U{3..100}, dims U[25,115]) with log-uniform 256..4096
+    decoders.  Latency = completion time - arrival time."""
+    import math
+    import random
+    rng = random.Random(seed)
+    t, arrivals = 0.0, []
+    while True:
+        t += rng.expovariate(rate)
+        if t >= seconds:
+            break
+        n_dec = int(round(math.exp(rng.uniform(math.log(256), math.log(4096)))))
+        arrivals.append((t, rng.randrange(1 << 30), n_dec))
+    log(f"serve: {len(arrivals)} requests over {seconds:.0f} s at {rate:.0f}/s")
+    insts = [plp_amd.Instance.from_text(synth.mixed_instance(s)) for (_, s, _) in arrivals]
+    prm = plp_amd.params(eps=EPS, theta=THETA)
+    srv = plp_amd.Server(0, max_inflight=max_inflight)
+    # warm every slot's buffers with the largest request shape
+    big = max(range(len(arrivals)), key=lambda i: arrivals[i][2] * insts[i].n_items)
+    warm = []
+    for _ in range(max_inflight):
+        rid = srv.submit(insts[big], arrivals[big][2], 1, passes, prm)
+        if rid is not None:
+            warm.append(rid)
+    while warm:
+        warm = [r for r in warm if srv.poll(r) is None]
+    pending, inflight, lat, done_dec = [], {}, [], 0
+    nxt = 0
+    t0 = time.perf_counter()
+    while nxt < len(arrivals) or pending or inflight:
+        now = time.perf_counter() - t0
+        while nxt < len(arrivals) and arrivals[nxt][0] <= now:
+            pending.append(nxt)
+            nxt += 1
+        while pending:
+            i = pending[0]
+            rid = srv.submit(insts[i], arrivals[i][2], 1 + i, passes, prm)
+            if rid is None:
+                break
+            inflight[rid] = i
+            pending.pop(0)
+        for rid in list(inflight):
+            res = srv.poll(rid)
+            if res is not None:
+                i = inflight.pop(rid)
+                lat.append(time.perf_counter() - t0 - arrivals[i][0])
+                done_dec += arrivals[i][2]
+        if not inflight and not pending and nxt < len(arrivals):
+            time.sleep(max(0.0, min(0.002, arrivals[nxt][0] - (time.perf_counter() - t0))))
+    wall = time.perf_counter() - t0
+    srv.close()
+    lat.sort()
+    q = lambda f: lat[min(len(lat) - 1, int(f * len(lat)))]  # noqa: E731
+    return {"requests": len(lat), "rate_req_per_s": rate, "seconds": round(wall, 2),
+            "passes_per_decoder": passes, "max_inflight": max_inflight,
+            "packings_per_s": round(done_dec * passes / wall, 1),
+            "p50_ms": round(1e3 * q(0.50), 2), "p99_ms": round(1e3 * q(0.99), 2),
+            "max_ms": round(1e3 * lat[-1], 2)}
 
 
 def main() -> int:
@@ -84,6 +147,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="length of the Poisson serving run in extras (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-inflight", type=int, default=8)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration)")
     a = ap.parse_args()
@@ -167,8 +234,11 @@ def main() -> int:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads")
         line["cpu_baseline"] = cpu_baseline(text, threads)
+        single = cpu_baseline(text, 1, n=64, passes=1)
+        if single is not None:
+            line.setdefault("extras", {})["cpu_single_core"] = single
     if not a.no_extras:
-        extras = {}
+        extras = line.get("extras", {})
         # Plain one-step-per-launch form, one decoder per launch sequence.
         n_plain = 8
         t0 = time.perf_counter()
@@ -191,6 +261,9 @@ def main() -> int:
         extras["batch_256x16_packings_per_s"] = round(3 * 4096 / (time.perf_counter() - t0), 1)
         extras["batch_256x16_kernel_ms"] = round(e2.last_ms(), 3)
         e2.close()
+        if a.serve_seconds > 0:
+            extras["serving"] = serve(plp_amd, synth, a.serve_seconds, a.serve_rate, 1,
+                                      a.serve_inflight)
         line["extras"] = extras
     print(json.dumps(line), flush=True)
     eng.close()

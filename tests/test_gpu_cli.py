@@ -1,0 +1,93 @@
+"""The drop-in `plp_gpu` command line on the GPU: the reference's parity
+workflow (parity/compare_parity.py: every JSON field but time_s equal) and
+its --batch / --device envelopes, --rngdump and --selftest."""
+import json
+import os
+import subprocess
+
+import pytest
+
+from conftest import REPO, golden_path
+from oracle_lib import Oracle, load_text
+
+pytestmark = pytest.mark.gpu
+
+CLI = os.path.join(REPO, "plp-gpu_amd", "build", "plp_gpu")
+NAMES = {"BR1": "data/instances/BR1_050.txt", "BR2": "data/instances/BR2_050.txt",
+         "BR3": "data/instances/BR3_050.txt", "SYN1": "data/synthetic/SYN1.txt"}
+
+
+def run(args, timeout=120):
+    p = subprocess.run([CLI] + args, capture_output=True, text=True, timeout=timeout)
+    assert p.returncode == 0, p.stderr
+    return p.stdout
+
+
+def strip(d):
+    d = dict(d)
+    d.pop("time_s")
+    return d
+
+
+@pytest.mark.parametrize("mode", [[], ["--plain"]])
+@pytest.mark.parametrize("name", sorted(NAMES))
+def test_cli_parity_all_match(name, mode):
+    out = run([golden_path(NAMES[name]), "--single-pass", "--eps", "0", "--no-theta", "--json"] + mode)
+    got = json.loads(out)
+    assert got["time_s"] >= 0
+    for kind in ("ref_cpu", "ref_host", "ref_gpu"):
+        with open(golden_path("parity", kind, name + ".json")) as f:
+            assert strip(got) == strip(json.load(f)), (name, kind)
+
+
+def test_cli_device_envelope_rows_match_oracle():
+    orc = Oracle()
+    inst = orc.parse(load_text(NAMES["BR2"]))[0]
+    out = json.loads(run([golden_path(NAMES["BR2"]), "--device", "40", "--seed", "5", "--passes",
+                          "2", "--eps", "0.3", "--json"]))
+    assert out["K"] == 40 and out["n_passes"] == 2 and len(out["rows"]) == 40
+    for k, row in enumerate(out["rows"]):
+        want = orc.run(inst, seed=5 + k, passes=2, eps=0.3, theta=0.9)
+        assert {x: row[x] for x in ("n_boxes", "volume_cm3", "stability_score", "fallback",
+                                    "n_passes", "seed", "placements")} == \
+            {x: want[x] for x in ("n_boxes", "volume_cm3", "stability_score", "fallback",
+                                  "n_passes", "seed", "placements")}, k
+
+
+def test_cli_batch_equals_device_rows():
+    args = [golden_path(NAMES["BR3"]), "--seed", "11", "--passes", "2", "--eps", "0.3", "--json"]
+    a = json.loads(run(args + ["--batch", "5"]))
+    b = json.loads(run(args + ["--device", "5"]))
+    assert [strip(r) for r in a["rows"]] == [strip(r) for r in b["rows"]]
+
+
+def test_cli_time_budget_runs_passes():
+    out = json.loads(run([golden_path(NAMES["SYN1"]), "--time", "0.2", "--eps", "0.3", "--json"]))
+    assert out["n_passes"] >= 1 and out["n_boxes"] > 0
+
+
+def test_cli_rngdump_passes():
+    out = run(["--rngdump", "8"])
+    assert out.strip().splitlines()[-1] == "RNGDUMP: PASS"
+    assert out.count(": PASS") == 9
+
+
+def test_cli_mu_env_and_flag_agree():
+    base = [golden_path(NAMES["BR1"]), "--single-pass", "--eps", "0.3", "--seed", "3", "--json"]
+    a = json.loads(run(base + ["--mu", "0.5"]))
+    env = dict(os.environ, PLP_MU="0.5")
+    p = subprocess.run([CLI] + base, capture_output=True, text=True, env=env, timeout=120)
+    assert p.returncode == 0
+    b = json.loads(p.stdout)
+    assert strip(a) == strip(b)
+    orc = Oracle()
+    want = orc.run(orc.parse(load_text(NAMES["BR1"]))[0], seed=3, eps=0.3, theta=0.9, mu=0.5)
+    assert a["stability_score"] == want["stability_score"]
+
+
+def test_cli_range_error_exit_code(tmp_path):
+    big = tmp_path / "big.txt"
+    big.write_text("1\n1 0\n2000 2000 2000\n1\n1 10 1 10 1 10 1 1\n")
+    p = subprocess.run([CLI, str(big), "--single-pass", "--json"], capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode == 2

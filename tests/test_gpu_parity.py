@@ -318,3 +318,28 @@ def test_plain_form_reference_cases(plp, ctx, ref_runs):
         a = c["args"]
         row, pl = ctx.plain_run(inst_from(plp, c["instance"]), a["seed"], a["passes"], prm(plp, a))
         assert plp.to_json_dict(row, pl) == ref_fields(c["result"]), c["case"]
+
+
+def test_engine_fsr_beyond_reference_cap(plp, ctx, orc):
+    """300 stacked boxes: FSR over more boxes than the reference device
+    kernel's cap of 256 (plp_gpu.cu:1121-1146), bit-exact against the oracle
+    and the closed form (mu / 0.3) * (n - i)."""
+    n = 300
+    text = f"1\n1 0\n10 10 {10 * n}\n1\n1 10 1 10 1 10 1 {n}\n"
+    inst = plp.Instance.from_text(text)
+    eng = plp.Engine(ctx)
+    eng.load([inst], [0], [0], plp.params(eps=0.0, use_theta=False, mu=0.3))
+    eng.run(1)
+    eng.sync()
+    rows, pl = eng.fetch()
+    want = orc.run(orc.parse(text)[0], seed=0, eps=0.0, theta=None, mu=0.3)
+    got = plp.to_json_dict(rows[0], pl[0])
+    assert got == {k: want[k] for k in got}
+    assert rows[0].n_boxes == n
+    placed7 = [[r[1], r[2], r[3], r[4], r[5], r[6], r[0]] for r in pl[0][:n]]
+    for mu in (0.3, 0.2):
+        fsr, score = ctx.fsr_eval([pl[0][:n]], mu=mu)
+        assert list(fsr[0]) == list(orc.compute_fsr(placed7, mu))            # bit-exact
+        assert list(fsr[0]) == pytest.approx([mu / 0.3 * (n - i) for i in range(n)], rel=1e-12)
+        assert score[0] == orc.stability_score(placed7, mu)
+    assert rows[0].score == orc.stability_score(placed7, 0.3)
