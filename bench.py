@@ -147,6 +147,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--shape", default="",
+                    help="fixed launch shape WPD,EP_CAP,BLOCKS_PER_CU instead of autotuning")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the Poisson serving run in extras (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -174,12 +176,19 @@ def main() -> int:
     text = synth.multi_instance()
     inst = plp_amd.Instance.from_text(text)
     ctx = plp_amd.Ctx(0)
-    eng = plp_amd.Engine(ctx, use_graph=True)
     prm = plp_amd.params(eps=EPS, theta=THETA)
-    eng.load([inst], [0] * K_DEC, list(range(1, K_DEC + 1)), prm)
-    log("tuning launch shape")
-    tuned, tune_ms = eng.autotune(1)
-    log(f"autotune: {tuned} ({tune_ms:.2f} ms/pass)")
+    if a.shape:
+        wpd, cap, bpc = (int(v) for v in a.shape.split(","))
+        eng = plp_amd.Engine(ctx, ep_cap=cap, blocks_per_cu=bpc, use_graph=True,
+                             waves_per_decoder=wpd)
+        eng.load([inst], [0] * K_DEC, list(range(1, K_DEC + 1)), prm)
+        tuned = {"waves_per_decoder": wpd, "ep_cap": cap, "blocks_per_cu": bpc, "fixed": True}
+    else:
+        eng = plp_amd.Engine(ctx, use_graph=True)
+        eng.load([inst], [0] * K_DEC, list(range(1, K_DEC + 1)), prm)
+        log("tuning launch shape")
+        tuned, tune_ms = eng.autotune(1)
+        log(f"autotune: {tuned} ({tune_ms:.2f} ms/pass)")
     for _ in range(a.warmup):
         eng.reset()
         eng.run(PASSES)

@@ -69,7 +69,7 @@ struct DecodeArgs {
 };
 
 // Dynamic LDS layout of one decoder block.  The pass-time region (extreme
-// points, candidate list, pair masks, per-wave culled box lists) is reused by
+// points, candidate list, pair masks, per-wave culled box copies) is reused by
 // FSR after the pass.  The box array has box_cap slots plus one sentinel.
 struct LdsLayout {
     int32_t off_ep, off_tmp, off_act, off_mask, n_mask, off_rel;
@@ -91,8 +91,8 @@ __host__ __device__ inline LdsLayout lds_layout(int32_t ep_cap, int32_t box_cap,
     l.off_act = 2 * ep_cap * 4;
     l.off_mask = (3 * ep_cap * 4 + 7) & ~7;
     l.n_mask = (ep_cap * 6 + 63) / 64;
-    l.off_rel = l.off_mask + l.n_mask * 8;
-    int32_t pass_end = l.off_rel + n_waves * (box_cap + 8) * 4;
+    l.off_rel = (l.off_mask + l.n_mask * 8 + 15) & ~15;
+    int32_t pass_end = l.off_rel + n_waves * (64 + 8) * 32;   // per-wave culled window
     l.n_key = next_pow2(box_cap > 0 ? box_cap : 1);
     l.off_load = 0;
     l.off_key = box_cap * 8;

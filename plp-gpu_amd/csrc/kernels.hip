@@ -44,69 +44,71 @@ __device__ __forceinline__ uint32_t xyz_key(int x, int y, int z, int xb, int yb)
 //       infeasible iff (double)s / (double)(w*d) < 0.5
 //   C1  no strict-interior overlap with any box
 // The verdict does not depend on which boxes are visited in what order as
-// long as every box that can overlap or support the pair is visited, so the
-// wave visits only the boxes in rel[] (the chunk's culled list, newest first,
-// padded with the sentinel slot to a multiple of kBoxGroup).  A group's LDS
-// reads issue together; the wave leaves the scan once every lane has failed,
-// and skips the support arithmetic of a box whose top is at no live lane's
-// level (both wave-uniform branches).  Areas fit 32 bits: L*W < 2^30 is a
-// load-time precondition, and ox, oy < 2^21 so the 24-bit multiply is exact.
+// long as every box that can overlap or support the pair is visited.  The
+// wave walks the placed boxes newest first in windows of 64 (one box per
+// lane), keeps the ones that intersect the chunk's region
+// [xlo,xhi) x [ylo,yhi) x [zlo,zhi) or whose top lies in [zlo, zhi) over that
+// footprint (a box outside it can neither overlap nor support any pair of
+// the chunk), copies them into the wave's cbox[] window, pads it with
+// sentinels to a multiple of kBoxGroup and scans it in groups whose LDS reads
+// issue together.  The wave leaves as soon as every lane has failed, and
+// skips the support arithmetic of a box whose top is at no live lane's level
+// (both wave-uniform branches).  Areas fit 32 bits: L*W < 2^30 is a load-time
+// precondition, and ox, oy < 2^21 so the 24-bit multiply is exact.
 constexpr int kBoxGroup = 8;
+constexpr int kCullWindow = kWave + kBoxGroup;   // cbox[] entries per wave
+
+__device__ __forceinline__ Box sentinel_box() {
+    Box b;
+    b.x = b.y = b.z = b.x2 = b.y2 = b.z2 = -1;   // never overlaps, never supports
+    b.type = 0;
+    b.pad = 0;
+    return b;
+}
 
 __device__ __forceinline__ bool pair_feasible(int x, int y, int z, int w, int d, int h,
                                               bool alive, const Box* __restrict__ boxes,
-                                              const uint32_t* __restrict__ rel, int n_rel) {
+                                              int n_box, int xlo, int xhi, int ylo, int yhi,
+                                              int zlo, int zhi, Box* __restrict__ cbox) {
+    const int lane = lane_id();
     const int x2 = x + w, y2 = y + d, z2 = z + h;
     uint32_t s = 0;
-    for (int g = 0; g < n_rel; g += kBoxGroup) {
+    for (int top = n_box - 1; top >= 0; top -= kWave) {
         if (ballot(alive) == 0) break;
-        Box B[kBoxGroup];
+        const int b = top - lane;
+        Box C = sentinel_box();
+        if (b >= 0) C = boxes[b];
+        const bool r = (b >= 0) & (C.x < xhi) & (C.x2 > xlo) & (C.y < yhi) & (C.y2 > ylo) &
+                       (C.z < zhi) & (C.z2 >= zlo);
+        const uint64_t rm = ballot(r);
+        const int n_rel = __popcll(rm);
+        if (n_rel == 0) continue;
+        if (r) cbox[rank_below(rm)] = C;
+        const int n_pad = (n_rel + kBoxGroup - 1) / kBoxGroup * kBoxGroup;
+        if (lane < n_pad - n_rel) cbox[n_rel + lane] = sentinel_box();
+        wave_sync();
+        for (int g = 0; g < n_pad; g += kBoxGroup) {
+            Box B[kBoxGroup];
 #pragma unroll
-        for (int u = 0; u < kBoxGroup; u++) B[u] = boxes[rel[g + u]];
+            for (int u = 0; u < kBoxGroup; u++) B[u] = cbox[g + u];
 #pragma unroll
-        for (int u = 0; u < kBoxGroup; u++) {
-            const bool ov = (x < B[u].x2) & (B[u].x < x2) & (y < B[u].y2) & (B[u].y < y2) &
-                            (z < B[u].z2) & (B[u].z < z2);
-            alive = alive & !ov;
-            if (ballot(alive & (B[u].z2 == z))) {
-                const int ox = min(x2, B[u].x2) - max(x, B[u].x);
-                const int oy = min(y2, B[u].y2) - max(y, B[u].y);
-                const bool sup = (B[u].z2 == z) & (ox > 0) & (oy > 0);
-                s += sup ? __umul24((uint32_t)ox, (uint32_t)oy) : 0u;
+            for (int u = 0; u < kBoxGroup; u++) {
+                const bool ov = (x < B[u].x2) & (B[u].x < x2) & (y < B[u].y2) & (B[u].y < y2) &
+                                (z < B[u].z2) & (B[u].z < z2);
+                alive = alive & !ov;
+                if (ballot(alive & (B[u].z2 == z))) {
+                    const int ox = min(x2, B[u].x2) - max(x, B[u].x);
+                    const int oy = min(y2, B[u].y2) - max(y, B[u].y);
+                    const bool sup = (B[u].z2 == z) & (ox > 0) & (oy > 0);
+                    s += sup ? __umul24((uint32_t)ox, (uint32_t)oy) : 0u;
+                }
             }
+            if (ballot(alive) == 0) break;
         }
+        wave_sync();   // the next window overwrites cbox[]
     }
     if (alive && z > 0) alive = !((double)s / (double)((int64_t)w * d) < 0.5);
     return alive;
-}
-
-// Culled box list of one 64-pair chunk: every placed box that intersects the
-// region [xlo,xhi) x [ylo,yhi) x [zlo,zhi) spanned by the chunk's live pairs
-// or whose top lies in [zlo, zhi) over that footprint (a possible support).
-// A box outside it can neither overlap nor support any pair of the chunk.
-// Written newest first into rel[], padded with box index `sentinel` to a
-// multiple of kBoxGroup; returns the padded length.
-__device__ __forceinline__ int cull_boxes(const Box* __restrict__ boxes, int n_box, int sentinel,
-                                          int xlo, int xhi, int ylo, int yhi, int zlo, int zhi,
-                                          uint32_t* __restrict__ rel) {
-    const int lane = lane_id();
-    int n_rel = 0;
-    for (int top = n_box - 1; top >= 0; top -= kWave) {
-        const int b = top - lane;
-        bool r = false;
-        if (b >= 0) {
-            const Box B = boxes[b];
-            r = (B.x < xhi) & (B.x2 > xlo) & (B.y < yhi) & (B.y2 > ylo) & (B.z < zhi) &
-                (B.z2 >= zlo);
-        }
-        const uint64_t m = ballot(r);
-        if (r) rel[n_rel + rank_below(m)] = (uint32_t)b;
-        n_rel += __popcll(m);
-    }
-    const int n_pad = (n_rel + kBoxGroup - 1) / kBoxGroup * kBoxGroup;
-    if (lane < n_pad - n_rel) rel[n_rel + lane] = (uint32_t)sentinel;
-    wave_sync();
-    return n_pad;
 }
 
 // ---------------------------------------------------------------- FSR ----
@@ -197,16 +199,17 @@ __device__ double fsr_score(const Box* __restrict__ box, int n, double mu, doubl
 
 // ------------------------------------------------------------ decoder ----
 
-// Block-wide scalars of the step in flight.  Wave 0 writes them between
-// barriers; the per-step fields are double-buffered by step parity so a
-// wave still reading step s never sees step s+1's values.
+// Block-wide scalars.  Wave 0 writes quit / n_act / n_box before the barrier
+// that ends a placement step; every wave reads them after it, and the next
+// write comes after the following step's verdict barrier.
 struct StepShared {
     int item;       // queue item of the block
     int quit;       // the decoder stopped (capacity error): leave the pass loop
-    int no[2];      // orientations to evaluate this step (0: nothing to evaluate)
-    int n_act[2];   // live extreme points
-    int n_box[2];   // placed boxes
-    int cnt[4];     // feasible pairs found by each wave
+    int no[2];      // unused
+    int n_act[2];   // live extreme points after the last placement ([0])
+    int n_box[2];   // placed boxes after the last placement ([0])
+    int cnt[2][4];  // feasible pairs found by each wave, by evaluated-step parity:
+                    // a wave may start step k+1 while others still read step k
 };
 
 __device__ __forceinline__ void write_rows(int32_t* dst, const Box* box, int n) {
@@ -243,7 +246,7 @@ __device__ __forceinline__ void write_rows(int32_t* dst, const Box* box, int n) 
 template <int NW>
 __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ int s_ors[18];
+    __shared__ int s_ors[18 * NW];
     __shared__ StepShared sh;
     const LdsLayout lay = lds_layout(a.ep_cap, a.box_cap, a.use_rng, NW);
     uint32_t* ep = (uint32_t*)(lds + lay.off_ep);
@@ -256,8 +259,8 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
     uint32_t* mt = (uint32_t*)(lds + lay.off_mt);
     const int lane = lane_id();
     const int wid = (int)(threadIdx.x >> 6);
-    uint32_t* rel = (uint32_t*)(lds + lay.off_rel) + wid * (a.box_cap + 8);
-    const int sentinel = a.box_cap;    // box slot that always holds a sentinel
+    Box* cbox = (Box*)(lds + lay.off_rel) + wid * kCullWindow;   // this wave's culled window
+    int* s_orw = s_ors + 18 * wid;                                   // this wave's orientations
 
     for (;;) {
         if (threadIdx.x == 0) sh.item = (int)atomicAdd(a.work_counter, 1u);
@@ -315,48 +318,32 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                     ep[0] = 0u;     // origin; L, W, H > 0 so it is live
                     act[0] = 0u;
                 }
-                for (int i = lane; i < a.box_cap + 8; i += kWave) {
-                    Box sb;
-                    sb.x = sb.y = sb.z = sb.x2 = sb.y2 = sb.z2 = -1;
-                    sb.type = 0;
-                    sb.pad = 0;
-                    box[i] = sb;
-                }
+                for (int i = lane; i < a.box_cap + 8; i += kWave) box[i] = sentinel_box();
             }
             block_sync();
+            // Every wave follows the step sequence itself: the memo decision
+            // needs only the type, the last failure and whether a placement
+            // happened, which all waves learn from the shared counts.
+            int nb_w = 0, na_w = 1;
+            int ev = 0;   // evaluated steps so far (same count in every wave)
             for (int s = 0; s < I.n_steps; s++) {
-                const int par = s & 1;
                 const int ti = uni(steps[s]);
-                // ---- step prologue (wave 0)
+                const int32_t* orow = a.ors_tab + (size_t)(I.ors_off + ti) * kOrsRow;
+                const int no = uni(orow[0]);
+                if (no == 0) continue;      // type with no allowed orientation
                 if (wid == 0) {
-                    const int32_t* orow = a.ors_tab + (size_t)(I.ors_off + ti) * kOrsRow;
-                    const int no = uni(orow[0]);
-                    int eval = 0;
-                    if (no > 0) {
-                        work += (uint64_t)n_ep * (uint64_t)no * (uint64_t)n_box;
-                        n_steps_eval++;
-                        // Same type, no placement since its last copy found
-                        // nothing: the verdicts repeat, nothing is drawn.
-                        if (ti != fail_ti) {
-                            eval = no;
-                            live += (uint64_t)n_act * (uint64_t)no * (uint64_t)n_box;
-                            if (lane < 3 * no) s_ors[lane] = orow[1 + lane];
-                        }
-                    }
-                    if (lane == 0) {
-                        sh.no[par] = eval;
-                        sh.n_act[par] = n_act;
-                        sh.n_box[par] = n_box;
-                    }
+                    work += (uint64_t)n_ep * (uint64_t)no * (uint64_t)n_box;
+                    n_steps_eval++;
                 }
-                block_sync();
-                const int no = uni(sh.no[par]);
-                if (no == 0) continue;
-                const int na = uni(sh.n_act[par]);
-                const int nb = uni(sh.n_box[par]);
+                // Same type, no placement since its last copy found nothing:
+                // the verdicts repeat, nothing is drawn.
+                if (ti == fail_ti) continue;
+                if (wid == 0) live += (uint64_t)n_act * (uint64_t)no * (uint64_t)n_box;
+                if (lane < 3 * no) s_orw[lane] = orow[1 + lane];
+                wave_sync();
 
                 // ---- pair verdicts (every wave)
-                const int n_pairs = na * no;
+                const int n_pairs = na_w * no;
                 const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)no - 1) / (uint64_t)no);
                 const int n_chunks = (n_pairs + kWave - 1) / kWave;
                 int cnt_w = 0;
@@ -368,9 +355,9 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                     int x = 0, y = 0, z = 0, w = 1, dd = 1, h = 1;
                     if (valid) {
                         key_xyz(act[e], xb, yb, x, y, z);
-                        w = s_ors[3 * o];
-                        dd = s_ors[3 * o + 1];
-                        h = s_ors[3 * o + 2];
+                        w = s_orw[3 * o];
+                        dd = s_orw[3 * o + 1];
+                        h = s_orw[3 * o + 2];
                     }
                     const bool alive = valid && (x + w <= L) && (y + dd <= W) && (z + h <= H);
                     uint64_t m = 0;
@@ -381,26 +368,29 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                         const int yhi = wave_max(alive ? y + dd : -1);
                         const int zlo = wave_min(alive ? z : 0x7fffffff);
                         const int zhi = wave_max(alive ? z + h : -1);
-                        const int n_rel = cull_boxes(box, nb, sentinel, xlo, xhi, ylo, yhi, zlo,
-                                                     zhi, rel);
-                        m = ballot(pair_feasible(x, y, z, w, dd, h, alive, box, rel, n_rel));
+                        m = ballot(pair_feasible(x, y, z, w, dd, h, alive, box, nb_w, xlo, xhi,
+                                                 ylo, yhi, zlo, zhi, cbox));
                     }
                     if (lane == 0) mask[c] = m;
                     cnt_w += __popcll(m);
                 }
-                if (lane == 0) sh.cnt[wid] = cnt_w;
+                const int par = ev & 1;
+                ev++;
+                if (lane == 0) sh.cnt[par][wid] = cnt_w;
                 block_sync();
+                int cnt = 0;
+#pragma unroll
+                for (int w2 = 0; w2 < NW; w2++) cnt += sh.cnt[par][w2];
+                cnt = uni(cnt);
+                if (cnt == 0) {
+                    fail_ti = ti;     // this copy is skipped; no RNG use
+                    continue;         // nothing changed: no barrier needed
+                }
+                fail_ti = -1;
 
                 // ---- pick, place, extreme-point merge (wave 0)
                 if (wid == 0) {
-                    int cnt = 0;
-#pragma unroll
-                    for (int w2 = 0; w2 < NW; w2++) cnt += sh.cnt[w2];
-                    cnt = uni(cnt);
-                    if (cnt == 0) {
-                        fail_ti = ti;     // this copy is skipped; no RNG use
-                    } else {
-                        fail_ti = -1;
+                    {
                         // feas[0], or feas[randrange(|feas|)] when random()
                         // >= 1 - eps (plp_common.hpp:427-435).
                         int idx = 0;
@@ -425,8 +415,8 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                         x = uni(x);
                         y = uni(y);
                         z = uni(z);
-                        const int w = uni(s_ors[3 * o]), dd = uni(s_ors[3 * o + 1]),
-                                  h = uni(s_ors[3 * o + 2]);
+                        const int w = uni(s_orw[3 * o]), dd = uni(s_orw[3 * o + 1]),
+                                  h = uni(s_orw[3 * o + 2]);
                         const int X2 = x + w, Y2 = y + dd, Z2 = z + h;
                         if (lane == 0) {
                             Box nbx;
@@ -536,11 +526,17 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                             n_act = uni(base);
                             wave_sync();
                         }
-                        if (lane == 0) sh.quit = st.err != 0;
+                        if (lane == 0) {
+                            sh.quit = st.err != 0;
+                            sh.n_act[0] = n_act;
+                            sh.n_box[0] = n_box;
+                        }
                     }
                 }
                 block_sync();
                 if (uni(sh.quit) != 0) break;
+                na_w = uni(sh.n_act[0]);
+                nb_w = uni(sh.n_box[0]);
             }
 
             // ---- pass end (wave 0): stability score and D4' selection

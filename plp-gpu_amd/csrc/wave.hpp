@@ -67,15 +67,22 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, kWave));
-    return uni(v);
+// Wave-wide min / max through DPP lane moves (VALU latency, no LDS round
+// trip): quad swaps, half-row and row mirrors reduce each 16-lane row, then
+// row_bcast:15 and row_bcast:31 fold the rows into lane 63.  Rows a DPP step
+// does not write keep their own value (old = v), which min/max leaves intact.
+template <bool kMax>
+__device__ __forceinline__ int wave_reduce_dpp(int v) {
+    auto op = [](int a, int b) { return kMax ? max(a, b) : min(a, b); };
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));  // row_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
 }
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
-    return uni(v);
-}
+__device__ __forceinline__ int wave_min(int v) { return wave_reduce_dpp<false>(v); }
+__device__ __forceinline__ int wave_max(int v) { return wave_reduce_dpp<true>(v); }
 
 }  // namespace plpk
