@@ -159,9 +159,10 @@ extern "C" int plp_engine_create(plp_ctx* ctx, const plp_engine_config* cfg, plp
     g->ctx = ctx;
     if (cfg) g->cfg = *cfg;
     const int wq = g->cfg.waves_per_decoder;
-    if ((wq != 0 && wq != 1 && wq != 2 && wq != 4) || g->cfg.ep_cap < 0 || g->cfg.blocks_per_cu < 0) {
+    if ((wq != 0 && wq != 1 && wq != 2 && wq != 4 && wq != 8) || g->cfg.ep_cap < 0 ||
+        g->cfg.blocks_per_cu < 0) {
         delete g;
-        return fail(PLP_E_ARG, "config: waves_per_decoder must be 0, 1, 2 or 4; caps >= 0");
+        return fail(PLP_E_ARG, "config: waves_per_decoder must be 0, 1, 2, 4 or 8; caps >= 0");
     }
     g->wpd = wq > 0 ? wq : 1;
     if ((e = hipEventCreate(&g->ev0)) != hipSuccess || (e = hipEventCreate(&g->ev1)) != hipSuccess) {
@@ -473,7 +474,7 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
     float bms = -1.0f;
     plp_engine_config bcfg = saved;
     const int saved_wpd = g->wpd;
-    for (int wpd : {1, 2, 4}) {
+    for (int wpd : {1, 2, 4, 8}) {
         if (saved.waves_per_decoder != 0 && wpd != saved.waves_per_decoder) continue;
         g->wpd = wpd;
         for (int cap : caps) {

@@ -55,7 +55,7 @@ __device__ __forceinline__ uint32_t xyz_key(int x, int y, int z, int xb, int yb)
 // skips the support arithmetic of a box whose top is at no live lane's level
 // (both wave-uniform branches).  Areas fit 32 bits: L*W < 2^30 is a load-time
 // precondition, and ox, oy < 2^21 so the 24-bit multiply is exact.
-constexpr int kBoxGroup = 8;
+constexpr int kBoxGroup = 4;
 constexpr int kCullWindow = kWave + kBoxGroup;   // cbox[] entries per wave
 
 __device__ __forceinline__ Box sentinel_box() {
@@ -208,7 +208,7 @@ struct StepShared {
     int no[2];      // unused
     int n_act[2];   // live extreme points after the last placement ([0])
     int n_box[2];   // placed boxes after the last placement ([0])
-    int cnt[2][4];  // feasible pairs found by each wave, by evaluated-step parity:
+    int cnt[2][8];  // feasible pairs found by each wave, by evaluated-step parity:
                     // a wave may start step k+1 while others still read step k
 };
 
@@ -696,6 +696,7 @@ hipError_t launch_decode(const DecodeArgs& a, int waves, int grid, int lds_bytes
         case 1: hipLaunchKernelGGL(decode_kernel<1>, dim3(grid), dim3(64), (size_t)lds_bytes, st, a); break;
         case 2: hipLaunchKernelGGL(decode_kernel<2>, dim3(grid), dim3(128), (size_t)lds_bytes, st, a); break;
         case 4: hipLaunchKernelGGL(decode_kernel<4>, dim3(grid), dim3(256), (size_t)lds_bytes, st, a); break;
+        case 8: hipLaunchKernelGGL(decode_kernel<8>, dim3(grid), dim3(512), (size_t)lds_bytes, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -730,6 +731,7 @@ hipError_t decode_occupancy(int waves, int lds_bytes, int* blocks_per_cu) {
         case 1: return occupancy_of<1>(lds_bytes, blocks_per_cu);
         case 2: return occupancy_of<2>(lds_bytes, blocks_per_cu);
         case 4: return occupancy_of<4>(lds_bytes, blocks_per_cu);
+        case 8: return occupancy_of<8>(lds_bytes, blocks_per_cu);
         default: return hipErrorInvalidValue;
     }
 }

@@ -13,7 +13,8 @@ import weakref
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "build", "libplp_hip.so")
+LIB_PATH = os.environ.get("PLP_HIP_LIB") or os.path.join(os.path.dirname(_HERE), "build",
+                                                          "libplp_hip.so")
 CLI_PATH = os.path.join(os.path.dirname(_HERE), "build", "plp_gpu")
 
 if not os.path.exists(LIB_PATH):
