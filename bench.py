@@ -12,7 +12,8 @@ the per-decoder error flags.
 Printed: one JSON line with the contract fields plus
   roofline      dominant kernel (decode_kernel) against the VALU issue roof
   cpu_baseline  the reference's g++ build (or the C oracle) on host cores
-  extras        plain one-step-per-launch GPU form, the 256-instance batch
+  extras        plain one-step-per-launch GPU form, the 256-instance batch,
+                the 100-type / 2048-decoder instance, Poisson serving
 usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu] [--no-extras]
 """
 from __future__ import annotations
@@ -30,23 +31,25 @@ sys.path.insert(0, os.path.join(REPO, "plp-gpu_amd"))
 K_DEC = 1024
 PASSES = 4
 EPS, THETA = 0.01, 0.9
-# Lane-ops of one brute-force AABB test of the reference predicate
-# (plp_common.hpp:297-321): C1 6 compares + 5 ands, C3 1 compare + 2 min +
-# 2 max + 2 sub + 1 mul + 1 add.
-OPS_PER_TEST = 20
-# MI355X: 256 CUs x 64 lanes x 2.4 GHz VALU issue (= FP64 vector 78.6 TFLOP/s
-# counting an FMA as two flops).
+# Algorithmic unit: one pair x box test of the reference predicate
+# (plp_common.hpp:297-321) over every (candidate, orientation, placed box) the
+# reference's check_all receives (plp_common.hpp:418), at 10 lane-ops per test
+# (C1 6 compares + ANDs; C3 1 compare + 2 min + 2 max + 2 sub + 1 mul + 1 add).
+OPS_PER_TEST = 10
+# MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz VALU issue = 39.3 T lane-ops/s
+# (= FP64 vector 78.6 TFLOP/s counting an FMA as two flops); HBM3E 8 TB/s.
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+HBM_PEAK_GBS = 8000.0
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(text: str, threads: int, n: int = 1024, passes: int = 3) -> dict | None:
+def cpu_baseline(text: str, threads: int, n: int, passes: int) -> dict | None:
     """The reference g++ build (oracle/_ref/ref_driver) or, when it was not
     built, the C oracle: n decoders x `passes` passes of the same workload
-    (about 20 CPU-seconds at the default size)."""
+    (1024 x 12 on 16 threads and 64 x 4 on one are each about 5-10 s)."""
     inst_path = os.path.join(REPO, f"_bench_multi_{os.getpid()}.txt")
     with open(inst_path, "w") as f:
         f.write(text)
@@ -187,7 +190,7 @@ def main() -> int:
         eng = plp_amd.Engine(ctx, use_graph=True)
         eng.load([inst], [0] * K_DEC, list(range(1, K_DEC + 1)), prm)
         log("tuning launch shape")
-        tuned, tune_ms = eng.autotune(1)
+        tuned, tune_ms = eng.autotune(PASSES)
         log(f"autotune: {tuned} ({tune_ms:.2f} ms/pass)")
     for _ in range(a.warmup):
         eng.reset()
@@ -203,18 +206,35 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     rows, _ = eng.fetch(placements=False)
     assert all(r.err == 0 for r in rows)
-    work = sum(r.work_pair_box for r in rows)          # brute-force pair-box tests of one step
     live = sum(r.work_live_pair_box for r in rows)     # the tests this algorithm needs
     shape = eng.describe()
+    # The reference's pair x box tests of the same step, from one untimed run
+    # of an engine that also keeps the dead extreme points (same results).
+    ec = plp_amd.Engine(ctx, waves_per_decoder=shape["waves_per_decoder"], count_work=True)
+    ec.load([inst], [0] * K_DEC, list(range(1, K_DEC + 1)), prm)
+    ec.run(PASSES)
+    ec.sync()
+    crows, _ = ec.fetch(placements=False)
+    ec.close()
+    assert [(r.n_boxes, r.volume, r.score) for r in crows] == \
+        [(r.n_boxes, r.volume, r.score) for r in rows]
+    work = sum(r.work_pair_box for r in crows)
     packings = K_DEC * PASSES * a.steps
     value = packings / elapsed
     ms_launch = sum(kernel_ms) / len(kernel_ms)
-    achieved = live * OPS_PER_TEST / (ms_launch * 1e-3) / 1e12
-    traffic = None
-    tfile = os.path.join(REPO, "profiles", "decode_kernel_hbm_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+    achieved = work * OPS_PER_TEST / (ms_launch * 1e-3) / 1e12
+    # Per-launch hardware counters of the same launch shape, from the
+    # committed rocprofv3 summary (FETCH_SIZE / WRITE_SIZE scaled by the copy
+    # calibration; SQ_INSTS_VALU wave instructions x 64 lanes).
+    prof = {}
+    pfile = os.path.join(REPO, "profiles", "decode_kernel_counters.json")
+    if os.path.exists(pfile):
+        with open(pfile) as f:
+            prof = json.load(f)
+    traffic = prof.get("hbm_bytes_per_launch")
+    executed = None
+    if prof.get("valu_insts_per_launch"):
+        executed = prof["valu_insts_per_launch"] * 64 / (ms_launch * 1e-3) / 1e12
     line = {
         "metric": "decoded packings/s at bit-parity vs plain GPU & g++ host; p50/p99 latency",
         "value": round(value, 1),
@@ -236,14 +256,20 @@ def main() -> int:
                      "unit": "Tlane-op/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
                      "traffic": traffic,
                      "kernel": "decode_kernel", "kernel_ms": round(ms_launch, 3),
+                     "reference_pair_box_tests_per_launch": work, "ops_per_test": OPS_PER_TEST,
                      "live_pair_box_tests_per_launch": live,
-                     "reference_pair_box_tests_per_launch": work, "ops_per_test": OPS_PER_TEST},
+                     "executed_valu_tops": None if executed is None else round(executed, 3),
+                     "executed_valu_frac": None if executed is None
+                     else round(executed / VALU_PEAK_TOPS, 4),
+                     "hbm_frac": None if traffic is None
+                     else round(traffic / (ms_launch * 1e-3) / 1e9 / HBM_PEAK_GBS, 6),
+                     "counters_from": prof.get("source")},
     }
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads")
-        line["cpu_baseline"] = cpu_baseline(text, threads)
-        single = cpu_baseline(text, 1, n=64, passes=1)
+        line["cpu_baseline"] = cpu_baseline(text, threads, n=K_DEC, passes=12)
+        single = cpu_baseline(text, 1, n=64, passes=4)
         if single is not None:
             line.setdefault("extras", {})["cpu_single_core"] = single
     if not a.no_extras:
@@ -270,6 +296,20 @@ def main() -> int:
         extras["batch_256x16_packings_per_s"] = round(3 * 4096 / (time.perf_counter() - t0), 1)
         extras["batch_256x16_kernel_ms"] = round(e2.last_ms(), 3)
         e2.close()
+        # 100 types, ~600 boxes, 2048 decoders, one pass.
+        e3 = plp_amd.Engine(ctx, use_graph=True)
+        e3.load([plp_amd.Instance.from_text(synth.large_instance())], [0] * 2048,
+                list(range(1, 2049)), prm)
+        e3.run(1)
+        e3.sync()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            e3.reset()
+            e3.run(1)
+            e3.sync()
+        extras["large_100types_2048_packings_per_s"] = round(3 * 2048 / (time.perf_counter() - t0), 1)
+        extras["large_100types_2048_kernel_ms"] = round(e3.last_ms(), 3)
+        e3.close()
         if a.serve_seconds > 0:
             extras["serving"] = serve(plp_amd, synth, a.serve_seconds, a.serve_rate, 1,
                                       a.serve_inflight)

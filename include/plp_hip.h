@@ -128,10 +128,13 @@ typedef struct {
 } plp_params;
 
 typedef struct {
-    int32_t waves_per_decoder;   /* wavefronts per decoder block: 1, 2, 4, 8; 0 = automatic (1) */
+    int32_t waves_per_decoder;   /* wavefronts per decoder block: 1, 2, 4, 8; 0 = automatic (4) */
     int32_t ep_cap;              /* extreme-point capacity per decoder, 0 = automatic */
     int32_t blocks_per_cu;       /* persistent blocks per CU, 0 = as many as fit */
     int32_t use_graph;           /* capture the launch sequence in a hipGraph */
+    int32_t count_work;          /* 1: also keep the dead extreme points so that rows report the
+                                    reference's list size (max_ep) and brute-force work
+                                    (work_pair_box); costs a third list of ep_cap entries */
 } plp_engine_config;
 
 typedef struct {
@@ -142,10 +145,11 @@ typedef struct {
     int64_t volume;       /* volume_cm3 */
     double score;         /* stability_score of the selected packing */
     uint64_t seed;
-    uint64_t work_pair_box;  /* sum over steps of n_cand * n_ors * n_boxes (brute-force AABB tests) */
+    uint64_t work_pair_box;  /* sum over steps of n_cand * n_ors * n_boxes (brute-force AABB tests);
+                                engine rows: only with count_work, else 0 */
     uint64_t work_live_pair_box;  /* the same over the steps and live points the kernel evaluates */
-    int32_t max_ep;          /* largest extreme-point list reached */
-    int32_t reserved;
+    int32_t max_ep;          /* largest extreme-point list of the reference (engine: count_work) */
+    int32_t ep_need;         /* engine: the extreme-point capacity this decoder needed */
 } plp_row_result;
 
 typedef struct plp_engine plp_engine;

@@ -29,6 +29,7 @@ struct plp_engine {
     bool loaded = false;
     int n_inst = 0, n_dec = 0, max_items = 1, box_cap = 1;
     int ep_cap = 0, lds = 0, grid = 0, occ = 0, wpd = 1;
+    int wide = 0;   // 32-bit pair-state cache
     plp_params params{};
     int use_rng = 0;
     int passes_done = 0;
@@ -61,7 +62,8 @@ void drop_graph(plp_engine* g) {
 
 // Launch shape for an EP capacity: LDS bytes, resident blocks per CU, grid.
 int shape(plp_engine* g, int ep_cap, int blocks_per_cu_req, int n_work) {
-    const plpk::LdsLayout lay = plpk::lds_layout(ep_cap, g->box_cap, g->use_rng, g->wpd);
+    const plpk::LdsLayout lay = plpk::lds_layout(ep_cap, g->box_cap, g->use_rng, g->wpd,
+                                                 g->cfg.count_work, g->wide);
     if (lay.total + 256 > g->ctx->max_lds)
         return fail(PLP_E_EPCAP, "decoder state (" + std::to_string(lay.total) +
                                      " B of LDS for " + std::to_string(ep_cap) +
@@ -99,6 +101,8 @@ plpk::DecodeArgs make_args(plp_engine* g, int n_passes, int reset, const int32_t
     a.use_rng = g->use_rng;
     a.use_theta = g->params.use_theta;
     a.reset = reset;
+    a.count_work = g->cfg.count_work ? 1 : 0;
+    a.wide_cache = g->wide;
     a.eps = g->params.eps;
     a.theta = g->params.theta;
     a.mu = g->params.mu;
@@ -164,7 +168,7 @@ extern "C" int plp_engine_create(plp_ctx* ctx, const plp_engine_config* cfg, plp
         delete g;
         return fail(PLP_E_ARG, "config: waves_per_decoder must be 0, 1, 2, 4 or 8; caps >= 0");
     }
-    g->wpd = wq > 0 ? wq : 1;
+    g->wpd = wq > 0 ? wq : 4;
     if ((e = hipEventCreate(&g->ev0)) != hipSuccess || (e = hipEventCreate(&g->ev1)) != hipSuccess) {
         delete g;
         return hip_fail(e, "hipEventCreate");
@@ -210,6 +214,7 @@ extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, in
     std::vector<int32_t> ors;
     std::vector<int64_t> cost((size_t)n_inst, 0);
     int max_items = 1;
+    int wide = 0;
     for (int i = 0; i < n_inst; i++) {
         plp::Instance in;
         int rc = to_instance(&insts[i], &in);
@@ -235,6 +240,7 @@ extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, in
                 row[1 + 3 * k] = o[k][0];
                 row[2 + 3 * k] = o[k][1];
                 row[3 + 3 * k] = o[k][2];
+                if ((int64_t)o[k][0] * o[k][1] > plpk::kNarrowMaxArea) wide = 1;
             }
             ors.insert(ors.end(), row, row + plpk::kOrsRow);
         }
@@ -250,6 +256,7 @@ extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, in
     g->n_dec = n_dec;
     g->max_items = max_items;
     g->box_cap = (max_items + 7) / 8 * 8;   // sentinel-padded groups of 8
+    g->wide = wide;
     g->params = *params;
     g->use_rng = (params->first_fit || params->eps == 0.0) ? 0 : 1;
     g->h_dec_inst.assign(dec_inst, dec_inst + n_dec);
@@ -292,13 +299,22 @@ extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, in
         if (e != hipSuccess) return hip_fail(e, "H2D engine tables");
     }
     if ((e = hipStreamSynchronize(g->ctx->stream)) != hipSuccess) return hip_fail(e, "table upload sync");
-    int cap = g->cfg.ep_cap > 0 ? g->cfg.ep_cap : round_up(4 * max_items + 64, 64);
-    // Shrink an automatic capacity until one block fits the LDS limit.
-    rc = shape(g, cap, g->cfg.blocks_per_cu, n_dec);
-    while (rc == PLP_E_EPCAP && g->cfg.ep_cap == 0 && cap > 256) {
-        cap = round_up(cap / 2, 64);
-        rc = shape(g, cap, g->cfg.blocks_per_cu, n_dec);
+    int cap = g->cfg.ep_cap;
+    if (cap <= 0) {
+        // Automatic capacity: the largest (up to 4 points per item) that
+        // keeps the residency the registers allow at the smallest capacity;
+        // a decoder that needs more is rerun with more (sync()).
+        cap = 64;
+        if ((rc = shape(g, cap, g->cfg.blocks_per_cu, n_dec)) != PLP_OK) return rc;
+        const int occ_min = g->occ;
+        const int want = g->cfg.blocks_per_cu > 0 ? std::min(g->cfg.blocks_per_cu, occ_min) : occ_min;
+        for (int c = round_up(4 * max_items + 64, 32); c > 64; c -= 32)
+            if (shape(g, c, g->cfg.blocks_per_cu, n_dec) == PLP_OK && g->occ >= want) {
+                cap = c;
+                break;
+            }
     }
+    rc = shape(g, cap, g->cfg.blocks_per_cu, n_dec);
     if (rc != PLP_OK) return rc;
     g->passes_done = 0;
     g->need_reset = true;
@@ -356,7 +372,7 @@ extern "C" int plp_engine_sync(plp_engine* g) {
         for (int d = 0; d < g->n_dec; d++)
             if (g->h_state[(size_t)d].err != 0) {
                 failed.push_back(d);
-                need = std::max(need, (int)g->h_state[(size_t)d].max_ep);
+                need = std::max(need, (int)g->h_state[(size_t)d].ep_need);
             }
         if (failed.empty()) return PLP_OK;
         int cap = std::max(round_up(2 * g->ep_cap, 64), round_up(need + need / 4, 64));
@@ -429,6 +445,7 @@ extern "C" int plp_engine_fetch(plp_engine* g, plp_row_result* rows, int32_t* pl
         r.work_pair_box = s.work_pair_box;
         r.work_live_pair_box = s.work_live;
         r.max_ep = (int32_t)s.max_ep;
+        r.ep_need = (int32_t)s.ep_need;
         if (placements) {
             const int32_t* src = (have_acc ? acc.data() : all.data()) + stride * (size_t)d;
             std::memcpy(placements + stride * (size_t)d, src, 4 * 7 * (size_t)r.n_boxes);
@@ -463,11 +480,11 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
     int rc = plp_engine_run(g, n_passes);
     if (rc == PLP_OK) rc = plp_engine_sync(g);
     if (rc != PLP_OK) return rc;
-    int max_ep = 1;
-    for (const DecState& s : g->h_state) max_ep = std::max(max_ep, (int)s.max_ep);
+    int need = 1;
+    for (const DecState& s : g->h_state) need = std::max(need, (int)s.ep_need);
     std::vector<int> caps;
     for (int num : {4, 5, 6})   // 1.0x, 1.25x, 1.5x the observed peak
-        caps.push_back(std::max(64, round_up(max_ep * num / 4, 32)));
+        caps.push_back(std::max(64, round_up(need * num / 4, 32)));
     caps.push_back(g->ep_cap);
     std::sort(caps.begin(), caps.end());
     caps.erase(std::unique(caps.begin(), caps.end()), caps.end());

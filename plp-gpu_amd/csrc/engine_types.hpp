@@ -28,22 +28,18 @@ struct DecState {
     int32_t n_all, n_acc;    // D4' best overall / best accepted packing size; -1 = none
     int64_t v_all, v_acc;    // their volumes
     double s_all, s_acc;     // their stability scores
-    uint64_t work_pair_box;  // brute-force AABB tests so far (sum of n_cand * n_ors * n_box)
+    uint64_t work_pair_box;  // brute-force AABB tests so far (sum of n_cand * n_ors * n_box);
+                             // counted only with count_work
     uint64_t work_live;      // the same over the steps and live points this kernel evaluates
     uint32_t work_steps;     // steps that evaluated feasibility
-    uint32_t max_ep;         // largest extreme-point list reached
-    uint32_t pad[2];
+    uint32_t max_ep;         // largest reference extreme-point list (count_work only)
+    uint32_t ep_need;        // largest list this decoder held (the capacity it needs)
+    uint32_t pad;
 };
 static_assert(sizeof(DecState) == 80, "DecState layout");
 
 constexpr int kMtWords = 625;      // 624 state words + index
 
-// Extreme-point key flag: bit 31 marks a point no box can ever occupy again
-// (outside the container's far walls, or inside the half-open extent of a
-// placed box); such a point keeps its place in the sorted list but is never
-// offered as a candidate.
-constexpr uint32_t kDead = 0x80000000u;
-constexpr uint32_t kKeyMask = 0x7fffffffu;
 
 struct DecodeArgs {
     const InstHdr* inst;
@@ -64,15 +60,21 @@ struct DecodeArgs {
     int32_t use_rng;
     int32_t use_theta;
     int32_t reset;           // 1: seed RNG and clear selection state first
+    int32_t count_work;      // 1: also keep the dead points, to count the reference's work
+    int32_t wide_cache;      // 1: 32-bit pair-state cache (some w*d > kNarrowMaxArea)
     double eps, theta, mu;
     uint32_t* work_counter;  // persistent-grid work queue head
 };
 
-// Dynamic LDS layout of one decoder block.  The pass-time region (extreme
-// points, candidate list, pair masks, per-wave culled box copies) is reused by
-// FSR after the pass.  The box array has box_cap slots plus one sentinel.
+// Dynamic LDS layout of one decoder block.  The pass-time region (two
+// live extreme-point lists used alternately, the previous-list index of each
+// entry of the current one, two pair-state caches used alternately, the
+// dead-point list when work is counted, pair masks, per-wave culled box
+// copies) is reused by FSR after the pass.  Each list has ep_cap + 8 slots: a
+// merge writes at most 8 entries past the capacity before the overflow is
+// detected.  The box array has box_cap slots plus one sentinel.
 struct LdsLayout {
-    int32_t off_ep, off_tmp, off_act, off_mask, n_mask, off_rel;
+    int32_t off_ep, off_ep2, off_oidx, off_cache, n_cache, off_dead, off_mask, n_mask, off_rel;
     int32_t off_load, off_key, n_key;       // FSR view of the first region
     int32_t off_box, off_mt, total;
 };
@@ -83,13 +85,22 @@ __host__ __device__ inline int32_t next_pow2(int32_t v) {
     return p;
 }
 
+// A 16-bit pair-state cache entry holds min(s, 65534) exactly enough for C3
+// when every orientation has w*d <= 2 * 65534: a clamped s is still >= w*d/2.
+constexpr int64_t kNarrowMaxArea = 2 * 65534;
+
 __host__ __device__ inline LdsLayout lds_layout(int32_t ep_cap, int32_t box_cap, int32_t use_rng,
-                                                int32_t n_waves = 1) {
+                                                int32_t n_waves = 1, int32_t count_work = 0,
+                                                int32_t wide_cache = 0) {
     LdsLayout l;
+    const int32_t list = (ep_cap + 8) * 4;
     l.off_ep = 0;
-    l.off_tmp = ep_cap * 4;
-    l.off_act = 2 * ep_cap * 4;
-    l.off_mask = (3 * ep_cap * 4 + 7) & ~7;
+    l.off_ep2 = list;
+    l.off_oidx = 2 * list;
+    l.off_cache = (l.off_oidx + (ep_cap + 8) * 2 + 15) & ~15;
+    l.n_cache = ep_cap * 6;
+    l.off_dead = (l.off_cache + 2 * l.n_cache * (wide_cache ? 4 : 2) + 15) & ~15;
+    l.off_mask = (l.off_dead + (count_work ? list : 0) + 7) & ~7;
     l.n_mask = (ep_cap * 6 + 63) / 64;
     l.off_rel = (l.off_mask + l.n_mask * 8 + 15) & ~15;
     int32_t pass_end = l.off_rel + n_waves * (64 + 8) * 32;   // per-wave culled window

@@ -43,18 +43,21 @@ __device__ __forceinline__ uint32_t xyz_key(int x, int y, int z, int xb, int yb)
 //   C3  z > 0: s = sum of top-face overlaps of boxes whose top is at z;
 //       infeasible iff (double)s / (double)(w*d) < 0.5
 //   C1  no strict-interior overlap with any box
-// The verdict does not depend on which boxes are visited in what order as
-// long as every box that can overlap or support the pair is visited.  The
-// wave walks the placed boxes newest first in windows of 64 (one box per
-// lane), keeps the ones that intersect the chunk's region
-// [xlo,xhi) x [ylo,yhi) x [zlo,zhi) or whose top lies in [zlo, zhi) over that
-// footprint (a box outside it can neither overlap nor support any pair of
-// the chunk), copies them into the wave's cbox[] window, pads it with
-// sentinels to a multiple of kBoxGroup and scans it in groups whose LDS reads
-// issue together.  The wave leaves as soon as every lane has failed, and
-// skips the support arithmetic of a box whose top is at no live lane's level
-// (both wave-uniform branches).  Areas fit 32 bits: L*W < 2^30 is a load-time
-// precondition, and ox, oy < 2^21 so the 24-bit multiply is exact.
+// pair_scan takes lanes that passed C2 (alive) and returns C1 together with
+// the support sum s, which is complete for every lane that passes C1; the
+// caller applies C3 (pair_verdict).  The result does not depend on which
+// boxes are visited in what order as long as every box that can overlap or
+// support the pair is visited.  The wave walks the placed boxes newest first
+// in windows of 64 (one box per lane), keeps the ones that intersect the
+// chunk's region [xlo,xhi) x [ylo,yhi) x [zlo,zhi) or whose top lies in
+// [zlo, zhi) over that footprint (a box outside it can neither overlap nor
+// support any pair of the chunk), copies them into the wave's cbox[] window,
+// pads it with sentinels to a multiple of kBoxGroup and scans it in groups
+// whose LDS reads issue together.  The wave leaves as soon as every lane has
+// failed, and skips the support arithmetic of a box whose top is at no live
+// lane's level (both wave-uniform branches).  Areas fit 32 bits: L*W < 2^30
+// is a load-time precondition, and ox, oy < 2^21 so the 24-bit multiply is
+// exact.
 constexpr int kBoxGroup = 4;
 constexpr int kCullWindow = kWave + kBoxGroup;   // cbox[] entries per wave
 
@@ -66,10 +69,10 @@ __device__ __forceinline__ Box sentinel_box() {
     return b;
 }
 
-__device__ __forceinline__ bool pair_feasible(int x, int y, int z, int w, int d, int h,
-                                              bool alive, const Box* __restrict__ boxes,
-                                              int n_box, int xlo, int xhi, int ylo, int yhi,
-                                              int zlo, int zhi, Box* __restrict__ cbox) {
+__device__ __forceinline__ bool pair_scan(int x, int y, int z, int w, int d, int h, bool alive,
+                                          const Box* __restrict__ boxes, int n_box, int xlo,
+                                          int xhi, int ylo, int yhi, int zlo, int zhi,
+                                          Box* __restrict__ cbox, uint32_t& s_out) {
     const int lane = lane_id();
     const int x2 = x + w, y2 = y + d, z2 = z + h;
     uint32_t s = 0;
@@ -107,8 +110,32 @@ __device__ __forceinline__ bool pair_feasible(int x, int y, int z, int w, int d,
         }
         wave_sync();   // the next window overwrites cbox[]
     }
-    if (alive && z > 0) alive = !((double)s / (double)((int64_t)w * d) < 0.5);
+    s_out = s;
     return alive;
+}
+
+// C3 on top of C1 and C2 (plp_common.hpp:304-312), in the reference's FP64 form.
+__device__ __forceinline__ bool pair_verdict(bool c12, int z, int w, int d, uint32_t s) {
+    return c12 && (z == 0 || !((double)s / (double)((int64_t)w * d) < 0.5));
+}
+
+// Pair-state cache entry: the support sum of a pair that passes C1 and C2,
+// or kBlocked.  Both only grow as boxes are added, so the state after one
+// more box follows from the old state and that box alone.
+constexpr uint32_t kBlocked = 0xffffffffu;
+constexpr uint16_t kNewPoint = 0xffffu;   // list entry with no cached state
+
+// Entries are 32-bit, or 16-bit with the support sum clamped to 65534 and
+// 0xffff for kBlocked (exact for C3 when every w*d <= kNarrowMaxArea).
+__device__ __forceinline__ uint32_t cache_get(const void* c, int i, bool wide) {
+    if (wide) return ((const uint32_t*)c)[i];
+    const uint32_t v = ((const uint16_t*)c)[i];
+    return v == 0xffffu ? kBlocked : v;
+}
+
+__device__ __forceinline__ void cache_put(void* c, int i, uint32_t v, bool wide) {
+    if (wide) ((uint32_t*)c)[i] = v;
+    else ((uint16_t*)c)[i] = (uint16_t)(v == kBlocked ? 0xffffu : min(v, 0xfffeu));
 }
 
 // ---------------------------------------------------------------- FSR ----
@@ -197,6 +224,147 @@ __device__ double fsr_score(const Box* __restrict__ box, int n, double mu, doubl
     return (double)c / (double)n;
 }
 
+// Extreme-point update after placing the box [x, X2) x [y, Y2) x [z, Z2)
+// (plp_common.hpp:439-453): points strictly inside the box leave the list,
+// the box's 8 corners join it unless already present, and the list stays
+// sorted and unique.  On the live list src -> dst: the points in the box's
+// half-open extent die (the strictly inside ones among them), a live corner
+// replaces an equal live point, dead corners are not added.  With count_work
+// the dead list (unsorted) drops its strictly inside points and receives the
+// points that die without being strictly inside and the dead corners found
+// in neither list.  n_act <= cap on entry keeps every write within the
+// cap + 8 slots of a list.  One wave; returns the larger new list size.
+__device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t* oidx, uint32_t* dead,
+                            int& n_act, int& n_dead, const Box* box, int n_box, int x, int y, int z,
+                            int X2, int Y2, int Z2, int L, int W, int H, int xb, int yb, int cap,
+                            bool count_work, int* jold) {
+    const int lane = lane_id();
+    if (lane < 8) jold[lane] = kNewPoint;
+    // Corner j: bit 0 -> x side, bit 1 -> y, bit 2 -> z.
+    uint32_t ck[8];
+    uint32_t cdead = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int jx = (j & 1) ? X2 : x, jy = (j & 2) ? Y2 : y, jz = (j & 4) ? Z2 : z;
+        ck[j] = xyz_key(jx, jy, jz, xb, yb);
+        if ((jx >= L) | (jy >= W) | (jz >= H)) cdead |= 1u << j;
+    }
+    for (int b0 = 0; b0 < n_box; b0 += kWave) {
+        const int bi = b0 + lane;
+        Box B = sentinel_box();
+        if (bi < n_box) B = box[bi];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int jx = (j & 1) ? X2 : x, jy = (j & 2) ? Y2 : y, jz = (j & 4) ? Z2 : z;
+            const bool in = (B.x <= jx) & (jx < B.x2) & (B.y <= jy) & (jy < B.y2) &
+                            (B.z <= jz) & (jz < B.z2);
+            if (ballot(in)) cdead |= 1u << j;
+        }
+    }
+    const uint32_t clive = ~cdead & 0xffu;
+
+    // Dead list: drop the strictly inside points in place (a point moves to
+    // an index no larger than its own, below the next 64-point chunk).
+    uint32_t eq_dead = 0;
+    int nd = n_dead;
+    if (count_work) {
+        int db = 0;
+        for (int i0 = 0; i0 < n_dead; i0 += kWave) {
+            const int i = i0 + lane;
+            const bool ok = i < n_dead;
+            const uint32_t v = ok ? dead[i] : 0u;
+            int px, py, pz;
+            key_xyz(v, xb, yb, px, py, pz);
+            const bool strict = (x < px) & (px < X2) & (y < py) & (py < Y2) & (z < pz) & (pz < Z2);
+            const bool keep = ok & !strict;
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                if (ballot(keep & (v == ck[j]))) eq_dead |= 1u << j;
+            const uint64_t km = ballot(keep);
+            if (keep) dead[db + rank_below(km)] = v;
+            db += __popcll(km);
+        }
+        nd = uni(db);
+        wave_sync();
+    }
+
+    // Live list: survivors go to dst after the live corners below them;
+    // oidx[] records where each dst entry was in src (a live corner that
+    // replaces an equal point takes that point's index).
+    wave_sync();
+    int lt[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) lt[j] = 0;
+    uint32_t eq_live = 0;
+    int base = 0;
+    for (int i0 = 0; i0 < n_act; i0 += kWave) {
+        const int i = i0 + lane;
+        const bool ok = i < n_act;
+        const uint32_t v = ok ? src[i] : 0u;
+        int px, py, pz;
+        key_xyz(v, xb, yb, px, py, pz);
+        const bool half = (x <= px) & (px < X2) & (y <= py) & (py < Y2) & (z <= pz) & (pz < Z2);
+        bool dup = false;
+        int shift = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const bool e = ok & (v == ck[j]);
+            if (ballot(e)) eq_live |= 1u << j;
+            const bool lj = (clive >> j) & 1u;
+            if (e & lj) jold[j] = i;
+            dup |= e & lj;
+            shift += (lj & (ck[j] < v)) ? 1 : 0;
+        }
+        const bool keep = ok & !half & !dup;
+        const uint64_t km = ballot(keep);
+        if (keep) {
+            const int pos = base + rank_below(km) + shift;
+            dst[pos] = v;
+            oidx[pos] = (uint16_t)i;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) lt[j] += __popcll(ballot(keep & (v < ck[j])));
+        base += __popcll(km);
+        if (count_work) {
+            const bool strict = (x < px) & (px < X2) & (y < py) & (py < Y2) & (z < pz) & (pz < Z2);
+            const bool mv = ok & half & !strict;
+            const uint64_t mm = ballot(mv);
+            const int pos = nd + rank_below(mm);
+            if (mv && pos < cap + 8) dead[pos] = v;
+            nd += __popcll(mm);
+        }
+    }
+    base = uni(base);
+    uint32_t myk = ck[0];
+    int mylt = lt[0];
+#pragma unroll
+    for (int j = 1; j < 8; j++)
+        if (lane == j) {
+            myk = ck[j];
+            mylt = lt[j];
+        }
+    wave_sync();
+    if (lane < 8 && ((clive >> lane) & 1u)) {
+        int r = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) r += (((clive >> j) & 1u) && ck[j] < myk) ? 1 : 0;
+        dst[mylt + r] = myk;
+        oidx[mylt + r] = (uint16_t)jold[lane];
+    }
+    if (count_work) {
+        const uint32_t add = cdead & ~eq_live & ~eq_dead;
+        if (lane < 8 && ((add >> lane) & 1u)) {
+            const int pos = nd + __popc(add & ((1u << lane) - 1u));
+            if (pos < cap + 8) dead[pos] = myk;
+        }
+        nd += __popc(add);
+    }
+    wave_sync();
+    n_act = base + __popc(clive);
+    n_dead = count_work ? uni(nd) : 0;
+    return max(n_act, n_dead);
+}
+
 // ------------------------------------------------------------ decoder ----
 
 // Block-wide scalars.  Wave 0 writes quit / n_act / n_box before the barrier
@@ -205,9 +373,9 @@ __device__ double fsr_score(const Box* __restrict__ box, int n, double mu, doubl
 struct StepShared {
     int item;       // queue item of the block
     int quit;       // the decoder stopped (capacity error): leave the pass loop
-    int no[2];      // unused
-    int n_act[2];   // live extreme points after the last placement ([0])
-    int n_box[2];   // placed boxes after the last placement ([0])
+    int n_act;      // live extreme points after the last placement
+    int n_box;      // placed boxes after the last placement
+    int jold[8];    // merge: previous-list index of a live point equal to corner j
     int cnt[2][8];  // feasible pairs found by each wave, by evaluated-step parity:
                     // a wave may start step k+1 while others still read step k
 };
@@ -234,24 +402,36 @@ __device__ __forceinline__ void write_rows(int32_t* dst, const Box* box, int n) 
 // for n_passes passes (plp_common.hpp:404-456 per pass, FSR and the D4'
 // selection after each), then takes the next queue item.
 //
-// Extreme points are packed (z, y, x) keys, sorted and unique in ep[]
-// (flag kDead on points no pair can ever use again); act[] holds the live
-// ones, so pair p = e * n_ors + o enumerates exactly the reference's
-// candidate-major feas order restricted to points that can be feasible.
+// Extreme points are packed (z, y, x) keys.  Only the live ones are kept,
+// sorted and unique: a point beyond a far wall or inside the half-open extent
+// of a placed box fails C2 or C1 for every box and orientation from then on
+// (boxes never move), so dropping it removes only infeasible pairs and pair
+// p = e * n_ors + o still enumerates the reference's candidate-major feas
+// order (plp_common.hpp:419-435) restricted to the pairs that can be
+// feasible.  The live list alternates between two LDS buffers, one merge
+// reading one and writing the other.  With count_work the dead points are
+// kept too (unsorted) so that the reference's list size, and from it its
+// brute-force work, can be reported.
 // Every wave evaluates 64-pair chunks c = wave, wave + NW, ...; each chunk's
-// verdicts become one ballot mask in LDS and the pick is the idx-th set bit
-// (plp_common.hpp:419-435).  Wave 0 alone runs the serial part of a step
-// (RNG, pick, place, extreme-point merge), FSR and the selection; the other
-// waves wait at the block barrier.
+// verdicts become one ballot mask in LDS and the pick is the idx-th set bit.
+// Wave 0 alone runs the serial part of a step (RNG, pick, place, extreme-
+// point merge), FSR and the selection; the other waves wait at the block
+// barrier.
 template <int NW>
 __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int s_ors[18 * NW];
     __shared__ StepShared sh;
-    const LdsLayout lay = lds_layout(a.ep_cap, a.box_cap, a.use_rng, NW);
-    uint32_t* ep = (uint32_t*)(lds + lay.off_ep);
-    uint32_t* tmp = (uint32_t*)(lds + lay.off_tmp);
-    uint32_t* act = (uint32_t*)(lds + lay.off_act);
+    __shared__ DecState sst;
+    const LdsLayout lay = lds_layout(a.ep_cap, a.box_cap, a.use_rng, NW, a.count_work,
+                                     a.wide_cache);
+    uint32_t* const ep_a = (uint32_t*)(lds + lay.off_ep);
+    uint32_t* const ep_b = (uint32_t*)(lds + lay.off_ep2);
+    uint32_t* const dead = (uint32_t*)(lds + lay.off_dead);
+    uint16_t* const oidx = (uint16_t*)(lds + lay.off_oidx);
+    unsigned char* const cache = lds + lay.off_cache;
+    const bool wide = a.wide_cache != 0;
+    const int cache_bytes = lay.n_cache * (wide ? 4 : 2);
     uint64_t* mask = (uint64_t*)(lds + lay.off_mask);
     double* load = (double*)(lds + lay.off_load);
     uint64_t* skey = (uint64_t*)(lds + lay.off_key);
@@ -273,25 +453,25 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
         const int L = I.L, W = I.W, H = I.H, xb = I.xbits, yb = I.ybits;
         const int32_t* steps = a.step_type + I.step_off;
 
-        // ---- decoder state (wave 0)
-        DecState st;
+        // ---- decoder state: in LDS (sst) between pass ends; wave 0
         int mti = 624;
         uint32_t* gmt = a.mt + (size_t)d * kMtWords;
         if (wid == 0) {
-            if (a.reset) {
-                st.passes = 0; st.err = 0; st.n_all = -1; st.n_acc = -1;
-                st.v_all = 0; st.v_acc = 0; st.s_all = 0.0; st.s_acc = 0.0;
-                st.work_pair_box = 0; st.work_live = 0; st.work_steps = 0; st.max_ep = 0;
-                st.pad[0] = st.pad[1] = 0;
-            } else {
-                st = a.state[d];
-                st.passes = uni(st.passes); st.err = uni(st.err);
-                st.n_all = uni(st.n_all); st.n_acc = uni(st.n_acc);
-                st.v_all = uni(st.v_all); st.v_acc = uni(st.v_acc);
-                st.work_pair_box = uni(st.work_pair_box); st.work_live = uni(st.work_live);
-                st.work_steps = uni(st.work_steps); st.max_ep = uni(st.max_ep);
+            if (lane == 0) {
+                DecState st0;
+                if (a.reset) {
+                    st0.passes = 0; st0.err = 0; st0.n_all = -1; st0.n_acc = -1;
+                    st0.v_all = 0; st0.v_acc = 0; st0.s_all = 0.0; st0.s_acc = 0.0;
+                    st0.work_pair_box = 0; st0.work_live = 0; st0.work_steps = 0; st0.max_ep = 0;
+                    st0.ep_need = 0; st0.pad = 0;
+                } else {
+                    st0 = a.state[d];
+                }
+                sst = st0;
+                sh.quit = st0.err != 0;
             }
-            if (st.err == 0 && a.use_rng) {
+            wave_sync();
+            if (uni(sh.quit) == 0 && a.use_rng) {
                 if (a.reset) {
                     mt_seed_lane0(mt, a.dec_seed[d]);
                 } else {
@@ -300,7 +480,6 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                 }
                 wave_sync();
             }
-            if (lane == 0) sh.quit = st.err != 0;
         }
         block_sync();
         const bool skip_decoder = uni(sh.quit) != 0;
@@ -309,14 +488,14 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
 
         for (int pass = 0; pass < a.n_passes && !skip_decoder; pass++) {
             // ---- pass start (wave 0): origin only, no boxes
-            int n_ep = 1, n_act = 1, n_box = 0;
+            int n_act = 1, n_dead = 0, n_box = 0;
             int fail_ti = -1;   // type whose last copy found nothing since the last placement
             uint64_t work = 0, live = 0;
-            uint32_t n_steps_eval = 0, max_ep = 1;
+            uint32_t n_steps_eval = 0, max_ep = 1, ep_need = 1;
             if (wid == 0) {
                 if (lane == 0) {
-                    ep[0] = 0u;     // origin; L, W, H > 0 so it is live
-                    act[0] = 0u;
+                    ep_a[0] = 0u;   // origin; L, W, H > 0 so it is live
+                    oidx[0] = kNewPoint;
                 }
                 for (int i = lane; i < a.box_cap + 8; i += kWave) box[i] = sentinel_box();
             }
@@ -325,14 +504,17 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
             // needs only the type, the last failure and whether a placement
             // happened, which all waves learn from the shared counts.
             int nb_w = 0, na_w = 1;
-            int ev = 0;   // evaluated steps so far (same count in every wave)
+            int ev = 0;     // evaluated steps so far (same count in every wave)
+            int cur = 0;    // live list in use: 0 -> ep_a, 1 -> ep_b (every wave)
+            int prev_ti = -1;   // type of the previous evaluated step
             for (int s = 0; s < I.n_steps; s++) {
                 const int ti = uni(steps[s]);
                 const int32_t* orow = a.ors_tab + (size_t)(I.ors_off + ti) * kOrsRow;
                 const int no = uni(orow[0]);
                 if (no == 0) continue;      // type with no allowed orientation
                 if (wid == 0) {
-                    work += (uint64_t)n_ep * (uint64_t)no * (uint64_t)n_box;
+                    if (a.count_work)
+                        work += (uint64_t)(n_act + n_dead) * (uint64_t)no * (uint64_t)n_box;
                     n_steps_eval++;
                 }
                 // Same type, no placement since its last copy found nothing:
@@ -341,8 +523,22 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                 if (wid == 0) live += (uint64_t)n_act * (uint64_t)no * (uint64_t)n_box;
                 if (lane < 3 * no) s_orw[lane] = orow[1 + lane];
                 wave_sync();
+                const uint32_t* act = cur ? ep_b : ep_a;
 
-                // ---- pair verdicts (every wave)
+                // ---- pair verdicts (every wave).  After a placement of
+                // the same type, a pair of a point that was in the previous
+                // list takes its cached state and tests the new box alone;
+                // the other pairs scan the boxes.
+                const bool inc = ti == prev_ti;
+                prev_ti = ti;
+                const unsigned char* c_rd = cache + ((ev + 1) & 1) * cache_bytes;
+                unsigned char* c_wr = cache + (ev & 1) * cache_bytes;
+                int nx = 0, ny = 0, nz = 0, nx2 = -1, ny2 = -1, nz2 = -1;
+                if (inc) {
+                    const Box NB = box[nb_w - 1];
+                    nx = uni(NB.x); ny = uni(NB.y); nz = uni(NB.z);
+                    nx2 = uni(NB.x2); ny2 = uni(NB.y2); nz2 = uni(NB.z2);
+                }
                 const int n_pairs = na_w * no;
                 const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)no - 1) / (uint64_t)no);
                 const int n_chunks = (n_pairs + kWave - 1) / kWave;
@@ -352,24 +548,47 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                     const bool valid = p < n_pairs;
                     const int e = no == 1 ? p : (int)__umulhi((uint32_t)p, magic);
                     const int o = p - e * no;
-                    int x = 0, y = 0, z = 0, w = 1, dd = 1, h = 1;
+                    int x = 0, y = 0, z = 0, w = 1, dd = 1, h = 1, oi = kNewPoint;
                     if (valid) {
                         key_xyz(act[e], xb, yb, x, y, z);
                         w = s_orw[3 * o];
                         dd = s_orw[3 * o + 1];
                         h = s_orw[3 * o + 2];
+                        if (inc) oi = oidx[e];
                     }
-                    const bool alive = valid && (x + w <= L) && (y + dd <= W) && (z + h <= H);
-                    uint64_t m = 0;
-                    if (ballot(alive) != 0) {
-                        const int xlo = wave_min(alive ? x : 0x7fffffff);
-                        const int xhi = wave_max(alive ? x + w : -1);
-                        const int ylo = wave_min(alive ? y : 0x7fffffff);
-                        const int yhi = wave_max(alive ? y + dd : -1);
-                        const int zlo = wave_min(alive ? z : 0x7fffffff);
-                        const int zhi = wave_max(alive ? z + h : -1);
-                        m = ballot(pair_feasible(x, y, z, w, dd, h, alive, box, nb_w, xlo, xhi,
-                                                 ylo, yhi, zlo, zhi, cbox));
+                    bool c12 = valid && (x + w <= L) && (y + dd <= W) && (z + h <= H);
+                    uint32_t sup = 0;
+                    bool scan = c12;
+                    if (inc && c12 && oi != kNewPoint) {
+                        scan = false;
+                        const uint32_t old = cache_get(c_rd, oi * no + o, wide);
+                        const bool ov = (x < nx2) & (nx < x + w) & (y < ny2) & (ny < y + dd) &
+                                        (z < nz2) & (nz < z + h);
+                        if (old == kBlocked || ov) {
+                            c12 = false;
+                        } else {
+                            const int ox = min(x + w, nx2) - max(x, nx);
+                            const int oy = min(y + dd, ny2) - max(y, ny);
+                            sup = old + ((nz2 == z && ox > 0 && oy > 0)
+                                             ? __umul24((uint32_t)ox, (uint32_t)oy) : 0u);
+                        }
+                    }
+                    // Settled lanes first, so that nothing of theirs stays
+                    // live across the scan.
+                    uint64_t m = ballot(!scan && pair_verdict(c12, z, w, dd, sup));
+                    if (valid && !scan) cache_put(c_wr, p, c12 ? sup : kBlocked, wide);
+                    if (ballot(scan) != 0) {
+                        const int xlo = wave_min(scan ? x : 0x7fffffff);
+                        const int xhi = wave_max(scan ? x + w : -1);
+                        const int ylo = wave_min(scan ? y : 0x7fffffff);
+                        const int yhi = wave_max(scan ? y + dd : -1);
+                        const int zlo = wave_min(scan ? z : 0x7fffffff);
+                        const int zhi = wave_max(scan ? z + h : -1);
+                        uint32_t s2 = 0;
+                        const bool a2 = pair_scan(x, y, z, w, dd, h, scan, box, nb_w, xlo, xhi, ylo,
+                                                  yhi, zlo, zhi, cbox, s2);
+                        if (scan) cache_put(c_wr, p, a2 ? s2 : kBlocked, wide);
+                        m |= ballot(scan && pair_verdict(a2, z, w, dd, s2));
                     }
                     if (lane == 0) mask[c] = m;
                     cnt_w += __popcll(m);
@@ -390,161 +609,70 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
 
                 // ---- pick, place, extreme-point merge (wave 0)
                 if (wid == 0) {
-                    {
-                        // feas[0], or feas[randrange(|feas|)] when random()
-                        // >= 1 - eps (plp_common.hpp:427-435).
-                        int idx = 0;
-                        if (a.use_rng) {
-                            if (mt_random(mt, mti) >= 1.0 - a.eps) idx = mt_randrange(mt, mti, cnt);
-                        }
-                        int c = 0;
-                        uint64_t m = uni(mask[0]);
-                        for (;;) {
-                            const int pc = __popcll(m);
-                            if (idx < pc) break;
-                            idx -= pc;
-                            c++;
-                            m = uni(mask[c]);
-                        }
-                        const bool hit = ((m >> lane) & 1ull) && rank_below(m) == idx;
-                        const int bit = __ffsll((unsigned long long)ballot(hit)) - 1;
-                        const int p = c * kWave + bit;
-                        const int e = p / no, o = p - (p / no) * no;
-                        int x, y, z;
-                        key_xyz(uni(act[e]), xb, yb, x, y, z);
-                        x = uni(x);
-                        y = uni(y);
-                        z = uni(z);
-                        const int w = uni(s_orw[3 * o]), dd = uni(s_orw[3 * o + 1]),
-                                  h = uni(s_orw[3 * o + 2]);
-                        const int X2 = x + w, Y2 = y + dd, Z2 = z + h;
-                        if (lane == 0) {
-                            Box nbx;
-                            nbx.x = x; nbx.y = y; nbx.z = z; nbx.x2 = X2; nbx.y2 = Y2; nbx.z2 = Z2;
-                            nbx.type = ti; nbx.pad = 0;
-                            box[n_box] = nbx;
-                        }
-                        n_box++;
-                        wave_sync();
-
-                        // Drop points strictly inside the new box (mark the
-                        // ones in its half-open extent dead), then merge the
-                        // 8 corners not already present (plp_common.hpp:439-453).
-                        int base = 0;
-                        for (int i0 = 0; i0 < n_ep; i0 += kWave) {
-                            const int i = i0 + lane;
-                            const bool v_ok = i < n_ep;
-                            const uint32_t v = v_ok ? ep[i] : 0u;
-                            int px, py, pz;
-                            key_xyz(v & kKeyMask, xb, yb, px, py, pz);
-                            const bool strict = (x < px) & (px < X2) & (y < py) & (py < Y2) &
-                                                (z < pz) & (pz < Z2);
-                            const bool half = (x <= px) & (px < X2) & (y <= py) & (py < Y2) &
-                                              (z <= pz) & (pz < Z2);
-                            const bool keep = v_ok && !strict;
-                            const uint64_t km = ballot(keep);
-                            if (keep) tmp[base + rank_below(km)] = half ? (v | kDead) : v;
-                            base += __popcll(km);
-                        }
-                        const int n_surv = uni(base);
-                        // Corner j (lanes 0..7): bit 0 -> x side, bit 1 -> y, bit 2 -> z.
-                        const int cx = (lane & 1) ? X2 : x;
-                        const int cy = (lane & 2) ? Y2 : y;
-                        const int cz = (lane & 4) ? Z2 : z;
-                        const uint32_t ck = xyz_key(cx, cy, cz, xb, yb);
-                        int jx[8], jy[8], jz[8];
-#pragma unroll
-                        for (int j = 0; j < 8; j++) {
-                            jx[j] = (j & 1) ? X2 : x;
-                            jy[j] = (j & 2) ? Y2 : y;
-                            jz[j] = (j & 4) ? Z2 : z;
-                        }
-                        uint32_t dead_bits = 0;
-                        for (int b0 = 0; b0 < n_box; b0 += kWave) {
-                            const int bi = b0 + lane;
-                            const bool bv = bi < n_box;
-                            Box B;
-                            if (bv) B = box[bi];
-                            else { B.x = B.y = B.z = 1; B.x2 = B.y2 = B.z2 = 0; }
-#pragma unroll
-                            for (int j = 0; j < 8; j++) {
-                                const bool in = (B.x <= jx[j]) & (jx[j] < B.x2) & (B.y <= jy[j]) &
-                                                (jy[j] < B.y2) & (B.z <= jz[j]) & (jz[j] < B.z2);
-                                if (ballot(in)) dead_bits |= 1u << j;
-                            }
-                        }
-                        wave_sync();
-                        const bool cdead = (cx >= L) | (cy >= W) | (cz >= H) |
-                                           (((dead_bits >> (lane & 7)) & 1u) != 0);
-                        int lo = 0;
-                        bool kept = false;
-                        if (lane < 8) {
-                            int hi = n_surv;
-                            while (lo < hi) {
-                                const int mid = (lo + hi) >> 1;
-                                if ((tmp[mid] & kKeyMask) < ck) lo = mid + 1;
-                                else hi = mid;
-                            }
-                            kept = !(lo < n_surv && (tmp[lo] & kKeyMask) == ck);
-                        }
-                        const uint64_t kmask = ballot(kept);
-                        const int n_kept = __popcll(kmask);
-                        if (n_surv + n_kept > a.ep_cap) {
-                            st.err = 4;   // PLP_E_EPCAP
-                            max_ep = max(max_ep, (uint32_t)(n_surv + n_kept));
-                        } else {
-                            uint32_t kk[8];
-#pragma unroll
-                            for (int j = 0; j < 8; j++)
-                                kk[j] = ((kmask >> j) & 1ull) ? read_lane(ck, j) : 0xffffffffu;
-                            if (kept) {
-                                int r = 0;
-#pragma unroll
-                                for (int j = 0; j < 8; j++) r += (kk[j] < ck) ? 1 : 0;
-                                ep[lo + r] = ck | (cdead ? kDead : 0u);
-                            }
-                            for (int i = lane; i < n_surv; i += kWave) {
-                                const uint32_t v = tmp[i];
-                                const uint32_t k = v & kKeyMask;
-                                int r = 0;
-#pragma unroll
-                                for (int j = 0; j < 8; j++) r += (kk[j] < k) ? 1 : 0;
-                                ep[i + r] = v;
-                            }
-                            n_ep = n_surv + n_kept;
-                            max_ep = max(max_ep, (uint32_t)n_ep);
-                            wave_sync();
-                            base = 0;
-                            for (int i0 = 0; i0 < n_ep; i0 += kWave) {
-                                const int i = i0 + lane;
-                                const uint32_t v = i < n_ep ? ep[i] : kDead;
-                                const bool lv = (v & kDead) == 0u;
-                                const uint64_t lm = ballot(lv);
-                                if (lv) act[base + rank_below(lm)] = v;
-                                base += __popcll(lm);
-                            }
-                            n_act = uni(base);
-                            wave_sync();
-                        }
-                        if (lane == 0) {
-                            sh.quit = st.err != 0;
-                            sh.n_act[0] = n_act;
-                            sh.n_box[0] = n_box;
-                        }
+                    // feas[0], or feas[randrange(|feas|)] when random() >=
+                    // 1 - eps (plp_common.hpp:427-435).
+                    int idx = 0;
+                    if (a.use_rng) {
+                        if (mt_random(mt, mti) >= 1.0 - a.eps) idx = mt_randrange(mt, mti, cnt);
+                    }
+                    // The idx-th feasible pair.
+                    int c = 0;
+                    uint64_t m = uni(mask[0]);
+                    for (;;) {
+                        const int pc = __popcll(m);
+                        if (idx < pc) break;
+                        idx -= pc;
+                        c++;
+                        m = uni(mask[c]);
+                    }
+                    const bool hit = ((m >> lane) & 1ull) && rank_below(m) == idx;
+                    const int bit = __ffsll((unsigned long long)ballot(hit)) - 1;
+                    const int p = c * kWave + bit;
+                    const int e = p / no, o = p - (p / no) * no;
+                    int x, y, z;
+                    key_xyz(uni(act[e]), xb, yb, x, y, z);
+                    x = uni(x);
+                    y = uni(y);
+                    z = uni(z);
+                    const int w = uni(s_orw[3 * o]), dd = uni(s_orw[3 * o + 1]),
+                              h = uni(s_orw[3 * o + 2]);
+                    const int X2 = x + w, Y2 = y + dd, Z2 = z + h;
+                    if (lane == 0) {
+                        Box nbx;
+                        nbx.x = x; nbx.y = y; nbx.z = z; nbx.x2 = X2; nbx.y2 = Y2; nbx.z2 = Z2;
+                        nbx.type = ti; nbx.pad = 0;
+                        box[n_box] = nbx;
+                    }
+                    n_box++;
+                    wave_sync();
+                    const int need = merge_points(act, cur ? ep_a : ep_b, oidx, dead, n_act, n_dead,
+                                                  box, n_box, x, y, z, X2, Y2, Z2, L, W, H, xb, yb,
+                                                  a.ep_cap, a.count_work != 0, sh.jold);
+                    max_ep = max(max_ep, (uint32_t)(n_act + n_dead));
+                    ep_need = max(ep_need, (uint32_t)need);
+                    if (lane == 0) {
+                        sh.quit = need > a.ep_cap;   // PLP_E_EPCAP
+                        sh.n_act = n_act;
+                        sh.n_box = n_box;
                     }
                 }
                 block_sync();
                 if (uni(sh.quit) != 0) break;
-                na_w = uni(sh.n_act[0]);
-                nb_w = uni(sh.n_box[0]);
+                na_w = uni(sh.n_act);
+                nb_w = uni(sh.n_box);
+                cur ^= 1;
             }
 
             // ---- pass end (wave 0): stability score and D4' selection
             if (wid == 0) {
+                const bool overflow = uni(sh.quit) != 0;
+                DecState st = sst;
+                if (overflow) st.err = 4;   // PLP_E_EPCAP
                 st.work_pair_box += work;
                 st.work_live += live;
                 st.work_steps += n_steps_eval;
-                st.max_ep = max(st.max_ep, max_ep);
+                if (a.count_work) st.max_ep = max(st.max_ep, max_ep);
+                st.ep_need = max(st.ep_need, ep_need);
                 if (st.err == 0) {
                     const int n = n_box;
                     int64_t vpart = 0;
@@ -568,6 +696,8 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                     }
                     st.passes++;
                 }
+                wave_sync();
+                if (lane == 0) sst = st;
             }
             block_sync();
             if (uni(sh.quit) != 0) break;
@@ -577,7 +707,7 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                 for (int i = lane; i < 624; i += kWave) gmt[i] = mt[i];
                 if (lane == 0) gmt[624] = (uint32_t)mti;
             }
-            if (lane == 0) a.state[d] = st;
+            if (lane == 0) a.state[d] = sst;
         }
         block_sync();
     }

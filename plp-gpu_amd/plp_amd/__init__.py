@@ -54,14 +54,14 @@ class Params(C.Structure):
 
 class EngineConfig(C.Structure):
     _fields_ = [("waves_per_decoder", C.c_int32), ("ep_cap", C.c_int32),
-                ("blocks_per_cu", C.c_int32), ("use_graph", C.c_int32)]
+                ("blocks_per_cu", C.c_int32), ("use_graph", C.c_int32), ("count_work", C.c_int32)]
 
 
 class RowResult(C.Structure):
     _fields_ = [("n_boxes", C.c_int32), ("fallback", C.c_int32), ("n_passes", C.c_int32),
                 ("err", C.c_int32), ("volume", C.c_int64), ("score", C.c_double),
                 ("seed", C.c_uint64), ("work_pair_box", C.c_uint64),
-                ("work_live_pair_box", C.c_uint64), ("max_ep", C.c_int32), ("reserved", C.c_int32)]
+                ("work_live_pair_box", C.c_uint64), ("max_ep", C.c_int32), ("ep_need", C.c_int32)]
 
 
 _P = C.c_void_p
@@ -242,11 +242,12 @@ class Ctx:
 class Engine:
     """Device-resident decoders (seam 3)."""
 
-    def __init__(self, ctx: Ctx, ep_cap=0, blocks_per_cu=0, use_graph=False, waves_per_decoder=0):
+    def __init__(self, ctx: Ctx, ep_cap=0, blocks_per_cu=0, use_graph=False, waves_per_decoder=0,
+                 count_work=False):
         self.ctx = ctx
         self.h = C.c_void_p()
         cfg = EngineConfig(int(waves_per_decoder), int(ep_cap), int(blocks_per_cu),
-                           int(bool(use_graph)))
+                           int(bool(use_graph)), int(bool(count_work)))
         _check(_lib.plp_engine_create(ctx.h, C.byref(cfg), C.byref(self.h)))
         ctx._children.add(self)
         self.n_dec = 0

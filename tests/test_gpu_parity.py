@@ -178,7 +178,7 @@ def test_engine_every_reference_case(plp, ctx, ref_runs):
         by_params.setdefault(key, []).append(c)
     for key, cases in by_params.items():
         insts = [inst_from(plp, c["instance"]) for c in cases]
-        eng = plp.Engine(ctx)
+        eng = plp.Engine(ctx, count_work=True)
         eng.load(insts, list(range(len(cases))), [c["args"]["seed"] for c in cases],
                  prm(plp, cases[0]["args"]))
         eng.run(key[0])
@@ -207,19 +207,25 @@ def test_engine_passes_in_pieces_equal_one_call(plp, ctx, ref_runs):
     assert plp.to_json_dict(rows[0], pl[0]) == ref_fields(c["result"])
 
 
-def test_engine_capacity_escalation(plp, ctx, ref_runs):
-    """A tiny EP capacity overflows; the engine reruns those decoders from
-    their seeds with a larger capacity and the results are unchanged."""
+@pytest.mark.parametrize("count_work", [False, True])
+def test_engine_capacity_escalation(plp, ctx, ref_runs, count_work):
+    """A tiny EP capacity overflows (the live list, and with count_work the
+    dead list); the engine reruns those decoders from their seeds with a
+    larger capacity and the results are unchanged."""
     cases = [c for c in ref_runs if c["case"] in ("BR1_eps03_s1", "BR15_eps0", "plain_c1")]
     for c in cases:
-        eng = plp.Engine(ctx, ep_cap=64)
+        eng = plp.Engine(ctx, ep_cap=16, count_work=count_work)
         eng.load([inst_from(plp, c["instance"])], [0], [c["args"]["seed"]], prm(plp, c["args"]))
         eng.run(c["args"]["passes"])
         eng.sync()
         rows, pl = eng.fetch()
         assert rows[0].err == 0
-        assert eng.describe()["ep_cap"] >= c["result"]["max_cand"]
+        assert rows[0].ep_need > 16
+        assert eng.describe()["ep_cap"] >= rows[0].ep_need
         assert plp.to_json_dict(rows[0], pl[0]) == ref_fields(c["result"]), c["case"]
+        if count_work:
+            assert rows[0].work_pair_box == sum(c["result"]["pass_pair_box"]), c["case"]
+            assert rows[0].max_ep >= c["result"]["max_cand"], c["case"]
 
 
 def test_engine_graph_replay_matches_eager(plp, ctx, ref_runs):
@@ -250,7 +256,8 @@ def test_engine_many_decoders_vs_oracle(plp, ctx, orc, wpd):
     inst = plp.Instance.from_text(text)
     o_inst = orc.parse(text)[0]
     K = 256
-    eng = plp.Engine(ctx, waves_per_decoder=wpd)
+    count = wpd == 2     # one width also checks the reference's work count
+    eng = plp.Engine(ctx, waves_per_decoder=wpd, count_work=count)
     eng.load([inst], [0] * K, list(range(1, K + 1)), plp.params(eps=0.01, theta=0.9))
     eng.run(2)
     eng.sync()
@@ -259,8 +266,12 @@ def test_engine_many_decoders_vs_oracle(plp, ctx, orc, wpd):
         want = orc.run(o_inst, seed=k + 1, passes=2, eps=0.01, theta=0.9)
         got = plp.to_json_dict(rows[k], pl[k])
         assert got == {x: want[x] for x in got}, k
-        assert rows[k].work_pair_box == want["pair_box"], k
-        assert 0 < rows[k].work_live_pair_box <= rows[k].work_pair_box, k
+        assert 0 < rows[k].work_live_pair_box, k
+        if count:
+            assert rows[k].work_pair_box == want["pair_box"], k
+            assert rows[k].work_live_pair_box <= rows[k].work_pair_box, k
+        else:
+            assert rows[k].work_pair_box == 0, k
 
 
 @pytest.mark.parametrize("wpd", [1, 4])
