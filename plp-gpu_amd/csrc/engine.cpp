@@ -61,10 +61,23 @@ void drop_graph(plp_engine* g) {
 }
 
 // Launch shape for an EP capacity: LDS bytes, resident blocks per CU, grid.
+// Dynamic plus static LDS of one decoder block for a capacity.
+bool lds_fits(plp_engine* g, int ep_cap, int* dyn_bytes) {
+    const plpk::LdsLayout lay = plpk::lds_layout(ep_cap, g->box_cap, g->use_rng, g->wpd,
+                                                 g->cfg.count_work, g->wide);
+    int st = 0;
+    if (plpk::decode_static_lds(g->wpd, &st) != hipSuccess) {
+        (void)hipGetLastError();
+        st = 4096;
+    }
+    if (dyn_bytes) *dyn_bytes = lay.total;
+    return lay.total + st <= g->ctx->max_lds;
+}
+
 int shape(plp_engine* g, int ep_cap, int blocks_per_cu_req, int n_work) {
     const plpk::LdsLayout lay = plpk::lds_layout(ep_cap, g->box_cap, g->use_rng, g->wpd,
                                                  g->cfg.count_work, g->wide);
-    if (lay.total + 256 > g->ctx->max_lds)
+    if (!lds_fits(g, ep_cap, nullptr))
         return fail(PLP_E_EPCAP, "decoder state (" + std::to_string(lay.total) +
                                      " B of LDS for " + std::to_string(ep_cap) +
                                      " extreme points) exceeds the block LDS limit");
@@ -78,6 +91,18 @@ int shape(plp_engine* g, int ep_cap, int blocks_per_cu_req, int n_work) {
     g->occ = occ;
     g->grid = std::max(1, std::min(n_work, bpc * g->ctx->n_cu));
     return PLP_OK;
+}
+
+// The largest extreme-point capacity whose LDS one block can hold.
+int largest_cap(plp_engine* g) {
+    int lo = 64, hi = 1 << 15;
+    if (!lds_fits(g, lo, nullptr)) return 0;
+    while (hi - lo > 32) {
+        const int mid = (lo + hi) / 2 / 32 * 32;
+        if (lds_fits(g, mid, nullptr)) lo = mid;
+        else hi = mid;
+    }
+    return lo;
 }
 
 plpk::DecodeArgs make_args(plp_engine* g, int n_passes, int reset, const int32_t* order,
@@ -229,6 +254,8 @@ extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, in
         h.step_off = (int32_t)steps.size();
         h.ors_off = (int32_t)(ors.size() / plpk::kOrsRow);
         h.n_types = (int32_t)in.items.size();
+        if (h.n_types >= (1 << 24))   // the decoder's step table packs the type into 24 bits
+            return fail(PLP_E_RANGE, "instance " + std::to_string(i) + ": more than 2^24 - 1 types");
         h.xbits = bitlen(in.L);
         h.ybits = bitlen(in.W);
         steps.insert(steps.end(), seq.begin(), seq.end());
@@ -364,8 +391,12 @@ extern "C" int plp_engine_sync(plp_engine* g) {
     if (rc != PLP_OK) return rc;
     // Capacity escalation: a decoder whose extreme-point list outgrew the
     // LDS capacity stopped with err set; rerun exactly those decoders from
-    // their seed with a doubled capacity.  Results do not depend on the
-    // capacity, so the rerun reproduces what an unlimited run would give.
+    // their seed with the largest capacity one block can hold.  Results do
+    // not depend on the capacity, so the rerun reproduces what an unlimited
+    // run would give.  Afterwards the capacity is the largest list any
+    // decoder needed, which these decoders' later runs fit with the best
+    // residency.
+    bool escalated = false;
     for (;;) {
         std::vector<int32_t> failed;
         int need = 0;
@@ -374,34 +405,37 @@ extern "C" int plp_engine_sync(plp_engine* g) {
                 failed.push_back(d);
                 need = std::max(need, (int)g->h_state[(size_t)d].ep_need);
             }
-        if (failed.empty()) return PLP_OK;
-        int cap = std::max(round_up(2 * g->ep_cap, 64), round_up(need + need / 4, 64));
-        rc = shape(g, cap, g->cfg.blocks_per_cu, g->n_dec);
-        if (rc != PLP_OK) {
-            fail(PLP_E_EPCAP, std::to_string(failed.size()) +
-                                  " decoder(s) exceeded the extreme-point capacity: " + plp_last_error());
-            return PLP_E_EPCAP;
-        }
+        if (failed.empty()) break;
+        const int cap = largest_cap(g);
+        if (cap <= g->ep_cap || cap < need || shape(g, cap, 1, g->n_dec) != PLP_OK)
+            return fail(PLP_E_EPCAP, std::to_string(failed.size()) +
+                                         " decoder(s) need more than " + std::to_string(g->ep_cap) +
+                                         " extreme points, beyond the block LDS limit");
         drop_graph(g);
         e = hipMemcpyAsync(g->d_retry.p, failed.data(), 4 * failed.size(), hipMemcpyHostToDevice,
                            c->stream);
         if (e != hipSuccess) return hip_fail(e, "H2D retry order");
-        const int grid_all = g->grid;
-        g->grid = std::max(1, std::min((int)failed.size(), grid_all));
+        g->grid = std::max(1, std::min((int)failed.size(), g->grid));
         plpk::DecodeArgs a = make_args(g, g->passes_done, 1, (const int32_t*)g->d_retry.p,
                                        (int)failed.size());
         if ((e = hipEventRecord(g->ev0, c->stream)) != hipSuccess) return hip_fail(e, "event");
         e = plpk::launch_decode(a, g->wpd, g->grid, g->lds, c->stream);
-        g->grid = std::max(1, std::min(g->n_dec, (g->cfg.blocks_per_cu > 0
-                                                      ? std::min(g->cfg.blocks_per_cu, g->occ)
-                                                      : g->occ) * c->n_cu));
         if (e != hipSuccess) return hip_fail(e, "decode retry launch");
         if ((e = hipEventRecord(g->ev1, c->stream)) != hipSuccess) return hip_fail(e, "event");
         if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "retry sync");
         if (hipEventElapsedTime(&ms, g->ev0, g->ev1) == hipSuccess) g->last_ms += ms;
         else (void)hipGetLastError();
         if ((rc = read_states(g)) != PLP_OK) return rc;
+        escalated = true;
     }
+    if (escalated) {
+        int need_all = 64;
+        for (const DecState& st : g->h_state) need_all = std::max(need_all, (int)st.ep_need);
+        if ((rc = shape(g, round_up(need_all, 32), g->cfg.blocks_per_cu, g->n_dec)) != PLP_OK)
+            return rc;
+        drop_graph(g);
+    }
+    return PLP_OK;
 }
 
 extern "C" int plp_engine_max_items(plp_engine* g, int32_t* max_items) {

@@ -72,11 +72,12 @@ struct DecodeArgs {
 // dead-point list when work is counted, pair masks, per-wave culled box
 // copies) is reused by FSR after the pass.  Each list has ep_cap + 8 slots: a
 // merge writes at most 8 entries past the capacity before the overflow is
-// detected.  The box array has box_cap slots plus one sentinel.
+// detected.  The box array has box_cap slots plus one sentinel; the step
+// table (box_cap >= steps) follows the MT state.
 struct LdsLayout {
     int32_t off_ep, off_ep2, off_oidx, off_cache, n_cache, off_dead, off_mask, n_mask, off_rel;
     int32_t off_load, off_key, n_key;       // FSR view of the first region
-    int32_t off_box, off_mt, total;
+    int32_t off_box, off_mt, off_steps, total;
 };
 
 __host__ __device__ inline int32_t next_pow2(int32_t v) {
@@ -111,7 +112,8 @@ __host__ __device__ inline LdsLayout lds_layout(int32_t ep_cap, int32_t box_cap,
     int32_t region = pass_end > fsr_end ? pass_end : fsr_end;
     l.off_box = (region + 15) & ~15;
     l.off_mt = l.off_box + (box_cap + 8) * 32;
-    l.total = l.off_mt + (use_rng ? 624 * 4 : 0);
+    l.off_steps = l.off_mt + (use_rng ? 624 * 4 : 0);   // per step: type | n_ors << 24
+    l.total = l.off_steps + box_cap * 4;
     return l;
 }
 

@@ -124,6 +124,7 @@ __device__ __forceinline__ bool pair_verdict(bool c12, int z, int w, int d, uint
 // more box follows from the old state and that box alone.
 constexpr uint32_t kBlocked = 0xffffffffu;
 constexpr uint16_t kNewPoint = 0xffffu;   // list entry with no cached state
+constexpr uint16_t kCorner = 0xfff0u;     // kCorner + j: corner j of the last box (state in cstate)
 
 // Entries are 32-bit, or 16-bit with the support sum clamped to 65534 and
 // 0xffff for kBlocked (exact for C3 when every w*d <= kNarrowMaxArea).
@@ -228,26 +229,42 @@ __device__ double fsr_score(const Box* __restrict__ box, int n, double mu, doubl
 // (plp_common.hpp:439-453): points strictly inside the box leave the list,
 // the box's 8 corners join it unless already present, and the list stays
 // sorted and unique.  On the live list src -> dst: the points in the box's
-// half-open extent die (the strictly inside ones among them), a live corner
-// replaces an equal live point, dead corners are not added.  With count_work
-// the dead list (unsorted) drops its strictly inside points and receives the
-// points that die without being strictly inside and the dead corners found
-// in neither list.  n_act <= cap on entry keeps every write within the
-// cap + 8 slots of a list.  One wave; returns the larger new list size.
+// half-open extent die (the strictly inside ones among them) and the live
+// corners not already in src are inserted; oidx[] records where each dst
+// entry was in src (kCorner + j for inserted corner j).  Only corner 0 (the
+// placement point itself) lies in the half-open extent, so no removed point
+// equals a live corner.  With count_work the dead list (unsorted) drops its
+// strictly inside points and receives the points that die without being
+// strictly inside and the dead corners found in neither list.
+// n_act <= cap on entry keeps every write within the cap + 8 slots of a
+// list.  One wave; returns the larger new list size.
 __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t* oidx, uint32_t* dead,
                             int& n_act, int& n_dead, const Box* box, int n_box, int x, int y, int z,
                             int X2, int Y2, int Z2, int L, int W, int H, int xb, int yb, int cap,
-                            bool count_work, int* jold) {
+                            bool count_work) {
     const int lane = lane_id();
-    if (lane < 8) jold[lane] = kNewPoint;
-    // Corner j: bit 0 -> x side, bit 1 -> y, bit 2 -> z.
+    // Corner j: bit 0 -> x side, bit 1 -> y, bit 2 -> z.  Lane j < 8 holds
+    // corner j's key in myk.
     uint32_t ck[8];
     uint32_t cdead = 0;
+    uint32_t myk = 0;
 #pragma unroll
     for (int j = 0; j < 8; j++) {
         const int jx = (j & 1) ? X2 : x, jy = (j & 2) ? Y2 : y, jz = (j & 4) ? Z2 : z;
         ck[j] = xyz_key(jx, jy, jz, xb, yb);
+        if (lane == j) myk = ck[j];
         if ((jx >= L) | (jy >= W) | (jz >= H)) cdead |= 1u << j;
+    }
+    // Corners inside a placed box (the new one included) are dead.  Lanes
+    // 0..7 meanwhile find each corner's lower bound in src.
+    int lo = 0;
+    if (lane < 8) {
+        int hi = n_act;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (src[mid] < myk) lo = mid + 1;
+            else hi = mid;
+        }
     }
     for (int b0 = 0; b0 < n_box; b0 += kWave) {
         const int bi = b0 + lane;
@@ -261,7 +278,8 @@ __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t* oidx, 
             if (ballot(in)) cdead |= 1u << j;
         }
     }
-    const uint32_t clive = ~cdead & 0xffu;
+    const uint32_t exists = (uint32_t)ballot(lane < 8 && lo < n_act && src[lo] == myk) & 0xffu;
+    const uint32_t add = ~cdead & ~exists & 0xffu;   // live corners that join the list
 
     // Dead list: drop the strictly inside points in place (a point moves to
     // an index no larger than its own, below the next 64-point chunk).
@@ -288,81 +306,98 @@ __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t* oidx, 
         wave_sync();
     }
 
-    // Live list: survivors go to dst after the live corners below them;
-    // oidx[] records where each dst entry was in src (a live corner that
-    // replaces an equal point takes that point's index).
-    wave_sync();
-    int lt[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) lt[j] = 0;
-    uint32_t eq_live = 0;
-    int base = 0;
+    // Live list: survivors go to dst after the inserted corners below them;
+    // lane j < 8 counts the removed points below corner j (rb).
+    int base = 0, rb = 0;
     for (int i0 = 0; i0 < n_act; i0 += kWave) {
         const int i = i0 + lane;
         const bool ok = i < n_act;
         const uint32_t v = ok ? src[i] : 0u;
         int px, py, pz;
         key_xyz(v, xb, yb, px, py, pz);
-        const bool half = (x <= px) & (px < X2) & (y <= py) & (py < Y2) & (z <= pz) & (pz < Z2);
-        bool dup = false;
+        const bool half = ok & (x <= px) & (px < X2) & (y <= py) & (py < Y2) & (z <= pz) &
+                          (pz < Z2);
         int shift = 0;
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const bool e = ok & (v == ck[j]);
-            if (ballot(e)) eq_live |= 1u << j;
-            const bool lj = (clive >> j) & 1u;
-            if (e & lj) jold[j] = i;
-            dup |= e & lj;
-            shift += (lj & (ck[j] < v)) ? 1 : 0;
-        }
-        const bool keep = ok & !half & !dup;
+        for (int j = 0; j < 8; j++) shift += (((add >> j) & 1u) && ck[j] < v) ? 1 : 0;
+        const bool keep = ok & !half;
         const uint64_t km = ballot(keep);
         if (keep) {
             const int pos = base + rank_below(km) + shift;
             dst[pos] = v;
             oidx[pos] = (uint16_t)i;
         }
-#pragma unroll
-        for (int j = 0; j < 8; j++) lt[j] += __popcll(ballot(keep & (v < ck[j])));
         base += __popcll(km);
-        if (count_work) {
-            const bool strict = (x < px) & (px < X2) & (y < py) & (py < Y2) & (z < pz) & (pz < Z2);
-            const bool mv = ok & half & !strict;
-            const uint64_t mm = ballot(mv);
-            const int pos = nd + rank_below(mm);
-            if (mv && pos < cap + 8) dead[pos] = v;
-            nd += __popcll(mm);
+        const uint64_t hm = ballot(half);
+        if (hm) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int c = __popcll(hm & ballot(v < ck[j]));
+                if (lane == j) rb += c;
+            }
+            if (count_work) {
+                const bool strict = (x < px) & (px < X2) & (y < py) & (py < Y2) & (z < pz) &
+                                    (pz < Z2);
+                const bool mv = half & !strict;
+                const uint64_t mm = ballot(mv);
+                const int pos = nd + rank_below(mm);
+                if (mv && pos < cap + 8) dead[pos] = v;
+                nd += __popcll(mm);
+            }
         }
     }
     base = uni(base);
-    uint32_t myk = ck[0];
-    int mylt = lt[0];
-#pragma unroll
-    for (int j = 1; j < 8; j++)
-        if (lane == j) {
-            myk = ck[j];
-            mylt = lt[j];
-        }
-    wave_sync();
-    if (lane < 8 && ((clive >> lane) & 1u)) {
+    if (lane < 8 && ((add >> lane) & 1u)) {
         int r = 0;
 #pragma unroll
-        for (int j = 0; j < 8; j++) r += (((clive >> j) & 1u) && ck[j] < myk) ? 1 : 0;
-        dst[mylt + r] = myk;
-        oidx[mylt + r] = (uint16_t)jold[lane];
+        for (int j = 0; j < 8; j++) r += (((add >> j) & 1u) && ck[j] < myk) ? 1 : 0;
+        dst[lo - rb + r] = myk;
+        oidx[lo - rb + r] = (uint16_t)(kCorner + lane);
     }
     if (count_work) {
-        const uint32_t add = cdead & ~eq_live & ~eq_dead;
-        if (lane < 8 && ((add >> lane) & 1u)) {
-            const int pos = nd + __popc(add & ((1u << lane) - 1u));
+        const uint32_t dadd = cdead & ~exists & ~eq_dead;
+        if (lane < 8 && ((dadd >> lane) & 1u)) {
+            const int pos = nd + __popc(dadd & ((1u << lane) - 1u));
             if (pos < cap + 8) dead[pos] = myk;
         }
-        nd += __popc(add);
+        nd += __popc(dadd);
     }
     wave_sync();
-    n_act = base + __popc(clive);
+    n_act = base + __popc(add);
     n_dead = count_work ? uni(nd) : 0;
     return max(n_act, n_dead);
+}
+
+// The pair states (cache entries) of the 8 corners of the box nb just
+// placed, for the orientations in ors (no of them), against all n_box boxes:
+// lane j * no + o.  One wave.
+__device__ void corner_states(const int* nb, uint32_t* cstate, const int* ors, int no,
+                              const Box* box, int n_box, int L, int W, int H, Box* cbox) {
+    const int lane = lane_id();
+    const int n = 8 * no;
+    const int j = lane / no, o = lane - (lane / no) * no;
+    int x = 0, y = 0, z = 0, w = 1, d = 1, h = 1;
+    if (lane < n) {
+        x = (j & 1) ? nb[3] : nb[0];
+        y = (j & 2) ? nb[4] : nb[1];
+        z = (j & 4) ? nb[5] : nb[2];
+        w = ors[3 * o];
+        d = ors[3 * o + 1];
+        h = ors[3 * o + 2];
+    }
+    const bool c2 = lane < n && (x + w <= L) && (y + d <= W) && (z + h <= H);
+    bool c12 = false;
+    uint32_t s = 0;
+    if (ballot(c2) != 0) {
+        const int xlo = wave_min(c2 ? x : 0x7fffffff);
+        const int xhi = wave_max(c2 ? x + w : -1);
+        const int ylo = wave_min(c2 ? y : 0x7fffffff);
+        const int yhi = wave_max(c2 ? y + d : -1);
+        const int zlo = wave_min(c2 ? z : 0x7fffffff);
+        const int zhi = wave_max(c2 ? z + h : -1);
+        c12 = pair_scan(x, y, z, w, d, h, c2, box, n_box, xlo, xhi, ylo, yhi, zlo, zhi, cbox, s);
+    }
+    if (lane < n) cstate[lane] = c12 ? s : kBlocked;
 }
 
 // ------------------------------------------------------------ decoder ----
@@ -375,7 +410,8 @@ struct StepShared {
     int quit;       // the decoder stopped (capacity error): leave the pass loop
     int n_act;      // live extreme points after the last placement
     int n_box;      // placed boxes after the last placement
-    int jold[8];    // merge: previous-list index of a live point equal to corner j
+    int nb[6];      // the box just placed: x, y, z, x2, y2, z2
+    uint32_t cstate[48];   // pair states of its 8 corners x this type's orientations
     int cnt[2][8];  // feasible pairs found by each wave, by evaluated-step parity:
                     // a wave may start step k+1 while others still read step k
 };
@@ -437,6 +473,7 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
     uint64_t* skey = (uint64_t*)(lds + lay.off_key);
     Box* box = (Box*)(lds + lay.off_box);
     uint32_t* mt = (uint32_t*)(lds + lay.off_mt);
+    int32_t* s_step = (int32_t*)(lds + lay.off_steps);
     const int lane = lane_id();
     const int wid = (int)(threadIdx.x >> 6);
     Box* cbox = (Box*)(lds + lay.off_rel) + wid * kCullWindow;   // this wave's culled window
@@ -452,6 +489,12 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
         const InstHdr I = a.inst[uni(a.dec_inst[d])];
         const int L = I.L, W = I.W, H = I.H, xb = I.xbits, yb = I.ybits;
         const int32_t* steps = a.step_type + I.step_off;
+        // Step table in LDS: type | n_ors << 24 (types < 2^24 at load time).
+        for (int i = threadIdx.x; i < I.n_steps; i += 64 * NW) {
+            const int t = steps[i];
+            s_step[i] = t | (a.ors_tab[(size_t)(I.ors_off + t) * kOrsRow] << 24);
+        }
+        int cur_ti = -1;   // type whose orientations this wave holds in s_orw
 
         // ---- decoder state: in LDS (sst) between pass ends; wave 0
         int mti = 624;
@@ -508,9 +551,8 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
             int cur = 0;    // live list in use: 0 -> ep_a, 1 -> ep_b (every wave)
             int prev_ti = -1;   // type of the previous evaluated step
             for (int s = 0; s < I.n_steps; s++) {
-                const int ti = uni(steps[s]);
-                const int32_t* orow = a.ors_tab + (size_t)(I.ors_off + ti) * kOrsRow;
-                const int no = uni(orow[0]);
+                const int sv = uni(s_step[s]);
+                const int ti = sv & 0xffffff, no = sv >> 24;
                 if (no == 0) continue;      // type with no allowed orientation
                 if (wid == 0) {
                     if (a.count_work)
@@ -521,7 +563,11 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                 // the verdicts repeat, nothing is drawn.
                 if (ti == fail_ti) continue;
                 if (wid == 0) live += (uint64_t)n_act * (uint64_t)no * (uint64_t)n_box;
-                if (lane < 3 * no) s_orw[lane] = orow[1 + lane];
+                if (ti != cur_ti) {
+                    const int32_t* orow = a.ors_tab + (size_t)(I.ors_off + ti) * kOrsRow;
+                    if (lane < 3 * no) s_orw[lane] = orow[1 + lane];
+                    cur_ti = ti;
+                }
                 wave_sync();
                 const uint32_t* act = cur ? ep_b : ep_a;
 
@@ -559,7 +605,12 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                     bool c12 = valid && (x + w <= L) && (y + dd <= W) && (z + h <= H);
                     uint32_t sup = 0;
                     bool scan = c12;
-                    if (inc && c12 && oi != kNewPoint) {
+                    if (inc && c12 && oi >= kCorner && oi != kNewPoint) {
+                        scan = false;
+                        const uint32_t cs = sh.cstate[(oi - kCorner) * no + o];
+                        c12 = cs != kBlocked;
+                        sup = cs;
+                    } else if (inc && c12 && oi != kNewPoint) {
                         scan = false;
                         const uint32_t old = cache_get(c_rd, oi * no + o, wide);
                         const bool ov = (x < nx2) & (nx < x + w) & (y < ny2) & (ny < y + dd) &
@@ -607,7 +658,8 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                 }
                 fail_ti = -1;
 
-                // ---- pick, place, extreme-point merge (wave 0)
+                // ---- pick and place (wave 0)
+                int x = 0, y = 0, z = 0, X2 = 0, Y2 = 0, Z2 = 0;
                 if (wid == 0) {
                     // feas[0], or feas[randrange(|feas|)] when random() >=
                     // 1 - eps (plp_common.hpp:427-435).
@@ -628,26 +680,34 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                     const bool hit = ((m >> lane) & 1ull) && rank_below(m) == idx;
                     const int bit = __ffsll((unsigned long long)ballot(hit)) - 1;
                     const int p = c * kWave + bit;
-                    const int e = p / no, o = p - (p / no) * no;
-                    int x, y, z;
+                    const int e = no == 1 ? p : (int)__umulhi((uint32_t)p, magic);
+                    const int o = p - e * no;
                     key_xyz(uni(act[e]), xb, yb, x, y, z);
                     x = uni(x);
                     y = uni(y);
                     z = uni(z);
-                    const int w = uni(s_orw[3 * o]), dd = uni(s_orw[3 * o + 1]),
-                              h = uni(s_orw[3 * o + 2]);
-                    const int X2 = x + w, Y2 = y + dd, Z2 = z + h;
+                    X2 = x + uni(s_orw[3 * o]);
+                    Y2 = y + uni(s_orw[3 * o + 1]);
+                    Z2 = z + uni(s_orw[3 * o + 2]);
                     if (lane == 0) {
                         Box nbx;
                         nbx.x = x; nbx.y = y; nbx.z = z; nbx.x2 = X2; nbx.y2 = Y2; nbx.z2 = Z2;
                         nbx.type = ti; nbx.pad = 0;
                         box[n_box] = nbx;
+                        sh.nb[0] = x; sh.nb[1] = y; sh.nb[2] = z;
+                        sh.nb[3] = X2; sh.nb[4] = Y2; sh.nb[5] = Z2;
                     }
                     n_box++;
                     wave_sync();
+                }
+                if (NW > 1) block_sync();   // every wave sees the new box
+                // ---- extreme-point merge (wave 0) and, meanwhile, the pair
+                // states of the new box's corners for this type (last wave),
+                // which the next step takes when it has the same type.
+                if (wid == 0) {
                     const int need = merge_points(act, cur ? ep_a : ep_b, oidx, dead, n_act, n_dead,
                                                   box, n_box, x, y, z, X2, Y2, Z2, L, W, H, xb, yb,
-                                                  a.ep_cap, a.count_work != 0, sh.jold);
+                                                  a.ep_cap, a.count_work != 0);
                     max_ep = max(max_ep, (uint32_t)(n_act + n_dead));
                     ep_need = max(ep_need, (uint32_t)need);
                     if (lane == 0) {
@@ -656,6 +716,8 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                         sh.n_box = n_box;
                     }
                 }
+                if (wid == NW - 1)
+                    corner_states(sh.nb, sh.cstate, s_orw, no, box, nb_w + 1, L, W, H, cbox);
                 block_sync();
                 if (uni(sh.quit) != 0) break;
                 na_w = uni(sh.n_act);
@@ -854,6 +916,24 @@ static hipError_t occupancy_of(int lds_bytes, int* blocks_per_cu) {
     if (e != hipSuccess) return e;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, decode_kernel<NW>, 64 * NW,
                                                         (size_t)lds_bytes);
+}
+
+template <int NW>
+static hipError_t static_lds_of(int* bytes) {
+    hipFuncAttributes at;
+    hipError_t e = hipFuncGetAttributes(&at, (const void*)decode_kernel<NW>);
+    if (e == hipSuccess) *bytes = (int)at.sharedSizeBytes;
+    return e;
+}
+
+hipError_t decode_static_lds(int waves, int* bytes) {
+    switch (waves) {
+        case 1: return static_lds_of<1>(bytes);
+        case 2: return static_lds_of<2>(bytes);
+        case 4: return static_lds_of<4>(bytes);
+        case 8: return static_lds_of<8>(bytes);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t decode_occupancy(int waves, int lds_bytes, int* blocks_per_cu) {
