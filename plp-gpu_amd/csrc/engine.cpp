@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -35,7 +36,7 @@ struct plp_engine {
     int passes_done = 0;
     bool need_reset = true;
     bool states_fresh = false;
-    DevBuf d_inst, d_step, d_ors, d_dec_inst, d_dec_seed, d_order, d_retry, d_mt, d_state,
+    DevBuf d_inst, d_step, d_step_min, d_ors, d_dec_inst, d_dec_seed, d_order, d_retry, d_mt, d_state,
         d_best_all, d_best_acc, d_counter;
     std::vector<int32_t> h_order;
     std::vector<int32_t> h_dec_inst;
@@ -49,6 +50,10 @@ struct plp_engine {
 };
 
 namespace {
+
+// "No later box": larger than any container side (< 2^21), small enough that
+// a coordinate plus it stays in int32.
+constexpr int32_t kNoBox = 1 << 30;
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
@@ -110,6 +115,7 @@ plpk::DecodeArgs make_args(plp_engine* g, int n_passes, int reset, const int32_t
     plpk::DecodeArgs a;
     a.inst = (const InstHdr*)g->d_inst.p;
     a.step_type = (const int32_t*)g->d_step.p;
+    a.step_min = (const int32_t*)g->d_step_min.p;
     a.ors_tab = (const int32_t*)g->d_ors.p;
     a.dec_inst = (const int32_t*)g->d_dec_inst.p;
     a.dec_seed = (const uint64_t*)g->d_dec_seed.p;
@@ -212,7 +218,7 @@ extern "C" void plp_engine_destroy(plp_engine* g) {
     (void)hipSetDevice(g->ctx->device);
     (void)hipStreamSynchronize(g->ctx->stream);
     drop_graph(g);
-    for (DevBuf* b : {&g->d_inst, &g->d_step, &g->d_ors, &g->d_dec_inst, &g->d_dec_seed,
+    for (DevBuf* b : {&g->d_inst, &g->d_step, &g->d_step_min, &g->d_ors, &g->d_dec_inst, &g->d_dec_seed,
                       &g->d_order, &g->d_retry, &g->d_mt, &g->d_state, &g->d_best_all,
                       &g->d_best_acc, &g->d_counter})
         b->release();
@@ -236,6 +242,7 @@ extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, in
     g->loaded = false;
     std::vector<InstHdr> hdr((size_t)n_inst);
     std::vector<int32_t> steps;
+    std::vector<int32_t> step_min;   // per step: min w, d, h over the later steps, 0
     std::vector<int32_t> ors;
     std::vector<int64_t> cost((size_t)n_inst, 0);
     int max_items = 1;
@@ -259,6 +266,22 @@ extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, in
         h.xbits = bitlen(in.L);
         h.ybits = bitlen(in.W);
         steps.insert(steps.end(), seq.begin(), seq.end());
+        // Componentwise smallest orientation of every type; a point where a
+        // box of the smallest dimensions of all later steps cannot go can
+        // host none of them.
+        std::vector<std::array<int32_t, 3>> tmin(in.items.size(), {kNoBox, kNoBox, kNoBox});
+        for (size_t t = 0; t < in.items.size(); t++)
+            for (const auto& o : plp::orientations(in.items[t]))
+                for (int c = 0; c < 3; c++) tmin[t][(size_t)c] = std::min(tmin[t][(size_t)c], (int32_t)o[(size_t)c]);
+        {
+            std::vector<int32_t> mins(4 * seq.size(), 0);
+            std::array<int32_t, 3> m = {kNoBox, kNoBox, kNoBox};
+            for (size_t k = seq.size(); k-- > 0;) {
+                for (int c = 0; c < 3; c++) mins[4 * k + (size_t)c] = m[(size_t)c];
+                for (int c = 0; c < 3; c++) m[(size_t)c] = std::min(m[(size_t)c], tmin[(size_t)seq[k]][(size_t)c]);
+            }
+            step_min.insert(step_min.end(), mins.begin(), mins.end());
+        }
         for (const plp::Item& it : in.items) {
             auto o = plp::orientations(it);
             int32_t row[plpk::kOrsRow] = {0};
@@ -276,6 +299,7 @@ extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, in
     }
     if (ors.empty()) ors.assign(plpk::kOrsRow, 0);
     if (steps.empty()) steps.push_back(0);
+    if (step_min.empty()) step_min.assign(4, kNoBox);
     for (int d = 0; d < n_dec; d++)
         if (dec_inst[d] < 0 || dec_inst[d] >= n_inst)
             return fail(PLP_E_ARG, "dec_inst out of range");
@@ -299,6 +323,7 @@ extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, in
     int rc;
     if ((rc = g->d_inst.reserve(sizeof(InstHdr) * hdr.size(), "instance headers")) ||
         (rc = g->d_step.reserve(4 * steps.size(), "step table")) ||
+        (rc = g->d_step_min.reserve(4 * step_min.size(), "step minimum dimensions")) ||
         (rc = g->d_ors.reserve(4 * ors.size(), "orientation table")) ||
         (rc = g->d_dec_inst.reserve(4 * (size_t)n_dec, "decoder map")) ||
         (rc = g->d_dec_seed.reserve(8 * (size_t)n_dec, "decoder seeds")) ||
@@ -316,6 +341,7 @@ extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, in
     } ups[] = {
         {g->d_inst.p, hdr.data(), sizeof(InstHdr) * hdr.size()},
         {g->d_step.p, steps.data(), 4 * steps.size()},
+        {g->d_step_min.p, step_min.data(), 4 * step_min.size()},
         {g->d_ors.p, ors.data(), 4 * ors.size()},
         {g->d_dec_inst.p, dec_inst, 4 * (size_t)n_dec},
         {g->d_dec_seed.p, dec_seed, 8 * (size_t)n_dec},

@@ -44,6 +44,7 @@ constexpr int kMtWords = 625;      // 624 state words + index
 struct DecodeArgs {
     const InstHdr* inst;
     const int32_t* step_type;
+    const int32_t* step_min;     // 4 ints per step: smallest w, d, h over the later steps
     const int32_t* ors_tab;
     const int32_t* dec_inst;
     const uint64_t* dec_seed;

@@ -228,20 +228,26 @@ __device__ double fsr_score(const Box* __restrict__ box, int n, double mu, doubl
 // Extreme-point update after placing the box [x, X2) x [y, Y2) x [z, Z2)
 // (plp_common.hpp:439-453): points strictly inside the box leave the list,
 // the box's 8 corners join it unless already present, and the list stays
-// sorted and unique.  On the live list src -> dst: the points in the box's
-// half-open extent die (the strictly inside ones among them) and the live
-// corners not already in src are inserted; oidx[] records where each dst
-// entry was in src (kCorner + j for inserted corner j).  Only corner 0 (the
-// placement point itself) lies in the half-open extent, so no removed point
-// equals a live corner.  With count_work the dead list (unsorted) drops its
-// strictly inside points and receives the points that die without being
-// strictly inside and the dead corners found in neither list.
-// n_act <= cap on entry keeps every write within the cap + 8 slots of a
-// list.  One wave; returns the larger new list size.
+// sorted and unique.  The live list keeps only points where a box of the
+// later steps' smallest dimensions (wm, dm, hm, componentwise over every
+// orientation) fits inside the walls without overlapping a placed box: any
+// later box contains that one, so elsewhere nothing can ever be placed.  On
+// the live list src -> dst: the points whose smallest box meets the new box
+// or a wall leave (the strictly inside ones among them leave the reference's
+// list too), and the useful corners not already in src are inserted; oidx[]
+// records where each dst entry was in src (kCorner + j for inserted corner
+// j).  Usefulness only shrinks as boxes are added and the later steps'
+// smallest box grows, so a useful corner never equals a point that left, and
+// a corner equal to a point still listed is not inserted.  With count_work
+// the other points of the reference's list are kept (unsorted) in dead[]: it
+// drops its strictly inside points and receives the points that leave the
+// live list without being strictly inside and the corners that are neither
+// useful nor already listed.  n_act <= cap on entry keeps every write within
+// the cap + 8 slots of a list.  One wave; returns the larger new list size.
 __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t* oidx, uint32_t* dead,
                             int& n_act, int& n_dead, const Box* box, int n_box, int x, int y, int z,
                             int X2, int Y2, int Z2, int L, int W, int H, int xb, int yb, int cap,
-                            bool count_work) {
+                            bool count_work, int wm, int dm, int hm) {
     const int lane = lane_id();
     // Corner j: bit 0 -> x side, bit 1 -> y, bit 2 -> z.  Lane j < 8 holds
     // corner j's key in myk.
@@ -253,10 +259,11 @@ __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t* oidx, 
         const int jx = (j & 1) ? X2 : x, jy = (j & 2) ? Y2 : y, jz = (j & 4) ? Z2 : z;
         ck[j] = xyz_key(jx, jy, jz, xb, yb);
         if (lane == j) myk = ck[j];
-        if ((jx >= L) | (jy >= W) | (jz >= H)) cdead |= 1u << j;
+        if ((jx + wm > L) | (jy + dm > W) | (jz + hm > H)) cdead |= 1u << j;
     }
-    // Corners inside a placed box (the new one included) are dead.  Lanes
-    // 0..7 meanwhile find each corner's lower bound in src.
+    // Corners whose smallest box meets a placed box (the new one included)
+    // are not useful.  Lanes 0..7 meanwhile find each corner's lower bound
+    // in src.
     int lo = 0;
     if (lane < 8) {
         int hi = n_act;
@@ -273,13 +280,13 @@ __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t* oidx, 
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int jx = (j & 1) ? X2 : x, jy = (j & 2) ? Y2 : y, jz = (j & 4) ? Z2 : z;
-            const bool in = (B.x <= jx) & (jx < B.x2) & (B.y <= jy) & (jy < B.y2) &
-                            (B.z <= jz) & (jz < B.z2);
+            const bool in = (jx < B.x2) & (B.x < jx + wm) & (jy < B.y2) & (B.y < jy + dm) &
+                            (jz < B.z2) & (B.z < jz + hm);
             if (ballot(in)) cdead |= 1u << j;
         }
     }
     const uint32_t exists = (uint32_t)ballot(lane < 8 && lo < n_act && src[lo] == myk) & 0xffu;
-    const uint32_t add = ~cdead & ~exists & 0xffu;   // live corners that join the list
+    const uint32_t add = ~cdead & ~exists & 0xffu;   // useful corners that join the list
 
     // Dead list: drop the strictly inside points in place (a point moves to
     // an index no larger than its own, below the next 64-point chunk).
@@ -307,7 +314,7 @@ __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t* oidx, 
     }
 
     // Live list: survivors go to dst after the inserted corners below them;
-    // lane j < 8 counts the removed points below corner j (rb).
+    // lane j < 8 counts the points below corner j that leave (rb).
     int base = 0, rb = 0;
     for (int i0 = 0; i0 < n_act; i0 += kWave) {
         const int i = i0 + lane;
@@ -315,12 +322,13 @@ __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t* oidx, 
         const uint32_t v = ok ? src[i] : 0u;
         int px, py, pz;
         key_xyz(v, xb, yb, px, py, pz);
-        const bool half = ok & (x <= px) & (px < X2) & (y <= py) & (py < Y2) & (z <= pz) &
-                          (pz < Z2);
+        const bool gone = ok & ((px + wm > L) | (py + dm > W) | (pz + hm > H) |
+                                ((px < X2) & (x < px + wm) & (py < Y2) & (y < py + dm) &
+                                 (pz < Z2) & (z < pz + hm)));
         int shift = 0;
 #pragma unroll
         for (int j = 0; j < 8; j++) shift += (((add >> j) & 1u) && ck[j] < v) ? 1 : 0;
-        const bool keep = ok & !half;
+        const bool keep = ok & !gone;
         const uint64_t km = ballot(keep);
         if (keep) {
             const int pos = base + rank_below(km) + shift;
@@ -328,17 +336,17 @@ __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t* oidx, 
             oidx[pos] = (uint16_t)i;
         }
         base += __popcll(km);
-        const uint64_t hm = ballot(half);
-        if (hm) {
+        const uint64_t gm = ballot(gone);
+        if (gm) {
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                const int c = __popcll(hm & ballot(v < ck[j]));
+                const int c = __popcll(gm & ballot(v < ck[j]));
                 if (lane == j) rb += c;
             }
             if (count_work) {
                 const bool strict = (x < px) & (px < X2) & (y < py) & (py < Y2) & (z < pz) &
                                     (pz < Z2);
-                const bool mv = half & !strict;
+                const bool mv = gone & !strict;
                 const uint64_t mm = ballot(mv);
                 const int pos = nd + rank_below(mm);
                 if (mv && pos < cap + 8) dead[pos] = v;
@@ -562,7 +570,14 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                 // Same type, no placement since its last copy found nothing:
                 // the verdicts repeat, nothing is drawn.
                 if (ti == fail_ti) continue;
-                if (wid == 0) live += (uint64_t)n_act * (uint64_t)no * (uint64_t)n_box;
+                int wm = 0, dm = 0, hm = 0;
+                if (wid == 0) {
+                    live += (uint64_t)n_act * (uint64_t)no * (uint64_t)n_box;
+                    const int32_t* mn = a.step_min + 4 * (size_t)(I.step_off + s);
+                    wm = mn[0];
+                    dm = mn[1];
+                    hm = mn[2];
+                }
                 if (ti != cur_ti) {
                     const int32_t* orow = a.ors_tab + (size_t)(I.ors_off + ti) * kOrsRow;
                     if (lane < 3 * no) s_orw[lane] = orow[1 + lane];
@@ -707,7 +722,8 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                 if (wid == 0) {
                     const int need = merge_points(act, cur ? ep_a : ep_b, oidx, dead, n_act, n_dead,
                                                   box, n_box, x, y, z, X2, Y2, Z2, L, W, H, xb, yb,
-                                                  a.ep_cap, a.count_work != 0);
+                                                  a.ep_cap, a.count_work != 0, uni(wm), uni(dm),
+                                                  uni(hm));
                     max_ep = max(max_ep, (uint32_t)(n_act + n_dead));
                     ep_need = max(ep_need, (uint32_t)need);
                     if (lane == 0) {
