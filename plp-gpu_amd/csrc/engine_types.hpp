@@ -71,13 +71,14 @@ struct DecodeArgs {
 // live extreme-point lists used alternately, the previous-list index of each
 // entry of the current one, two pair-state caches used alternately, the
 // dead-point list when work is counted, pair masks, per-wave culled box
-// copies) is reused by FSR after the pass.  Each list has ep_cap + 8 slots: a
-// merge writes at most 8 entries past the capacity before the overflow is
-// detected.  The box array has box_cap slots plus one sentinel; the step
-// table (box_cap >= steps) follows the MT state.
+// copies) is reused by FSR after the pass (loads, sort keys, contact
+// totals).  Each list has ep_cap + 8 slots: a merge writes at most 8 entries
+// past the capacity before the overflow is detected.  The box array has
+// box_cap slots plus one sentinel; the step table (box_cap >= steps) follows
+// the MT state.
 struct LdsLayout {
     int32_t off_ep, off_ep2, off_oidx, off_cache, n_cache, off_dead, off_mask, n_mask, off_rel;
-    int32_t off_load, off_key, n_key;       // FSR view of the first region
+    int32_t off_load, off_key, n_key, off_tot;   // FSR view of the first region
     int32_t off_box, off_mt, off_steps, total;
 };
 
@@ -109,7 +110,8 @@ __host__ __device__ inline LdsLayout lds_layout(int32_t ep_cap, int32_t box_cap,
     l.n_key = next_pow2(box_cap > 0 ? box_cap : 1);
     l.off_load = 0;
     l.off_key = box_cap * 8;
-    int32_t fsr_end = l.off_key + l.n_key * 8;
+    l.off_tot = l.off_key + l.n_key * 8;
+    int32_t fsr_end = l.off_tot + box_cap * 4;
     int32_t region = pass_end > fsr_end ? pass_end : fsr_end;
     l.off_box = (region + 15) & ~15;
     l.off_mt = l.off_box + (box_cap + 8) * 32;

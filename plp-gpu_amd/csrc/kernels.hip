@@ -148,17 +148,31 @@ __device__ __forceinline__ void cache_put(void* c, int i, uint32_t v, bool wide)
 // reference's order (bit-exact FP64).  Returns the score; per-box FSR goes to
 // fsr_out when non-null.
 __device__ double fsr_score(const Box* __restrict__ box, int n, double mu, double* load,
-                            uint64_t* skey, int n_key, double* fsr_out) {
+                            uint64_t* skey, uint32_t* tot, double* fsr_out) {
     const int lane = lane_id();
     if (n <= 0) return 0.0;
     const int n2 = next_pow2(n);
     for (int i = lane; i < n2; i += kWave)
         skey[i] = i < n ? (((uint64_t)(0x7fffffffu - (uint32_t)box[i].z) << 32) | (uint32_t)i)
                         : ~0ull;
-    for (int i = lane; i < n; i += kWave) {
-        const int64_t v = (int64_t)(box[i].x2 - box[i].x) * (box[i].y2 - box[i].y) *
-                          (box[i].z2 - box[i].z);
-        load[i] = 50.0 * (double)v / 1.0e6;   // m = RHO * V / 1e6
+    // load = m = RHO * V / 1e6; tot[k] = contact area of box k with the boxes
+    // whose top is at its bottom (lane per k, every j in turn).  Those boxes
+    // do not overlap one another, so tot[k] <= w * d < 2^30.
+    for (int k = lane; k < n; k += kWave) {
+        const Box Bk = box[k];
+        const int64_t v = (int64_t)(Bk.x2 - Bk.x) * (Bk.y2 - Bk.y) * (Bk.z2 - Bk.z);
+        load[k] = 50.0 * (double)v / 1.0e6;
+        uint32_t t = 0;
+        if (Bk.z > 0) {
+            for (int j = 0; j < n; j++) {
+                const Box Bj = box[j];
+                const int ox = min(Bk.x2, Bj.x2) - max(Bk.x, Bj.x);
+                const int oy = min(Bk.y2, Bj.y2) - max(Bk.y, Bj.y);
+                if (j != k && Bj.z2 == Bk.z && ox > 0 && oy > 0)
+                    t += (uint32_t)ox * (uint32_t)oy;
+            }
+        }
+        tot[k] = t;
     }
     wave_sync();
     // Bitonic sort, ascending.
@@ -178,23 +192,13 @@ __device__ double fsr_score(const Box* __restrict__ box, int n, double mu, doubl
             wave_sync();
         }
     }
-    (void)n_key;
     for (int t = 0; t < n; t++) {
         const int k = (int)(uint32_t)uni(skey[t]);
         const Box Bk = box[k];
         const int bkz = uni(Bk.z);
         if (bkz == 0) break;   // the rest are floor boxes: nothing supports them
         const int bkx = uni(Bk.x), bky = uni(Bk.y), bkx2 = uni(Bk.x2), bky2 = uni(Bk.y2);
-        uint64_t part = 0;
-        for (int j = lane; j < n; j += kWave) {
-            const Box Bj = box[j];
-            if (j != k && Bj.z2 == bkz) {
-                const int ox = min(bkx2, Bj.x2) - max(bkx, Bj.x);
-                const int oy = min(bky2, Bj.y2) - max(bky, Bj.y);
-                if (ox > 0 && oy > 0) part += (uint64_t)ox * (uint64_t)oy;
-            }
-        }
-        const uint64_t total = wave_sum_u64(part);
+        const uint32_t total = uni(tot[k]);
         if (total > 0) {
             const double lk = load[k];
             for (int j = lane; j < n; j += kWave) {
@@ -261,18 +265,22 @@ __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t* oidx, 
         if (lane == j) myk = ck[j];
         if ((jx + wm > L) | (jy + dm > W) | (jz + hm > H)) cdead |= 1u << j;
     }
-    // Corners whose smallest box meets a placed box (the new one included)
-    // are not useful.  Lanes 0..7 meanwhile find each corner's lower bound
-    // in src.
+    // Each corner's lower bound in src (lane j < 8 keeps corner j's in lo)
+    // and whether src holds it.
     int lo = 0;
-    if (lane < 8) {
-        int hi = n_act;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (src[mid] < myk) lo = mid + 1;
-            else hi = mid;
+    uint32_t exists = 0;
+    for (int i0 = 0; i0 < n_act; i0 += kWave) {
+        const int i = i0 + lane;
+        const uint32_t v = i < n_act ? src[i] : 0xffffffffu;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int c = __popcll(ballot(v < ck[j]));
+            if (lane == j) lo += c;
+            if (ballot(v == ck[j])) exists |= 1u << j;
         }
     }
+    // Corners whose smallest box meets a placed box (the new one included)
+    // are not useful.
     for (int b0 = 0; b0 < n_box; b0 += kWave) {
         const int bi = b0 + lane;
         Box B = sentinel_box();
@@ -285,7 +293,6 @@ __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t* oidx, 
             if (ballot(in)) cdead |= 1u << j;
         }
     }
-    const uint32_t exists = (uint32_t)ballot(lane < 8 && lo < n_act && src[lo] == myk) & 0xffu;
     const uint32_t add = ~cdead & ~exists & 0xffu;   // useful corners that join the list
 
     // Dead list: drop the strictly inside points in place (a point moves to
@@ -479,6 +486,7 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
     uint64_t* mask = (uint64_t*)(lds + lay.off_mask);
     double* load = (double*)(lds + lay.off_load);
     uint64_t* skey = (uint64_t*)(lds + lay.off_key);
+    uint32_t* stot = (uint32_t*)(lds + lay.off_tot);
     Box* box = (Box*)(lds + lay.off_box);
     uint32_t* mt = (uint32_t*)(lds + lay.off_mt);
     int32_t* s_step = (int32_t*)(lds + lay.off_steps);
@@ -758,7 +766,7 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                         vpart += (int64_t)(box[i].x2 - box[i].x) * (box[i].y2 - box[i].y) *
                                  (box[i].z2 - box[i].z);
                     const int64_t vol = wave_sum_i64(vpart);
-                    const double score = fsr_score(box, n, a.mu, load, skey, lay.n_key, nullptr);
+                    const double score = fsr_score(box, n, a.mu, load, skey, stot, nullptr);
                     // D4' (plp_common.hpp:493-511), kept incrementally: the best
                     // of all packings and the best accepted one, strict > on
                     // (volume, n_boxes) so the earliest winner stays.
@@ -855,6 +863,7 @@ __global__ void __launch_bounds__(64) fsr_kernel(FsrArgs f) {
     const LdsLayout lay = lds_layout(0, f.box_cap, 0);
     double* load = (double*)(lds + lay.off_load);
     uint64_t* skey = (uint64_t*)(lds + lay.off_key);
+    uint32_t* stot = (uint32_t*)(lds + lay.off_tot);
     Box* box = (Box*)(lds + lay.off_box);
     for (int i = lane_id(); i < n; i += kWave) {
         const int32_t* q = f.rows7 + (size_t)(r0 + i) * 7;
@@ -864,7 +873,7 @@ __global__ void __launch_bounds__(64) fsr_kernel(FsrArgs f) {
         box[i] = B;
     }
     block_sync();
-    const double sc = fsr_score(box, n, f.mu, load, skey, lay.n_key,
+    const double sc = fsr_score(box, n, f.mu, load, skey, stot,
                                 f.fsr_out ? f.fsr_out + r0 : nullptr);
     if (lane_id() == 0) f.score_out[pk] = sc;
 }
