@@ -310,6 +310,19 @@ def main() -> int:
         extras["large_100types_2048_packings_per_s"] = round(3 * 2048 / (time.perf_counter() - t0), 1)
         extras["large_100types_2048_kernel_ms"] = round(e3.last_ms(), 3)
         e3.close()
+        # JSON wire format: one 4096-decoder request through `plp_gpu --serve`;
+        # format_ms is the host time to write its 4096 reference-format rows.
+        cli = os.path.join(REPO, "plp-gpu_amd", "build", "plp_gpu")
+        req = json.dumps({"id": 1, "instance_text": text, "decoders": 4096, "seed": 1,
+                          "passes": 1, "eps": EPS, "theta": THETA})
+        try:
+            resp = json.loads(subprocess.run([cli, "--serve"], input=req + "\n", check=True,
+                                             capture_output=True, text=True,
+                                             timeout=300).stdout)
+            extras["json_rows_per_s"] = round(4096 / (resp["format_ms"] * 1e-3), 1)
+            extras["json_4096_rows_bytes"] = sum(len(json.dumps(r)) for r in resp["rows"])
+        except Exception as e:  # noqa: BLE001 -- reported, not fatal
+            log(f"serve wire-format probe failed: {e}")
         if a.serve_seconds > 0:
             extras["serving"] = serve(plp_amd, synth, a.serve_seconds, a.serve_rate, 1,
                                       a.serve_inflight)

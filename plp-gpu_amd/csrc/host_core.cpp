@@ -230,26 +230,42 @@ std::string py_repr(double x) {
     return out;
 }
 
+namespace {
+
+template <class T>
+void append_int(std::string& o, T v) {
+    char buf[24];
+    const auto res = std::to_chars(buf, buf + sizeof buf, v);
+    o.append(buf, res.ptr);
+}
+
+}  // namespace
+
 std::string to_json(const RunResult& r) {
     std::string o;
-    o.reserve(64 + r.placements.size() * 40);
-    o += "{\"n_boxes\": " + std::to_string(r.n_boxes);
-    o += ", \"volume_cm3\": " + std::to_string(r.volume);
-    o += ", \"stability_score\": " + py_repr(r.score);
-    o += std::string(", \"fallback\": ") + (r.fallback ? "true" : "false");
-    o += ", \"n_passes\": " + std::to_string(r.n_passes);
-    o += ", \"time_s\": " + py_repr(r.time_s);
-    o += ", \"seed\": " + std::to_string(r.seed);
+    o.reserve(160 + r.placements.size() * 36);
+    o += "{\"n_boxes\": ";
+    append_int(o, r.n_boxes);
+    o += ", \"volume_cm3\": ";
+    append_int(o, r.volume);
+    o += ", \"stability_score\": ";
+    o += py_repr(r.score);
+    o += r.fallback ? ", \"fallback\": true" : ", \"fallback\": false";
+    o += ", \"n_passes\": ";
+    append_int(o, r.n_passes);
+    o += ", \"time_s\": ";
+    o += py_repr(r.time_s);
+    o += ", \"seed\": ";
+    append_int(o, r.seed);
     o += ", \"placements\": [";
     for (size_t i = 0; i < r.placements.size(); i++) {
         const auto& p = r.placements[i];
-        if (i) o += ", ";
-        o += "[";
+        o += i ? ", [" : "[";
         for (int c = 0; c < 7; c++) {
             if (c) o += ", ";
-            o += std::to_string(p[c]);
+            append_int(o, p[(size_t)c]);
         }
-        o += "]";
+        o += ']';
     }
     o += "]}";
     return o;

@@ -6,9 +6,11 @@
 //           [--order volume|input] [--first-fit] [--single-pass] [--mu F]
 //           [--batch K | --device K] [--json]
 //   plp_gpu --selftest | --rngdump N
-// plus two additions that leave every default unchanged:
+// plus additions that leave every default unchanged:
 //   --passes P   exactly P passes (the reference can only stop on --time)
 //   --plain      the one-step-per-launch form for the default single run
+//   --serve [--max-inflight N]   JSON-lines request/response server on
+//                stdin/stdout (serve_cli.cpp)
 // Default single runs use the device-resident decoder; --batch K drives K
 // lockstep rows from the host with one batched feasibility launch per step;
 // --device K runs K device-resident decoders (no K <= 16 limit).
@@ -24,6 +26,7 @@
 #include "host_core.hpp"
 #include "plain.hpp"
 #include "plp_hip.h"
+#include "serve_cli.hpp"
 
 namespace {
 
@@ -107,7 +110,7 @@ void usage(const char* prog) {
                  "                 [--order volume|input] [--first-fit] [--single-pass] "
                  "[--passes P]\n"
                  "                 [--batch K | --device K] [--plain] [--mu F] [--json]\n"
-                 "       %s --selftest | --rngdump N\n",
+                 "       %s --selftest | --rngdump N | --serve [--max-inflight N] [--mu F]\n",
                  prog, prog);
 }
 
@@ -135,7 +138,8 @@ int main(int argc, char** argv) {
     long long seed = 0;
     double time_s = 300.0, eps = 0.3, theta = 0.9, mu = 0.2;
     bool no_theta = false, first_fit = false, single_pass = false, as_json = false, plain = false;
-    int batch = 0, device = 0, passes = 0;
+    int batch = 0, device = 0, passes = 0, max_inflight = 8;
+    bool serve = false;
     if (const char* e = std::getenv("PLP_MU")) mu = std::strtod(e, nullptr);
     for (int i = 1; i < argc; i++) {
         std::string a = argv[i];
@@ -162,6 +166,8 @@ int main(int argc, char** argv) {
         else if (a == "--device") device = std::atoi(next(a.c_str()).c_str());
         else if (a == "--plain") plain = true;
         else if (a == "--json") as_json = true;
+        else if (a == "--serve") serve = true;
+        else if (a == "--max-inflight") max_inflight = std::atoi(next(a.c_str()).c_str());
         else if (a == "-h" || a == "--help") { usage(argv[0]); return 0; }
         else if (!a.empty() && a[0] != '-') {
             if (!instance_path.empty()) { usage(argv[0]); return 2; }
@@ -171,6 +177,13 @@ int main(int argc, char** argv) {
             usage(argv[0]);
             return 2;
         }
+    }
+    if (serve) {
+        if (!instance_path.empty() || max_inflight < 1 || max_inflight > 64) {
+            usage(argv[0]);
+            return 2;
+        }
+        return serve_main(max_inflight, mu);
     }
     if (instance_path.empty()) { usage(argv[0]); return 2; }
     plp::Instance inst;

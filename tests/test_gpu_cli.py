@@ -91,3 +91,38 @@ def test_cli_range_error_exit_code(tmp_path):
     p = subprocess.run([CLI, str(big), "--single-pass", "--json"], capture_output=True, text=True,
                        timeout=120)
     assert p.returncode == 2
+
+
+def test_cli_serve_json_lines_match_device_rows():
+    """--serve: JSON-lines requests (file path, inline text, a bad one) get
+    responses whose rows equal the --device rows / the oracle's."""
+    syn1 = load_text(NAMES["SYN1"])
+    reqs = [
+        {"id": 1, "instance": golden_path(NAMES["BR2"]), "decoders": 6, "seed": 5, "passes": 2,
+         "eps": 0.3, "theta": 0.9},
+        {"id": "b", "instance_text": syn1, "decoders": 3, "seed": 11, "passes": 1, "eps": 0.0,
+         "theta": None},
+        {"id": 3, "decoders": 2},
+        {"id": 4, "instance": golden_path(NAMES["BR1"]), "decoders": 2, "order": "sideways"},
+    ]
+    text = "".join(json.dumps(r) + "\n" for r in reqs) + "this is not json\n"
+    p = subprocess.run([CLI, "--serve", "--max-inflight", "2"], input=text, capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    out = [json.loads(l) for l in p.stdout.strip().splitlines()]
+    by_id = {o["id"]: o for o in out}
+    assert len(out) == 5 and set(by_id) == {1, "b", 3, 4, None}
+    assert "error" in by_id[3] and "error" in by_id[4] and "error" in by_id[None]
+    a = by_id[1]
+    assert a["K"] == 6 and a["n_passes"] == 2 and a["format_ms"] >= 0 and a["time_s"] > 0
+    want = json.loads(run([golden_path(NAMES["BR2"]), "--device", "6", "--seed", "5", "--passes",
+                           "2", "--eps", "0.3", "--json"]))
+    assert [strip(r) for r in a["rows"]] == [strip(r) for r in want["rows"]]
+    orc = Oracle()
+    inst = orc.parse(syn1)[0]
+    for k, row in enumerate(by_id["b"]["rows"]):
+        w = orc.run(inst, seed=11 + k, passes=1, eps=0.0, theta=None)
+        assert {x: row[x] for x in ("n_boxes", "volume_cm3", "stability_score", "fallback",
+                                    "placements")} == \
+            {x: w[x] for x in ("n_boxes", "volume_cm3", "stability_score", "fallback",
+                               "placements")}, k
