@@ -100,6 +100,35 @@ int orc_run(const orc_instance *inst, uint64_t seed, int n_passes, double eps,
             int use_theta, double theta, int order, int first_fit, double mu,
             orc_result *out);
 
+/* ---- block-building engine (blockproof/plp_block.cpp), plp_block_oracle.c */
+
+/* One block shape: cnt = a*b*k copies of item ti in orientation oi as a
+ * bw x bd x bh stack (BlockShape, plp_block.cpp:27-33). */
+typedef struct {
+    int ti, oi, a, b, k;
+    long long cnt;
+    int bw, bd, bh;
+} orc_shape;
+
+/* Catalogue of run_block (plp_block.cpp:193-218): shapes in generation order;
+ * by_type[type_off[t] .. type_off[t+1]) lists type t's shapes in
+ * shape_precedence order (plp_block.cpp:38-45).  Returns the shape count or
+ * -1 if cap is too small. */
+int orc_block_catalogue(const orc_instance *inst, int amax, int bmax, int kmax, orc_shape *cat,
+                        int cap, int *by_type, int *type_off);
+
+/* block_pass (plp_block.cpp:95-204); pick_rule 0 first, 1 support, 2 contact.
+ * items7 gets the decomposed items (x, y, z, w, d, h, type); returns their
+ * count or -1. */
+int orc_block_pass(const orc_instance *inst, const orc_shape *cat, const int *by_type,
+                   const int *type_off, double eps, int pick_rule, int topk, orc_rng *rng,
+                   int *items7, int cap, long long *volume);
+
+/* Fixed-pass run_block with online D4' (plp_block.cpp:220-278). */
+int orc_block_run(const orc_instance *inst, uint64_t seed, int n_passes, double eps,
+                  int use_theta, double theta, int amax, int bmax, int kmax, int topk,
+                  int pick_rule, double mu, orc_result *out);
+
 /* CPython float repr, plp_common.hpp:535-590.  Returns the length. */
 int orc_py_repr(double x, char *buf, int len);
 

@@ -212,17 +212,91 @@ def write_repr_vectors() -> None:
         f.write("\n")
 
 
+BLOCK_DRIVER = os.path.join(REPO, "oracle", "_ref", "block_driver")
+BLOCK_CLI = os.path.join(REPO, "oracle", "_ref", "plp_block_ref")
+
+
+def block_args(seed, passes, eps, theta, pick, amax, bmax, kmax, topk):
+    a = ["--seed", str(seed), "--eps", repr(eps), "--pick", pick, "--amax", str(amax),
+         "--bmax", str(bmax), "--kmax", str(kmax), "--topk", str(topk)]
+    return a + (["--no-theta"] if theta is None else ["--theta", repr(theta)])
+
+
+def write_block_runs(synth_files: dict) -> None:
+    """Fixed-pass runs of the reference block engine (blockproof/plp_block.cpp)
+    through oracle/block_driver, plus single-pass runs of the reference's own
+    CLI that pin the driver."""
+    cases = []
+
+    def add(name, inst_rel, seed=0, passes=1, eps=0.3, theta=0.9, pick="first", amax=3, bmax=3,
+            kmax=4, topk=8):
+        args = block_args(seed, passes, eps, theta, pick, amax, bmax, kmax, topk)
+        out = subprocess.run([BLOCK_DRIVER, "run", os.path.join(HERE, inst_rel), "--passes",
+                              str(passes)] + args, check=True, capture_output=True, text=True)
+        res = json.loads(out.stdout)
+        cli = None
+        if passes == 1 and seed < 2**31:
+            c = subprocess.run([BLOCK_CLI, os.path.join(HERE, inst_rel), "--single-pass", "--json"]
+                               + args, check=True, capture_output=True, text=True)
+            cli = json.loads(c.stdout)
+        cases.append({"case": name, "instance": inst_rel,
+                      "args": {"seed": seed, "passes": passes, "eps": eps, "theta": theta,
+                               "pick": pick, "amax": amax, "bmax": bmax, "kmax": kmax,
+                               "topk": topk},
+                      "result": res, "reference_cli": cli})
+        print(name, res["n_boxes"], res["volume_cm3"], res["stability_score"], flush=True)
+
+    br = {f"BR{i}": f"data/instances/BR{i}_050.txt" for i in range(19)}
+    br["SYN1"] = "data/synthetic/SYN1.txt"
+    for name, rel in br.items():
+        add(f"blk_{name}_s0", rel)
+    for name in ("BR1", "BR2", "BR3"):
+        for pick in ("first", "support", "contact"):
+            for seed in (1, 2):
+                add(f"blk_{name}_{pick}_s{seed}_p3", br[name], seed=seed, passes=3, pick=pick)
+        add(f"blk_{name}_eps0", br[name], eps=0.0)
+        add(f"blk_{name}_eps0_support", br[name], eps=0.0, pick="support")
+        add(f"blk_{name}_notheta_p2", br[name], seed=4, passes=2, theta=None)
+    add("blk_BR4_single_items", br["BR4"], seed=3, amax=1, bmax=1, kmax=1)
+    add("blk_BR5_222_p2", br["BR5"], seed=5, passes=2, amax=2, bmax=2, kmax=2)
+    add("blk_BR6_446", br["BR6"], seed=6, amax=4, bmax=4, kmax=6, pick="contact")
+    add("blk_BR7_topk1", br["BR7"], seed=7, topk=1)
+    add("blk_BR8_topk3_eps1", br["BR8"], seed=8, eps=1.0, topk=3)
+    add("blk_BR9_theta05_p3", br["BR9"], seed=9, passes=3, theta=0.5)
+    add("blk_SYN1_bigseed", br["SYN1"], seed=2**32 + 99, passes=2)
+    for name in ("tower10", "side_by_side", "stack3", "skip", "two_types", "no_orient",
+                 "empty_qty", "too_big"):
+        add(f"blk_{name}", synth_files[name])
+    for seed in range(1, 5):
+        add(f"blk_multi_s{seed}", synth_files["multi"], seed=seed, passes=2, eps=0.01)
+    for i in range(6):
+        add(f"blk_mixed_{i:03d}", synth_files[f"mixed_{i:03d}"], seed=i + 1, passes=2, eps=0.01,
+            pick=("first", "support", "contact")[i % 3])
+    add("blk_large_s1", synth_files["large"], seed=1, eps=0.01)
+    with open(os.path.join(HERE, "block_runs.json"), "w") as f:
+        json.dump(cases, f, separators=(",", ":"))
+        f.write("\n")
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
+    ap.add_argument("--block-only", action="store_true",
+                    help="only rewrite block_runs.json (the other fixtures stay as they are)")
     a = ap.parse_args()
     if not (os.path.exists(REF_DRIVER) and os.path.exists(REF_CLI)):
         sys.exit("build the reference harness first: make -C oracle ref")
+    if a.block_only:
+        synth_files = {os.path.splitext(n)[0]: os.path.join("synth", n)
+                       for n in os.listdir(os.path.join(HERE, "synth"))}
+        write_block_runs(synth_files)
+        return
     copy_reference_data(a.ref)
     synth_files = write_synthetic()
     write_rng_vectors()
     write_repr_vectors()
     write_ref_runs(synth_files)
+    write_block_runs(synth_files)
 
 
 if __name__ == "__main__":

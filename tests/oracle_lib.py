@@ -83,6 +83,10 @@ class Oracle:
                               C.c_int, C.c_int, C.c_double, C.POINTER(Result)]
         L.orc_py_repr.restype = C.c_int
         L.orc_py_repr.argtypes = [C.c_double, C.c_char_p, C.c_int]
+        L.orc_block_run.restype = C.c_int
+        L.orc_block_run.argtypes = [C.POINTER(Inst), C.c_uint64, C.c_int, C.c_double, C.c_int,
+                                    C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_double, C.POINTER(Result)]
 
     # ---- parser -------------------------------------------------------
     def parse(self, text: str):
@@ -176,6 +180,31 @@ class Oracle:
             "placements": [list(pl[7 * i:7 * i + 7]) for i in range(res.n_boxes)],
             "pair_box": res.work.pair_box,
             "max_cand": res.work.max_cand,
+        }
+
+    PICK = {"first": 0, "support": 1, "contact": 2}
+
+    def block_run(self, inst_tuple, seed=0, passes=1, eps=0.3, theta=0.9, amax=3, bmax=3, kmax=4,
+                  topk=8, pick="first", mu=0.2):
+        """Fixed-pass block-building engine (blockproof/plp_block.cpp)."""
+        L, W, H, types = inst_tuple
+        inst = self.make_inst(L, W, H, types)
+        cap = max(1, sum(t[6] for t in types))
+        pl = (C.c_int * (7 * cap))()
+        res = Result()
+        res.placements = C.cast(pl, C.POINTER(C.c_int))
+        rc = self.L.orc_block_run(C.byref(inst), seed, passes, eps, 0 if theta is None else 1,
+                                  0.0 if theta is None else theta, amax, bmax, kmax, topk,
+                                  self.PICK[pick], mu, C.byref(res))
+        assert rc == 0
+        return {
+            "n_boxes": res.n_boxes,
+            "volume_cm3": res.volume,
+            "stability_score": res.score,
+            "fallback": bool(res.fallback),
+            "n_passes": res.n_passes,
+            "seed": res.seed,
+            "placements": [list(pl[7 * i:7 * i + 7]) for i in range(res.n_boxes)],
         }
 
     def decode_pass(self, inst_tuple, order="volume", first_fit=False, eps=0.0, rng=None):
