@@ -176,6 +176,27 @@ int plp_engine_load(plp_engine *eng, const plp_instance_desc *insts, int32_t n_i
                     const int32_t *dec_inst, const uint64_t *dec_seed, int32_t n_dec,
                     const plp_params *params);
 
+/* ----------------------------- block-building engine (blockproof/) */
+/* The reference's second search unit (blockproof/plp_block.cpp:95-278): a x b
+ * bases of one item type and orientation stacked k high, planned per sweep
+ * and placed at extreme points, then decomposed into item boxes for FSR. */
+typedef struct {
+    int32_t amax, bmax, kmax;   /* block bounds (--amax 3 --bmax 3 --kmax 4, plp_block.cpp:297) */
+    int32_t topk;               /* eps-greedy shape choice among the topk largest (--topk 8) */
+    int32_t pick;               /* 0 first, 1 support, 2 contact (--pick, plp_block.cpp:56) */
+    int32_t reserved[3];
+} plp_block_params;
+
+/* Replaces run_block (plp_block.cpp:220-278) for many (instance, seed)
+ * decoders: like plp_engine_load, but the engine then runs the block decoder
+ * (fixed pass count, D4' per plp_block.cpp:228-275); run, sync, fetch, reset
+ * and autotune work unchanged, and rows hold the decomposed item boxes.
+ * params->eps, theta, use_theta and mu apply; order and first_fit do not
+ * exist in the block engine and are ignored.  count_work is not supported. */
+int plp_engine_load_blocks(plp_engine *eng, const plp_instance_desc *insts, int32_t n_inst,
+                           const int32_t *dec_inst, const uint64_t *dec_seed, int32_t n_dec,
+                           const plp_params *params, const plp_block_params *blocks);
+
 /* Return every decoder to its freshly loaded state (RNG reseeded on the next
  * run, no packing kept) without re-uploading the tables. */
 int plp_engine_reset(plp_engine *eng);

@@ -1,0 +1,513 @@
+// block_kernel.hip -- gfx950 block-building decoder (the reference's second
+// search unit, blockproof/plp_block.cpp:95-278).
+//
+// A pass repeats sweeps of plan -> placement until nothing is left or a sweep
+// places nothing.  Plan (plp_block.cpp:129-153): the types with copies left
+// in item-volume order (eps: Fisher-Yates shuffle), each decomposed into
+// blocks (a x b base of one orientation, k high) by taking, while copies
+// remain, the first shape in shape_precedence order whose copy count fits
+// (eps: uniformly among the topk largest that fit).  The plan is sorted into
+// placement order (plp_block.cpp:156-165) by a bitonic network in LDS over
+// the shapes' placement ranks.  Placement (plp_block.cpp:168-183): every
+// planned block still affordable is tested at every extreme point, in (z, y,
+// x) order, with the decoder's feasibility predicate; the pick is the first
+// feasible point, a random one (eps), or the one with the most bottom support
+// (ties: side contact) / the most total contact (ties: bottom), earliest on
+// full ties.  After the pass the blocks are decomposed into item boxes
+// (plp_block.cpp:188-201), which are scored with FSR and kept by the D4' rule
+// (plp_block.cpp:228-275).
+//
+// One block of NW waves per (instance, seed), persistent over a work queue
+// like decode_kernel.  Every wave tests 64-point chunks c = wave, wave + NW,
+// ...; wave 0 runs the plan, the picks, the extreme-point merge and the pass
+// end.  The extreme-point list keeps only points where a box of the
+// remaining types' smallest dimensions fits (every later block contains one
+// such box), so the feasible points, and so every pick, are the reference's.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "decoder_dev.hpp"
+#include "engine_types.hpp"
+#include "kernels.hpp"
+#include "mt19937.hpp"
+#include "wave.hpp"
+
+namespace plpk {
+
+namespace {
+
+struct BlockShared {
+    int item;       // queue item of the block
+    int quit;       // the decoder stopped (capacity error)
+    int n_act;      // live extreme points
+    int n_box;      // placed blocks
+    int n_plan;     // blocks planned by the current sweep
+    int cnt[2][8];  // feasible points found by each wave, by evaluated-test parity
+};
+
+// Componentwise smallest orientation over the types with copies left.
+__device__ void remaining_min(const int* rem, const BlockTypeRec* tr, int nt, int& wm, int& dm,
+                              int& hm) {
+    const int lane = lane_id();
+    int w = 0x3fffffff, d = 0x3fffffff, h = 0x3fffffff;
+    for (int t = lane; t < nt; t += kWave)
+        if (rem[t] > 0 && tr[t].has_ors) {
+            w = min(w, tr[t].wmin);
+            d = min(d, tr[t].dmin);
+            h = min(h, tr[t].hmin);
+        }
+    wm = wave_min(w);
+    dm = wave_min(d);
+    hm = wave_min(h);
+}
+
+// contact_stats (plp_block.cpp:64-81) of a bw x bd x bh block at (x, y, z):
+// bottom support (the floor counts in full) and flush side contact.
+__device__ void contact_stats(int x, int y, int z, int bw, int bd, int bh, const Box* box,
+                              int n_box, int64_t& bottom, int64_t& side) {
+    int64_t bt = z == 0 ? (int64_t)bw * bd : 0, sd = 0;
+    for (int i = 0; i < n_box; i++) {
+        const Box B = box[i];
+        const int64_t ox = (int64_t)min(x + bw, B.x2) - max(x, B.x);
+        const int64_t oy = (int64_t)min(y + bd, B.y2) - max(y, B.y);
+        const int64_t oz = (int64_t)min(z + bh, B.z2) - max(z, B.z);
+        if (B.z2 == z && ox > 0 && oy > 0) bt += ox * oy;
+        if (oy > 0 && oz > 0 && (B.x2 == x || B.x == x + bw)) sd += oy * oz;
+        if (ox > 0 && oz > 0 && (B.y2 == y || B.y == y + bd)) sd += ox * oz;
+    }
+    bottom = bt;
+    side = sd;
+}
+
+// Ascending bitonic sort of n2 (a power of two) keys in LDS, one wave.
+__device__ void bitonic_u32(uint32_t* key, int n2) {
+    const int lane = lane_id();
+    for (int k = 2; k <= n2; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = lane; i < n2; i += kWave) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const uint32_t p = key[i], q = key[ixj];
+                    if ((p > q) == ((i & k) == 0)) {
+                        key[i] = q;
+                        key[ixj] = p;
+                    }
+                }
+            }
+            wave_sync();
+        }
+}
+
+}  // namespace
+
+template <int NW>
+__global__ void __launch_bounds__(64 * NW) block_kernel(BlockArgs ba) {
+    const DecodeArgs& a = ba.d;
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ BlockShared sh;
+    __shared__ DecState sst;
+    const BlockLds lay = block_lds_layout(a.ep_cap, a.box_cap, ba.max_types, a.use_rng, NW);
+    uint32_t* const ep_a = (uint32_t*)(lds + lay.off_ep);
+    uint32_t* const ep_b = (uint32_t*)(lds + lay.off_ep2);
+    uint16_t* const oidx = (uint16_t*)(lds + lay.off_oidx);
+    uint64_t* const mask = (uint64_t*)(lds + lay.off_mask);
+    int* const rem = (int*)(lds + lay.off_rem);
+    int* const tlist = (int*)(lds + lay.off_tlist);
+    uint32_t* const plan = (uint32_t*)(lds + lay.off_plan);
+    double* const load = (double*)(lds + lay.off_load);
+    uint64_t* const skey = (uint64_t*)(lds + lay.off_key);
+    uint32_t* const stot = (uint32_t*)(lds + lay.off_tot);
+    Box* const items = (Box*)(lds + lay.off_items);
+    Box* const box = (Box*)(lds + lay.off_box);
+    uint32_t* const mt = (uint32_t*)(lds + lay.off_mt);
+    const int lane = lane_id();
+    const int wid = (int)(threadIdx.x >> 6);
+    Box* const cbox = (Box*)(lds + lay.off_rel) + wid * kCullWindow;
+    const bool use_eps = a.eps > 0.0;
+
+    for (;;) {
+        if (threadIdx.x == 0) sh.item = (int)atomicAdd(a.work_counter, 1u);
+        block_sync();
+        const int item = uni(sh.item);
+        block_sync();
+        if (item >= a.n_work) break;
+        const int d = uni(a.work_order[item]);
+        const InstHdr I = a.inst[uni(a.dec_inst[d])];
+        const int L = I.L, W = I.W, H = I.H, xb = I.xbits, yb = I.ybits, nt = I.n_types;
+        const BlockShapeRec* const shapes = ba.shape + I.shape_off;
+        const BlockTypeRec* const tr = ba.tinfo + I.tinfo_off;
+        const int32_t* const vord = ba.vorder + I.vorder_off;
+
+        // ---- decoder state: in LDS (sst) between pass ends; wave 0
+        int mti = 624;
+        uint32_t* gmt = a.mt + (size_t)d * kMtWords;
+        if (wid == 0) {
+            if (lane == 0) {
+                DecState st0;
+                if (a.reset) {
+                    st0.passes = 0; st0.err = 0; st0.n_all = -1; st0.n_acc = -1;
+                    st0.v_all = 0; st0.v_acc = 0; st0.s_all = 0.0; st0.s_acc = 0.0;
+                    st0.work_pair_box = 0; st0.work_live = 0; st0.work_steps = 0; st0.max_ep = 0;
+                    st0.ep_need = 0; st0.pad = 0;
+                } else {
+                    st0 = a.state[d];
+                }
+                sst = st0;
+                sh.quit = st0.err != 0;
+            }
+            wave_sync();
+            if (uni(sh.quit) == 0 && a.use_rng) {
+                if (a.reset) {
+                    mt_seed_lane0(mt, a.dec_seed[d]);
+                } else {
+                    for (int i = lane; i < 624; i += kWave) mt[i] = gmt[i];
+                    mti = uni((int)gmt[624]);
+                }
+                wave_sync();
+            }
+        }
+        block_sync();
+        const bool skip_decoder = uni(sh.quit) != 0;
+        int32_t* out_all = a.best_all + (size_t)d * a.max_items * 7;
+        int32_t* out_acc = a.best_acc + (size_t)d * a.max_items * 7;
+
+        for (int pass = 0; pass < a.n_passes && !skip_decoder; pass++) {
+            // ---- pass start (wave 0): all copies left, origin only, no blocks
+            int n_act = 1, n_dead = 0, n_box = 0;
+            uint32_t ep_need = 1;
+            int wm = 0, dm = 0, hm = 0;
+            if (wid == 0) {
+                for (int t = lane; t < nt; t += kWave) rem[t] = tr[t].qty;
+                if (lane == 0) {
+                    ep_a[0] = 0u;   // origin; L, W, H > 0
+                    oidx[0] = kNewPoint;
+                }
+                for (int i = lane; i < a.box_cap + 8; i += kWave) box[i] = sentinel_box();
+                wave_sync();
+                remaining_min(rem, tr, nt, wm, dm, hm);
+            }
+            block_sync();
+            int na_w = 1, nb_w = 0, cur = 0, ev = 0;
+            bool quit = false;
+            for (;;) {   // sweeps
+                // ---- plan (wave 0)
+                if (wid == 0) {
+                    int n_types = 0;
+                    for (int q0 = 0; q0 < nt; q0 += kWave) {
+                        const int q = q0 + lane;
+                        const int t = q < nt ? vord[q] : 0;
+                        const bool in = q < nt && rem[t] > 0 && tr[t].has_ors;
+                        const uint64_t m = ballot(in);
+                        if (in) tlist[n_types + rank_below(m)] = t;
+                        n_types += __popcll(m);
+                    }
+                    n_types = uni(n_types);
+                    wave_sync();
+                    int n_plan = 0;
+                    if (n_types > 0) {
+                        if (use_eps && mt_random(mt, mti) >= 1.0 - a.eps) {
+                            for (int i = n_types - 1; i > 0; i--) {
+                                const int j = mt_randrange(mt, mti, i + 1);
+                                if (lane == 0) {
+                                    const int tmp = tlist[i];
+                                    tlist[i] = tlist[j];
+                                    tlist[j] = tmp;
+                                }
+                                wave_sync();
+                            }
+                        }
+                        for (int q = 0; q < n_types; q++) {
+                            const int t = uni(tlist[q]);
+                            int r = uni(rem[t]);
+                            const int off = tr[t].shp_off, size = tr[t].shp_cnt;
+                            while (r > 0) {
+                                // shapes are cnt-descending: those that fit
+                                // form a suffix starting after every cnt > r
+                                int start = 0;
+                                for (int j0 = 0; j0 < size; j0 += kWave) {
+                                    const int j = j0 + lane;
+                                    const bool big = j < size && shapes[ba.by_type[off + j]].cnt > r;
+                                    start += __popcll(ballot(big));
+                                }
+                                if (start == size) break;
+                                const int tk = min(ba.topk, size - start);
+                                int pk = 0;
+                                if (use_eps && mt_random(mt, mti) >= 1.0 - a.eps)
+                                    pk = mt_randrange(mt, mti, tk);
+                                const int s = uni(ba.by_type[off + start + pk]);
+                                if (lane == 0) plan[n_plan] = (uint32_t)s;
+                                n_plan++;
+                                r -= uni(shapes[s].cnt);
+                            }
+                        }
+                        const int n2 = next_pow2(max(n_plan, 1));
+                        for (int i = n_plan + lane; i < n2; i += kWave) plan[i] = 0xffffffffu;
+                        wave_sync();
+                        bitonic_u32(plan, n2);
+                    }
+                    if (lane == 0) sh.n_plan = n_plan;
+                }
+                block_sync();
+                const int n_plan = uni(sh.n_plan);
+                if (n_plan == 0) break;
+
+                // ---- placement
+                bool any = false;
+                int fail_s = -1;   // shape whose last test found nothing, no block placed since
+                for (int q = 0; q < n_plan; q++) {
+                    const int s = uni((int)plan[q]);
+                    const BlockShapeRec S = shapes[s];
+                    const int ti = uni(S.ti), cnt_s = uni(S.cnt);
+                    const int bw = uni(S.bw), bd = uni(S.bd), bh = uni(S.bh);
+                    if (uni(rem[ti]) < cnt_s) continue;   // planned before, stale
+                    if (s == fail_s) continue;            // same test, same (empty) answer
+                    const uint32_t* act = cur ? ep_b : ep_a;
+                    const int n_chunks = (na_w + kWave - 1) / kWave;
+                    int cnt_w = 0;
+                    for (int c = wid; c < n_chunks; c += NW) {
+                        const int e = c * kWave + lane;
+                        int x = 0, y = 0, z = 0;
+                        if (e < na_w) key_xyz(act[e], xb, yb, x, y, z);
+                        const bool c2 = e < na_w && x + bw <= L && y + bd <= W && z + bh <= H;
+                        uint64_t m = 0;
+                        if (ballot(c2) != 0) {
+                            const int xlo = wave_min(c2 ? x : 0x7fffffff);
+                            const int xhi = wave_max(c2 ? x + bw : -1);
+                            const int ylo = wave_min(c2 ? y : 0x7fffffff);
+                            const int yhi = wave_max(c2 ? y + bd : -1);
+                            const int zlo = wave_min(c2 ? z : 0x7fffffff);
+                            const int zhi = wave_max(c2 ? z + bh : -1);
+                            uint32_t sup = 0;
+                            const bool c12 = pair_scan(x, y, z, bw, bd, bh, c2, box, nb_w, xlo, xhi,
+                                                       ylo, yhi, zlo, zhi, cbox, sup);
+                            m = ballot(pair_verdict(c12, z, bw, bd, sup));
+                        }
+                        if (lane == 0) mask[c] = m;
+                        cnt_w += __popcll(m);
+                    }
+                    const int par = ev & 1;
+                    ev++;
+                    if (lane == 0) sh.cnt[par][wid] = cnt_w;
+                    block_sync();
+                    int cnt = 0;
+#pragma unroll
+                    for (int w2 = 0; w2 < NW; w2++) cnt += sh.cnt[par][w2];
+                    cnt = uni(cnt);
+                    if (cnt == 0) {
+                        fail_s = s;
+                        continue;
+                    }
+                    fail_s = -1;
+                    // ---- pick, place, merge (wave 0)
+                    if (wid == 0) {
+                        int pe = -1;   // picked extreme point
+                        if (use_eps && mt_random(mt, mti) >= 1.0 - a.eps) {
+                            int idx = mt_randrange(mt, mti, cnt);
+                            for (int c = 0;; c++) {
+                                const uint64_t m = uni(mask[c]);
+                                const int pc = __popcll(m);
+                                if (idx < pc) {
+                                    const bool hit = ((m >> lane) & 1ull) && rank_below(m) == idx;
+                                    pe = c * kWave + __ffsll((unsigned long long)ballot(hit)) - 1;
+                                    break;
+                                }
+                                idx -= pc;
+                            }
+                        } else if (ba.pick == 0) {
+                            for (int c = 0;; c++) {
+                                const uint64_t m = uni(mask[c]);
+                                if (m) {
+                                    pe = c * kWave + __ffsll((unsigned long long)m) - 1;
+                                    break;
+                                }
+                            }
+                        } else {
+                            // lane-parallel contact statistics; the wave keeps
+                            // the largest (primary, secondary), earliest point
+                            int64_t k0 = -1, k1 = -1;
+                            int bi = 0x7fffffff;
+                            for (int c = 0; c < n_chunks; c++) {
+                                const uint64_t m = uni(mask[c]);
+                                if (!m) continue;
+                                const int e = c * kWave + lane;
+                                if ((m >> lane) & 1ull) {
+                                    int x, y, z;
+                                    key_xyz(act[e], xb, yb, x, y, z);
+                                    int64_t sup, side;
+                                    contact_stats(x, y, z, bw, bd, bh, box, n_box, sup, side);
+                                    const int64_t p0 = ba.pick == 1 ? sup : sup + side;
+                                    const int64_t p1 = ba.pick == 1 ? side : sup;
+                                    if (p0 > k0 || (p0 == k0 && p1 > k1)) {
+                                        k0 = p0;
+                                        k1 = p1;
+                                        bi = e;
+                                    }
+                                }
+                            }
+                            // wave argmax: (k0, k1) largest, then smallest index
+                            for (int off = 32; off >= 1; off >>= 1) {
+                                const int64_t o0 = __shfl_xor(k0, off, kWave);
+                                const int64_t o1 = __shfl_xor(k1, off, kWave);
+                                const int ob = __shfl_xor(bi, off, kWave);
+                                const bool take = o0 > k0 || (o0 == k0 && (o1 > k1 ||
+                                                                          (o1 == k1 && ob < bi)));
+                                if (take) {
+                                    k0 = o0;
+                                    k1 = o1;
+                                    bi = ob;
+                                }
+                            }
+                            pe = uni(bi);
+                        }
+                        int x, y, z;
+                        key_xyz(uni(act[pe]), xb, yb, x, y, z);
+                        x = uni(x);
+                        y = uni(y);
+                        z = uni(z);
+                        const int X2 = x + bw, Y2 = y + bd, Z2 = z + bh;
+                        if (lane == 0) {
+                            Box nbx;
+                            nbx.x = x; nbx.y = y; nbx.z = z; nbx.x2 = X2; nbx.y2 = Y2; nbx.z2 = Z2;
+                            nbx.type = ti;
+                            nbx.pad = uni(S.a) | (uni(S.b) << 8) | (uni(S.k) << 16);
+                            box[n_box] = nbx;
+                            rem[ti] -= cnt_s;
+                        }
+                        n_box++;
+                        wave_sync();
+                        if (uni(rem[ti]) <= 0) remaining_min(rem, tr, nt, wm, dm, hm);
+                        const int need = merge_points(act, cur ? ep_a : ep_b, oidx, nullptr, n_act,
+                                                      n_dead, box, n_box, x, y, z, X2, Y2, Z2, L, W,
+                                                      H, xb, yb, a.ep_cap, false, wm, dm, hm);
+                        ep_need = max(ep_need, (uint32_t)need);
+                        if (lane == 0) {
+                            sh.quit = need > a.ep_cap;   // PLP_E_EPCAP
+                            sh.n_act = n_act;
+                            sh.n_box = n_box;
+                        }
+                    }
+                    block_sync();
+                    if (uni(sh.quit) != 0) {
+                        quit = true;
+                        break;
+                    }
+                    na_w = uni(sh.n_act);
+                    nb_w = uni(sh.n_box);
+                    cur ^= 1;
+                    any = true;
+                }
+                if (quit || !any) break;
+            }
+
+            // ---- pass end (wave 0): decomposed items, FSR, D4'
+            if (wid == 0) {
+                const bool overflow = uni(sh.quit) != 0;
+                DecState st = sst;
+                if (overflow) st.err = 4;   // PLP_E_EPCAP
+                st.ep_need = max(st.ep_need, ep_need);
+                if (st.err == 0) {
+                    // blocks -> items in block order, i (x) outer, j (y), t (z) inner
+                    int n = 0;
+                    int64_t vol = 0;
+                    for (int b = 0; b < n_box; b++) {
+                        const Box B = box[b];
+                        const int ba_ = uni(B.pad) & 0xff, bb = (uni(B.pad) >> 8) & 0xff,
+                                  bk = (uni(B.pad) >> 16) & 0xff;
+                        const int iw = (uni(B.x2) - uni(B.x)) / ba_, id = (uni(B.y2) - uni(B.y)) / bb,
+                                  ih = (uni(B.z2) - uni(B.z)) / bk;
+                        const int cnt_b = ba_ * bb * bk;
+                        for (int l = lane; l < cnt_b; l += kWave) {
+                            const int i = l / (bb * bk), j = (l / bk) % bb, t = l % bk;
+                            Box it;
+                            it.x = uni(B.x) + i * iw; it.y = uni(B.y) + j * id;
+                            it.z = uni(B.z) + t * ih;
+                            it.x2 = it.x + iw; it.y2 = it.y + id; it.z2 = it.z + ih;
+                            it.type = uni(B.type);
+                            it.pad = 0;
+                            items[n + l] = it;
+                        }
+                        vol += (int64_t)cnt_b * iw * id * ih;
+                        n += cnt_b;
+                    }
+                    wave_sync();
+                    const double score = fsr_score(items, n, a.mu, load, skey, stot, nullptr);
+                    if (st.n_all < 0 || vol > st.v_all || (vol == st.v_all && n > st.n_all)) {
+                        st.n_all = n; st.v_all = vol; st.s_all = score;
+                        write_rows(out_all, items, n);
+                    }
+                    const bool accepted = !a.use_theta || score >= a.theta;
+                    if (accepted &&
+                        (st.n_acc < 0 || vol > st.v_acc || (vol == st.v_acc && n > st.n_acc))) {
+                        st.n_acc = n; st.v_acc = vol; st.s_acc = score;
+                        write_rows(out_acc, items, n);
+                    }
+                    st.passes++;
+                }
+                wave_sync();
+                if (lane == 0) sst = st;
+            }
+            block_sync();
+            if (uni(sh.quit) != 0) break;
+        }
+        if (wid == 0 && !skip_decoder) {
+            if (a.use_rng) {
+                for (int i = lane; i < 624; i += kWave) gmt[i] = mt[i];
+                if (lane == 0) gmt[624] = (uint32_t)mti;
+            }
+            if (lane == 0) a.state[d] = sst;
+        }
+        block_sync();
+    }
+}
+
+hipError_t launch_block(const BlockArgs& b, int waves, int grid, int lds_bytes, hipStream_t st) {
+    hipError_t e = launch_fill_u32(b.d.work_counter, 0u, 1, st);
+    if (e != hipSuccess) return e;
+    switch (waves) {
+        case 1: hipLaunchKernelGGL(block_kernel<1>, dim3(grid), dim3(64), (size_t)lds_bytes, st, b); break;
+        case 2: hipLaunchKernelGGL(block_kernel<2>, dim3(grid), dim3(128), (size_t)lds_bytes, st, b); break;
+        case 4: hipLaunchKernelGGL(block_kernel<4>, dim3(grid), dim3(256), (size_t)lds_bytes, st, b); break;
+        case 8: hipLaunchKernelGGL(block_kernel<8>, dim3(grid), dim3(512), (size_t)lds_bytes, st, b); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int NW>
+static hipError_t block_occupancy_of(int lds_bytes, int* blocks_per_cu) {
+    hipError_t e = hipFuncSetAttribute((const void*)block_kernel<NW>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, block_kernel<NW>, 64 * NW,
+                                                        (size_t)lds_bytes);
+}
+
+template <int NW>
+static hipError_t block_static_of(int* bytes) {
+    hipFuncAttributes at;
+    hipError_t e = hipFuncGetAttributes(&at, (const void*)block_kernel<NW>);
+    if (e == hipSuccess) *bytes = (int)at.sharedSizeBytes;
+    return e;
+}
+
+hipError_t block_occupancy(int waves, int lds_bytes, int* blocks_per_cu) {
+    switch (waves) {
+        case 1: return block_occupancy_of<1>(lds_bytes, blocks_per_cu);
+        case 2: return block_occupancy_of<2>(lds_bytes, blocks_per_cu);
+        case 4: return block_occupancy_of<4>(lds_bytes, blocks_per_cu);
+        case 8: return block_occupancy_of<8>(lds_bytes, blocks_per_cu);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t block_static_lds(int waves, int* bytes) {
+    switch (waves) {
+        case 1: return block_static_of<1>(bytes);
+        case 2: return block_static_of<2>(bytes);
+        case 4: return block_static_of<4>(bytes);
+        case 8: return block_static_of<8>(bytes);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace plpk

@@ -31,6 +31,10 @@ struct plp_engine {
     int n_inst = 0, n_dec = 0, max_items = 1, box_cap = 1;
     int ep_cap = 0, lds = 0, grid = 0, occ = 0, wpd = 1;
     int wide = 0;   // 32-bit pair-state cache
+    bool blocks = false;   // block-building decoder (plp_engine_load_blocks)
+    plp_block_params bp{};
+    int max_types = 1;
+    DevBuf d_shape, d_tinfo, d_byt, d_vorder;
     plp_params params{};
     int use_rng = 0;
     int passes_done = 0;
@@ -65,34 +69,47 @@ void drop_graph(plp_engine* g) {
     g->g_passes = g->g_reset = g->g_grid = g->g_cap = g->g_wpd = -1;
 }
 
-// Launch shape for an EP capacity: LDS bytes, resident blocks per CU, grid.
-// Dynamic plus static LDS of one decoder block for a capacity.
-bool lds_fits(plp_engine* g, int ep_cap, int* dyn_bytes) {
-    const plpk::LdsLayout lay = plpk::lds_layout(ep_cap, g->box_cap, g->use_rng, g->wpd,
-                                                 g->cfg.count_work, g->wide);
+// Dynamic LDS bytes of one decoder block for a capacity, for the loaded
+// decoder kind.
+int dyn_lds(plp_engine* g, int ep_cap) {
+    if (g->blocks)
+        return plpk::block_lds_layout(ep_cap, g->box_cap, g->max_types, g->use_rng, g->wpd).total;
+    return plpk::lds_layout(ep_cap, g->box_cap, g->use_rng, g->wpd, g->cfg.count_work, g->wide)
+        .total;
+}
+
+// The decoder kernel's own static LDS bytes.
+int static_lds(plp_engine* g) {
     int st = 0;
-    if (plpk::decode_static_lds(g->wpd, &st) != hipSuccess) {
+    hipError_t e = g->blocks ? plpk::block_static_lds(g->wpd, &st)
+                             : plpk::decode_static_lds(g->wpd, &st);
+    if (e != hipSuccess) {
         (void)hipGetLastError();
         st = 4096;
     }
-    if (dyn_bytes) *dyn_bytes = lay.total;
-    return lay.total + st <= g->ctx->max_lds;
+    return st;
 }
 
+// Whether one block's dynamic plus static LDS fits for a capacity.
+bool lds_fits(plp_engine* g, int ep_cap) {
+    return dyn_lds(g, ep_cap) + static_lds(g) <= g->ctx->max_lds;
+}
+
+// Launch shape for an EP capacity: LDS bytes, resident blocks per CU, grid.
 int shape(plp_engine* g, int ep_cap, int blocks_per_cu_req, int n_work) {
-    const plpk::LdsLayout lay = plpk::lds_layout(ep_cap, g->box_cap, g->use_rng, g->wpd,
-                                                 g->cfg.count_work, g->wide);
-    if (!lds_fits(g, ep_cap, nullptr))
-        return fail(PLP_E_EPCAP, "decoder state (" + std::to_string(lay.total) +
-                                     " B of LDS for " + std::to_string(ep_cap) +
+    const int lds = dyn_lds(g, ep_cap);
+    if (!lds_fits(g, ep_cap))
+        return fail(PLP_E_EPCAP, "decoder state (" + std::to_string(lds) + " B of LDS for " +
+                                     std::to_string(ep_cap) +
                                      " extreme points) exceeds the block LDS limit");
     int occ = 0;
-    hipError_t e = plpk::decode_occupancy(g->wpd, lay.total, &occ);
-    if (e != hipSuccess) return hip_fail(e, "decode_kernel occupancy");
-    if (occ < 1) return fail(PLP_E_EPCAP, "decode_kernel cannot be resident with this LDS size");
+    hipError_t e = g->blocks ? plpk::block_occupancy(g->wpd, lds, &occ)
+                             : plpk::decode_occupancy(g->wpd, lds, &occ);
+    if (e != hipSuccess) return hip_fail(e, "decoder occupancy");
+    if (occ < 1) return fail(PLP_E_EPCAP, "the decoder cannot be resident with this LDS size");
     int bpc = blocks_per_cu_req > 0 ? std::min(blocks_per_cu_req, occ) : occ;
     g->ep_cap = ep_cap;
-    g->lds = lay.total;
+    g->lds = lds;
     g->occ = occ;
     g->grid = std::max(1, std::min(n_work, bpc * g->ctx->n_cu));
     return PLP_OK;
@@ -101,10 +118,10 @@ int shape(plp_engine* g, int ep_cap, int blocks_per_cu_req, int n_work) {
 // The largest extreme-point capacity whose LDS one block can hold.
 int largest_cap(plp_engine* g) {
     int lo = 64, hi = 1 << 15;
-    if (!lds_fits(g, lo, nullptr)) return 0;
+    if (!lds_fits(g, lo)) return 0;
     while (hi - lo > 32) {
         const int mid = (lo + hi) / 2 / 32 * 32;
-        if (lds_fits(g, mid, nullptr)) lo = mid;
+        if (lds_fits(g, mid)) lo = mid;
         else hi = mid;
     }
     return lo;
@@ -141,6 +158,21 @@ plpk::DecodeArgs make_args(plp_engine* g, int n_passes, int reset, const int32_t
     return a;
 }
 
+// One launch of the loaded decoder kind over `grid` blocks.
+hipError_t launch(plp_engine* g, const plpk::DecodeArgs& a, int grid, hipStream_t st) {
+    if (!g->blocks) return plpk::launch_decode(a, g->wpd, grid, g->lds, st);
+    plpk::BlockArgs b;
+    b.d = a;
+    b.shape = (const plpk::BlockShapeRec*)g->d_shape.p;
+    b.tinfo = (const plpk::BlockTypeRec*)g->d_tinfo.p;
+    b.by_type = (const int32_t*)g->d_byt.p;
+    b.vorder = (const int32_t*)g->d_vorder.p;
+    b.topk = g->bp.topk;
+    b.pick = g->bp.pick;
+    b.max_types = g->max_types;
+    return plpk::launch_block(b, g->wpd, grid, g->lds, st);
+}
+
 int launch_run(plp_engine* g, int n_passes, int reset) {
     plp_ctx* c = g->ctx;
     plpk::DecodeArgs a = make_args(g, n_passes, reset, (const int32_t*)g->d_order.p, g->n_dec);
@@ -151,7 +183,7 @@ int launch_run(plp_engine* g, int n_passes, int reset) {
             drop_graph(g);
             e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
             if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-            hipError_t le = plpk::launch_decode(a, g->wpd, g->grid, g->lds, c->stream);
+            hipError_t le = launch(g, a, g->grid, c->stream);
             e = hipStreamEndCapture(c->stream, &g->graph);
             if (le != hipSuccess) return hip_fail(le, "decode capture");
             if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
@@ -167,8 +199,8 @@ int launch_run(plp_engine* g, int n_passes, int reset) {
         if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
         return PLP_OK;
     }
-    e = plpk::launch_decode(a, g->wpd, g->grid, g->lds, c->stream);
-    if (e != hipSuccess) return hip_fail(e, "decode_kernel launch");
+    e = launch(g, a, g->grid, c->stream);
+    if (e != hipSuccess) return hip_fail(e, "decoder launch");
     return PLP_OK;
 }
 
@@ -218,7 +250,8 @@ extern "C" void plp_engine_destroy(plp_engine* g) {
     (void)hipSetDevice(g->ctx->device);
     (void)hipStreamSynchronize(g->ctx->stream);
     drop_graph(g);
-    for (DevBuf* b : {&g->d_inst, &g->d_step, &g->d_step_min, &g->d_ors, &g->d_dec_inst, &g->d_dec_seed,
+    for (DevBuf* b : {&g->d_inst, &g->d_step, &g->d_step_min, &g->d_shape, &g->d_tinfo, &g->d_byt,
+                      &g->d_vorder, &g->d_ors, &g->d_dec_inst, &g->d_dec_seed,
                       &g->d_order, &g->d_retry, &g->d_mt, &g->d_state, &g->d_best_all,
                       &g->d_best_acc, &g->d_counter})
         b->release();
@@ -227,9 +260,131 @@ extern "C" void plp_engine_destroy(plp_engine* g) {
     delete g;
 }
 
+namespace {
+
+// Block catalogue of one instance (blockproof/plp_block.cpp:193-218): shapes
+// stored in placement order (plp_block.cpp:156-165), so a shape's index is
+// its plan sort key; each type's shape list in shape_precedence order
+// (plp_block.cpp:38-45); the types in item-volume order (plp_block.cpp:134-138).
+struct BlockTables {
+    std::vector<plpk::BlockShapeRec> shape;
+    std::vector<plpk::BlockTypeRec> tinfo;
+    std::vector<int32_t> by_type;   // instance-local shape indices
+    std::vector<int32_t> vorder;
+};
+
+void block_tables(const plp::Instance& in, const plp_block_params& bp, int32_t by_type_base,
+                  BlockTables* out) {
+    struct Gen {
+        int ti, oi, a, b, k, cnt, bw, bd, bh;
+    };
+    std::vector<Gen> cat;
+    const int nt = (int)in.items.size();
+    std::vector<plpk::BlockTypeRec> tinfo((size_t)nt);
+    for (int ti = 0; ti < nt; ti++) {
+        const plp::Item& it = in.items[(size_t)ti];
+        const auto ors = plp::orientations(it);
+        plpk::BlockTypeRec& r = tinfo[(size_t)ti];
+        r = plpk::BlockTypeRec{it.qty, ors.empty() ? 0 : 1, 0, 0, kNoBox, kNoBox, kNoBox, 0};
+        for (size_t oi = 0; oi < ors.size(); oi++) {
+            const int w = ors[oi][0], d = ors[oi][1], h = ors[oi][2];
+            r.wmin = std::min(r.wmin, w);
+            r.dmin = std::min(r.dmin, d);
+            r.hmin = std::min(r.hmin, h);
+            for (int a = 1; a <= bp.amax && a * w <= in.L; a++)
+                for (int b = 1; b <= bp.bmax && b * d <= in.W; b++)
+                    for (int k = 1; k <= bp.kmax && k * h <= in.H; k++) {
+                        const long long cnt = (long long)a * b * k;
+                        if (cnt > it.qty) break;
+                        cat.push_back({ti, (int)oi, a, b, k, (int)cnt, a * w, b * d, k * h});
+                    }
+        }
+    }
+    std::vector<int> by_rank(cat.size());
+    for (size_t i = 0; i < cat.size(); i++) by_rank[i] = (int)i;
+    std::sort(by_rank.begin(), by_rank.end(), [&](int x, int y) {
+        const Gen& s = cat[(size_t)x];
+        const Gen& t = cat[(size_t)y];
+        const long long vs = (long long)s.bw * s.bd * s.bh, vt = (long long)t.bw * t.bd * t.bh;
+        if (vs != vt) return vs > vt;
+        if (s.ti != t.ti) return s.ti < t.ti;
+        if (s.oi != t.oi) return s.oi < t.oi;
+        if (s.k != t.k) return s.k > t.k;
+        if (s.a != t.a) return s.a > t.a;
+        return s.b > t.b;
+    });
+    std::vector<int> rank(cat.size());
+    out->shape.clear();
+    for (size_t r = 0; r < by_rank.size(); r++) {
+        const Gen& g = cat[(size_t)by_rank[r]];
+        rank[(size_t)by_rank[r]] = (int)r;
+        out->shape.push_back(plpk::BlockShapeRec{g.ti, g.a, g.b, g.k, g.cnt, g.bw, g.bd, g.bh});
+    }
+    out->by_type.clear();
+    for (int ti = 0; ti < nt; ti++) {
+        std::vector<int> mine;
+        for (size_t i = 0; i < cat.size(); i++)
+            if (cat[i].ti == ti) mine.push_back((int)i);
+        std::sort(mine.begin(), mine.end(), [&](int x, int y) {
+            const Gen& p = cat[(size_t)x];
+            const Gen& q = cat[(size_t)y];
+            if (p.cnt != q.cnt) return p.cnt > q.cnt;
+            if (p.bh != q.bh) return p.bh > q.bh;
+            if (p.a != q.a) return p.a > q.a;
+            if (p.b != q.b) return p.b > q.b;
+            return p.oi < q.oi;
+        });
+        tinfo[(size_t)ti].shp_off = by_type_base + (int32_t)out->by_type.size();
+        tinfo[(size_t)ti].shp_cnt = (int32_t)mine.size();
+        for (int i : mine) out->by_type.push_back(rank[(size_t)i]);
+    }
+    out->tinfo = tinfo;
+    out->vorder.resize((size_t)nt);
+    for (int t = 0; t < nt; t++) out->vorder[(size_t)t] = t;
+    std::sort(out->vorder.begin(), out->vorder.end(), [&](int x, int y) {
+        const plp::Item& a = in.items[(size_t)x];
+        const plp::Item& b = in.items[(size_t)y];
+        const long long vx = (long long)a.l * a.w * a.h, vy = (long long)b.l * b.w * b.h;
+        if (vx != vy) return vx > vy;
+        return x < y;
+    });
+}
+
+int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
+              const int32_t* dec_inst, const uint64_t* dec_seed, int32_t n_dec,
+              const plp_params* params, const plp_block_params* bp);
+
+}  // namespace
+
 extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
                                const int32_t* dec_inst, const uint64_t* dec_seed, int32_t n_dec,
                                const plp_params* params) {
+    return load_impl(g, insts, n_inst, dec_inst, dec_seed, n_dec, params, nullptr);
+}
+
+extern "C" int plp_engine_load_blocks(plp_engine* g, const plp_instance_desc* insts,
+                                      int32_t n_inst, const int32_t* dec_inst,
+                                      const uint64_t* dec_seed, int32_t n_dec,
+                                      const plp_params* params, const plp_block_params* blocks) {
+    if (!blocks) return fail(PLP_E_ARG, "NULL block parameters");
+    plp_block_params bp = *blocks;
+    // the reference CLI clamps these to >= 1 (plp_block.cpp:325-327, 315)
+    bp.amax = std::max(1, bp.amax);
+    bp.bmax = std::max(1, bp.bmax);
+    bp.kmax = std::max(1, bp.kmax);
+    bp.topk = std::max(1, bp.topk);
+    if (bp.amax > 255 || bp.bmax > 255 || bp.kmax > 255)
+        return fail(PLP_E_ARG, "amax, bmax and kmax must be <= 255");
+    if (bp.pick < 0 || bp.pick > 2) return fail(PLP_E_ARG, "pick must be 0, 1 or 2");
+    if (g && g->cfg.count_work) return fail(PLP_E_ARG, "count_work is not supported for blocks");
+    return load_impl(g, insts, n_inst, dec_inst, dec_seed, n_dec, params, &bp);
+}
+
+namespace {
+
+int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
+              const int32_t* dec_inst, const uint64_t* dec_seed, int32_t n_dec,
+              const plp_params* params, const plp_block_params* bp) {
     if (!g || !insts || !dec_inst || !dec_seed || !params) return fail(PLP_E_ARG, "NULL argument");
     if (n_inst < 1 || n_dec < 1) return fail(PLP_E_ARG, "need at least one instance and decoder");
     if (params->order != 0 && params->order != 1) return fail(PLP_E_ARG, "order must be 0 or 1");
@@ -244,6 +399,8 @@ extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, in
     std::vector<int32_t> steps;
     std::vector<int32_t> step_min;   // per step: min w, d, h over the later steps, 0
     std::vector<int32_t> ors;
+    BlockTables bt_all;               // block engine: every instance's tables
+    int max_types = 1;
     std::vector<int64_t> cost((size_t)n_inst, 0);
     int max_items = 1;
     int wide = 0;
@@ -265,6 +422,18 @@ extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, in
             return fail(PLP_E_RANGE, "instance " + std::to_string(i) + ": more than 2^24 - 1 types");
         h.xbits = bitlen(in.L);
         h.ybits = bitlen(in.W);
+        max_types = std::max(max_types, h.n_types);
+        if (bp) {
+            BlockTables bt;
+            block_tables(in, *bp, (int32_t)bt_all.by_type.size(), &bt);
+            h.shape_off = (int32_t)bt_all.shape.size();
+            h.tinfo_off = (int32_t)bt_all.tinfo.size();
+            h.vorder_off = (int32_t)bt_all.vorder.size();
+            bt_all.shape.insert(bt_all.shape.end(), bt.shape.begin(), bt.shape.end());
+            bt_all.tinfo.insert(bt_all.tinfo.end(), bt.tinfo.begin(), bt.tinfo.end());
+            bt_all.by_type.insert(bt_all.by_type.end(), bt.by_type.begin(), bt.by_type.end());
+            bt_all.vorder.insert(bt_all.vorder.end(), bt.vorder.begin(), bt.vorder.end());
+        }
         steps.insert(steps.end(), seq.begin(), seq.end());
         // Componentwise smallest orientation of every type; a point where a
         // box of the smallest dimensions of all later steps cannot go can
@@ -309,7 +478,15 @@ extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, in
     g->box_cap = (max_items + 7) / 8 * 8;   // sentinel-padded groups of 8
     g->wide = wide;
     g->params = *params;
-    g->use_rng = (params->first_fit || params->eps == 0.0) ? 0 : 1;
+    g->blocks = bp != nullptr;
+    if (bp) g->bp = *bp;
+    g->max_types = max_types;
+    g->use_rng = bp ? (params->eps > 0.0 ? 1 : 0)
+                    : ((params->first_fit || params->eps == 0.0) ? 0 : 1);
+    if (bt_all.shape.empty()) bt_all.shape.push_back(plpk::BlockShapeRec{});
+    if (bt_all.tinfo.empty()) bt_all.tinfo.push_back(plpk::BlockTypeRec{});
+    if (bt_all.by_type.empty()) bt_all.by_type.push_back(0);
+    if (bt_all.vorder.empty()) bt_all.vorder.push_back(0);
     g->h_dec_inst.assign(dec_inst, dec_inst + n_dec);
     g->h_seed.assign(dec_seed, dec_seed + n_dec);
     // Work order: most expensive instances first (longest-processing-time
@@ -324,6 +501,10 @@ extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, in
     if ((rc = g->d_inst.reserve(sizeof(InstHdr) * hdr.size(), "instance headers")) ||
         (rc = g->d_step.reserve(4 * steps.size(), "step table")) ||
         (rc = g->d_step_min.reserve(4 * step_min.size(), "step minimum dimensions")) ||
+        (rc = g->d_shape.reserve(sizeof(plpk::BlockShapeRec) * bt_all.shape.size(), "block shapes")) ||
+        (rc = g->d_tinfo.reserve(sizeof(plpk::BlockTypeRec) * bt_all.tinfo.size(), "block types")) ||
+        (rc = g->d_byt.reserve(4 * bt_all.by_type.size(), "block shape lists")) ||
+        (rc = g->d_vorder.reserve(4 * bt_all.vorder.size(), "block type order")) ||
         (rc = g->d_ors.reserve(4 * ors.size(), "orientation table")) ||
         (rc = g->d_dec_inst.reserve(4 * (size_t)n_dec, "decoder map")) ||
         (rc = g->d_dec_seed.reserve(8 * (size_t)n_dec, "decoder seeds")) ||
@@ -342,6 +523,10 @@ extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, in
         {g->d_inst.p, hdr.data(), sizeof(InstHdr) * hdr.size()},
         {g->d_step.p, steps.data(), 4 * steps.size()},
         {g->d_step_min.p, step_min.data(), 4 * step_min.size()},
+        {g->d_shape.p, bt_all.shape.data(), sizeof(plpk::BlockShapeRec) * bt_all.shape.size()},
+        {g->d_tinfo.p, bt_all.tinfo.data(), sizeof(plpk::BlockTypeRec) * bt_all.tinfo.size()},
+        {g->d_byt.p, bt_all.by_type.data(), 4 * bt_all.by_type.size()},
+        {g->d_vorder.p, bt_all.vorder.data(), 4 * bt_all.vorder.size()},
         {g->d_ors.p, ors.data(), 4 * ors.size()},
         {g->d_dec_inst.p, dec_inst, 4 * (size_t)n_dec},
         {g->d_dec_seed.p, dec_seed, 8 * (size_t)n_dec},
@@ -375,6 +560,8 @@ extern "C" int plp_engine_load(plp_engine* g, const plp_instance_desc* insts, in
     g->loaded = true;
     return PLP_OK;
 }
+
+}  // namespace
 
 extern "C" int plp_engine_reset(plp_engine* g) {
     if (!g || !g->loaded) return fail(PLP_E_ARG, "engine not loaded");
@@ -445,7 +632,7 @@ extern "C" int plp_engine_sync(plp_engine* g) {
         plpk::DecodeArgs a = make_args(g, g->passes_done, 1, (const int32_t*)g->d_retry.p,
                                        (int)failed.size());
         if ((e = hipEventRecord(g->ev0, c->stream)) != hipSuccess) return hip_fail(e, "event");
-        e = plpk::launch_decode(a, g->wpd, g->grid, g->lds, c->stream);
+        e = launch(g, a, g->grid, c->stream);
         if (e != hipSuccess) return hip_fail(e, "decode retry launch");
         if ((e = hipEventRecord(g->ev1, c->stream)) != hipSuccess) return hip_fail(e, "event");
         if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "retry sync");

@@ -14,7 +14,10 @@ struct InstHdr {
     int32_t n_types;
     int32_t xbits;      // bit length of L: packed key = z << (xbits+ybits) | y << xbits | x
     int32_t ybits;      // bit length of W
-    int32_t pad[7];
+    // block-building engine only: first block shape, type record and entry of
+    // the volume-ordered type list of this instance
+    int32_t shape_off, tinfo_off, vorder_off;
+    int32_t pad[4];
 };
 static_assert(sizeof(InstHdr) == 64, "InstHdr layout");
 
@@ -117,6 +120,73 @@ __host__ __device__ inline LdsLayout lds_layout(int32_t ep_cap, int32_t box_cap,
     l.off_mt = l.off_box + (box_cap + 8) * 32;
     l.off_steps = l.off_mt + (use_rng ? 624 * 4 : 0);   // per step: type | n_ors << 24
     l.total = l.off_steps + box_cap * 4;
+    return l;
+}
+
+// ---------------------------------------------- block-building engine ----
+
+// One block shape (blockproof/plp_block.cpp:27-33): cnt = a*b*k copies of
+// item ti in one orientation as a bw x bd x bh stack.  An instance's shapes
+// are stored in placement order (plp_block.cpp:156-165), so a shape's index
+// is its sort key when a sweep's plan is ordered.
+struct BlockShapeRec {
+    int32_t ti, a, b, k, cnt, bw, bd, bh;
+};
+
+// Per (instance, type): copies, whether any orientation is allowed, the
+// type's shapes in shape_precedence order (by_type[shp_off .. +shp_cnt)),
+// and the componentwise smallest orientation.
+struct BlockTypeRec {
+    int32_t qty, has_ors, shp_off, shp_cnt, wmin, dmin, hmin, pad;
+};
+
+struct BlockArgs {
+    DecodeArgs d;                  // queue, decoder state, outputs, capacities, eps / theta / mu
+    const BlockShapeRec* shape;    // all instances' shapes (InstHdr::shape_off)
+    const BlockTypeRec* tinfo;     // all instances' types (InstHdr::tinfo_off)
+    const int32_t* by_type;        // shape indices per type, precedence order
+    const int32_t* vorder;         // per instance: types by item volume desc, index asc
+    int32_t topk;                  // eps-greedy choice among the topk largest fitting shapes
+    int32_t pick;                  // 0 first, 1 support, 2 contact (plp_block.cpp:56)
+    int32_t max_types;             // largest type count of the batch
+};
+
+// LDS of one block-decoder block: two live lists + their previous-list
+// index, verdict masks, remaining copies and the type list, the sweep plan
+// (sorted in place), per-wave culled box windows; after the pass the first
+// region holds FSR's arrays and the decomposed items.  Then the placed
+// blocks and the MT state.
+struct BlockLds {
+    int32_t off_ep, off_ep2, off_oidx, off_mask, off_rem, off_tlist, off_plan, n_plan, off_rel;
+    int32_t off_load, off_key, n_key, off_tot, off_items;
+    int32_t off_box, off_mt, total;
+};
+
+__host__ __device__ inline BlockLds block_lds_layout(int32_t ep_cap, int32_t box_cap,
+                                                     int32_t max_types, int32_t use_rng,
+                                                     int32_t n_waves) {
+    BlockLds l;
+    const int32_t list = (ep_cap + 8) * 4;
+    l.off_ep = 0;
+    l.off_ep2 = list;
+    l.off_oidx = 2 * list;
+    l.off_mask = (l.off_oidx + (ep_cap + 8) * 2 + 7) & ~7;
+    l.off_rem = l.off_mask + ((ep_cap + 63) / 64) * 8;
+    l.off_tlist = l.off_rem + max_types * 4;
+    l.off_plan = l.off_tlist + max_types * 4;
+    l.n_plan = next_pow2(box_cap > 0 ? box_cap : 1);
+    l.off_rel = (l.off_plan + l.n_plan * 4 + 15) & ~15;
+    const int32_t pass_end = l.off_rel + n_waves * (64 + 8) * 32;
+    l.n_key = next_pow2(box_cap > 0 ? box_cap : 1);
+    l.off_load = 0;
+    l.off_key = box_cap * 8;
+    l.off_tot = l.off_key + l.n_key * 8;
+    l.off_items = (l.off_tot + box_cap * 4 + 15) & ~15;
+    const int32_t fsr_end = l.off_items + box_cap * 32;
+    const int32_t region = pass_end > fsr_end ? pass_end : fsr_end;
+    l.off_box = (region + 15) & ~15;
+    l.off_mt = l.off_box + (box_cap + 8) * 32;
+    l.total = l.off_mt + (use_rng ? 624 * 4 : 0);
     return l;
 }
 

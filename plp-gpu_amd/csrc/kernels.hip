@@ -17,403 +17,12 @@
 //   rng_script_kernel  device MT19937 probe (plp_gpu.cu:642-661)
 #include "kernels.hpp"
 
+#include "decoder_dev.hpp"
 #include "engine_types.hpp"
 #include "mt19937.hpp"
 #include "wave.hpp"
 
 namespace plpk {
-
-// Placed box in LDS: AABB + item type (32 bytes; the hot loop reads 24).
-struct Box {
-    int32_t x, y, z, x2, y2, z2, type, pad;
-};
-
-__device__ __forceinline__ void key_xyz(uint32_t key, int xb, int yb, int& x, int& y, int& z) {
-    x = (int)(key & ((1u << xb) - 1u));
-    y = (int)((key >> xb) & ((1u << yb) - 1u));
-    z = (int)(key >> (xb + yb));
-}
-
-__device__ __forceinline__ uint32_t xyz_key(int x, int y, int z, int xb, int yb) {
-    return ((uint32_t)z << (xb + yb)) | ((uint32_t)y << xb) | (uint32_t)x;
-}
-
-// check_feasible_one (plp_common.hpp:297-321) for this lane's pair:
-//   C2  x+w <= L, y+d <= W, z+h <= H
-//   C3  z > 0: s = sum of top-face overlaps of boxes whose top is at z;
-//       infeasible iff (double)s / (double)(w*d) < 0.5
-//   C1  no strict-interior overlap with any box
-// pair_scan takes lanes that passed C2 (alive) and returns C1 together with
-// the support sum s, which is complete for every lane that passes C1; the
-// caller applies C3 (pair_verdict).  The result does not depend on which
-// boxes are visited in what order as long as every box that can overlap or
-// support the pair is visited.  The wave walks the placed boxes newest first
-// in windows of 64 (one box per lane), keeps the ones that intersect the
-// chunk's region [xlo,xhi) x [ylo,yhi) x [zlo,zhi) or whose top lies in
-// [zlo, zhi) over that footprint (a box outside it can neither overlap nor
-// support any pair of the chunk), copies them into the wave's cbox[] window,
-// pads it with sentinels to a multiple of kBoxGroup and scans it in groups
-// whose LDS reads issue together.  The wave leaves as soon as every lane has
-// failed, and skips the support arithmetic of a box whose top is at no live
-// lane's level (both wave-uniform branches).  Areas fit 32 bits: L*W < 2^30
-// is a load-time precondition, and ox, oy < 2^21 so the 24-bit multiply is
-// exact.
-constexpr int kBoxGroup = 4;
-constexpr int kCullWindow = kWave + kBoxGroup;   // cbox[] entries per wave
-
-__device__ __forceinline__ Box sentinel_box() {
-    Box b;
-    b.x = b.y = b.z = b.x2 = b.y2 = b.z2 = -1;   // never overlaps, never supports
-    b.type = 0;
-    b.pad = 0;
-    return b;
-}
-
-__device__ __forceinline__ bool pair_scan(int x, int y, int z, int w, int d, int h, bool alive,
-                                          const Box* __restrict__ boxes, int n_box, int xlo,
-                                          int xhi, int ylo, int yhi, int zlo, int zhi,
-                                          Box* __restrict__ cbox, uint32_t& s_out) {
-    const int lane = lane_id();
-    const int x2 = x + w, y2 = y + d, z2 = z + h;
-    uint32_t s = 0;
-    for (int top = n_box - 1; top >= 0; top -= kWave) {
-        if (ballot(alive) == 0) break;
-        const int b = top - lane;
-        Box C = sentinel_box();
-        if (b >= 0) C = boxes[b];
-        const bool r = (b >= 0) & (C.x < xhi) & (C.x2 > xlo) & (C.y < yhi) & (C.y2 > ylo) &
-                       (C.z < zhi) & (C.z2 >= zlo);
-        const uint64_t rm = ballot(r);
-        const int n_rel = __popcll(rm);
-        if (n_rel == 0) continue;
-        if (r) cbox[rank_below(rm)] = C;
-        const int n_pad = (n_rel + kBoxGroup - 1) / kBoxGroup * kBoxGroup;
-        if (lane < n_pad - n_rel) cbox[n_rel + lane] = sentinel_box();
-        wave_sync();
-        for (int g = 0; g < n_pad; g += kBoxGroup) {
-            Box B[kBoxGroup];
-#pragma unroll
-            for (int u = 0; u < kBoxGroup; u++) B[u] = cbox[g + u];
-#pragma unroll
-            for (int u = 0; u < kBoxGroup; u++) {
-                const bool ov = (x < B[u].x2) & (B[u].x < x2) & (y < B[u].y2) & (B[u].y < y2) &
-                                (z < B[u].z2) & (B[u].z < z2);
-                alive = alive & !ov;
-                if (ballot(alive & (B[u].z2 == z))) {
-                    const int ox = min(x2, B[u].x2) - max(x, B[u].x);
-                    const int oy = min(y2, B[u].y2) - max(y, B[u].y);
-                    const bool sup = (B[u].z2 == z) & (ox > 0) & (oy > 0);
-                    s += sup ? __umul24((uint32_t)ox, (uint32_t)oy) : 0u;
-                }
-            }
-            if (ballot(alive) == 0) break;
-        }
-        wave_sync();   // the next window overwrites cbox[]
-    }
-    s_out = s;
-    return alive;
-}
-
-// C3 on top of C1 and C2 (plp_common.hpp:304-312), in the reference's FP64 form.
-__device__ __forceinline__ bool pair_verdict(bool c12, int z, int w, int d, uint32_t s) {
-    return c12 && (z == 0 || !((double)s / (double)((int64_t)w * d) < 0.5));
-}
-
-// Pair-state cache entry: the support sum of a pair that passes C1 and C2,
-// or kBlocked.  Both only grow as boxes are added, so the state after one
-// more box follows from the old state and that box alone.
-constexpr uint32_t kBlocked = 0xffffffffu;
-constexpr uint16_t kNewPoint = 0xffffu;   // list entry with no cached state
-constexpr uint16_t kCorner = 0xfff0u;     // kCorner + j: corner j of the last box (state in cstate)
-
-// Entries are 32-bit, or 16-bit with the support sum clamped to 65534 and
-// 0xffff for kBlocked (exact for C3 when every w*d <= kNarrowMaxArea).
-__device__ __forceinline__ uint32_t cache_get(const void* c, int i, bool wide) {
-    if (wide) return ((const uint32_t*)c)[i];
-    const uint32_t v = ((const uint16_t*)c)[i];
-    return v == 0xffffu ? kBlocked : v;
-}
-
-__device__ __forceinline__ void cache_put(void* c, int i, uint32_t v, bool wide) {
-    if (wide) ((uint32_t*)c)[i] = v;
-    else ((uint16_t*)c)[i] = (uint16_t)(v == kBlocked ? 0xffffu : min(v, 0xfffeu));
-}
-
-// ---------------------------------------------------------------- FSR ----
-// compute_fsr + stability_score (plp_common.hpp:356-399) for the n boxes in
-// LDS, on one wave.  The reference adds load[k] * a / total into load[j]
-// while visiting k in (z desc, index asc) order; that order is produced by a
-// bitonic sort of 64-bit keys ((2^31-1-z) << 32 | index) in LDS, and the k
-// loop stays serial so every load[j] receives its additions in the
-// reference's order (bit-exact FP64).  Returns the score; per-box FSR goes to
-// fsr_out when non-null.
-__device__ double fsr_score(const Box* __restrict__ box, int n, double mu, double* load,
-                            uint64_t* skey, uint32_t* tot, double* fsr_out) {
-    const int lane = lane_id();
-    if (n <= 0) return 0.0;
-    const int n2 = next_pow2(n);
-    for (int i = lane; i < n2; i += kWave)
-        skey[i] = i < n ? (((uint64_t)(0x7fffffffu - (uint32_t)box[i].z) << 32) | (uint32_t)i)
-                        : ~0ull;
-    // load = m = RHO * V / 1e6; tot[k] = contact area of box k with the boxes
-    // whose top is at its bottom (lane per k, every j in turn).  Those boxes
-    // do not overlap one another, so tot[k] <= w * d < 2^30.
-    for (int k = lane; k < n; k += kWave) {
-        const Box Bk = box[k];
-        const int64_t v = (int64_t)(Bk.x2 - Bk.x) * (Bk.y2 - Bk.y) * (Bk.z2 - Bk.z);
-        load[k] = 50.0 * (double)v / 1.0e6;
-        uint32_t t = 0;
-        if (Bk.z > 0) {
-            for (int j = 0; j < n; j++) {
-                const Box Bj = box[j];
-                const int ox = min(Bk.x2, Bj.x2) - max(Bk.x, Bj.x);
-                const int oy = min(Bk.y2, Bj.y2) - max(Bk.y, Bj.y);
-                if (j != k && Bj.z2 == Bk.z && ox > 0 && oy > 0)
-                    t += (uint32_t)ox * (uint32_t)oy;
-            }
-        }
-        tot[k] = t;
-    }
-    wave_sync();
-    // Bitonic sort, ascending.
-    for (int k = 2; k <= n2; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = lane; i < n2; i += kWave) {
-                const int ixj = i ^ j;
-                if (ixj > i) {
-                    const uint64_t p = skey[i], q = skey[ixj];
-                    const bool up = (i & k) == 0;
-                    if ((p > q) == up) {
-                        skey[i] = q;
-                        skey[ixj] = p;
-                    }
-                }
-            }
-            wave_sync();
-        }
-    }
-    for (int t = 0; t < n; t++) {
-        const int k = (int)(uint32_t)uni(skey[t]);
-        const Box Bk = box[k];
-        const int bkz = uni(Bk.z);
-        if (bkz == 0) break;   // the rest are floor boxes: nothing supports them
-        const int bkx = uni(Bk.x), bky = uni(Bk.y), bkx2 = uni(Bk.x2), bky2 = uni(Bk.y2);
-        const uint32_t total = uni(tot[k]);
-        if (total > 0) {
-            const double lk = load[k];
-            for (int j = lane; j < n; j += kWave) {
-                const Box Bj = box[j];
-                if (j != k && Bj.z2 == bkz) {
-                    const int ox = min(bkx2, Bj.x2) - max(bkx, Bj.x);
-                    const int oy = min(bky2, Bj.y2) - max(bky, Bj.y);
-                    if (ox > 0 && oy > 0) {
-                        const uint64_t a = (uint64_t)ox * (uint64_t)oy;
-                        load[j] = load[j] + lk * (double)a / (double)total;
-                    }
-                }
-            }
-        }
-        wave_sync();
-    }
-    uint32_t cnt = 0;
-    for (int i = lane; i < n; i += kWave) {
-        const int64_t v = (int64_t)(box[i].x2 - box[i].x) * (box[i].y2 - box[i].y) *
-                          (box[i].z2 - box[i].z);
-        const double m = 50.0 * (double)v / 1.0e6;
-        const double f = (mu / 0.3) * load[i] / m;
-        if (fsr_out) fsr_out[i] = f;
-        cnt += (f >= 1.0) ? 1u : 0u;
-    }
-    const uint64_t c = wave_sum_u64(cnt);
-    wave_sync();
-    return (double)c / (double)n;
-}
-
-// Extreme-point update after placing the box [x, X2) x [y, Y2) x [z, Z2)
-// (plp_common.hpp:439-453): points strictly inside the box leave the list,
-// the box's 8 corners join it unless already present, and the list stays
-// sorted and unique.  The live list keeps only points where a box of the
-// later steps' smallest dimensions (wm, dm, hm, componentwise over every
-// orientation) fits inside the walls without overlapping a placed box: any
-// later box contains that one, so elsewhere nothing can ever be placed.  On
-// the live list src -> dst: the points whose smallest box meets the new box
-// or a wall leave (the strictly inside ones among them leave the reference's
-// list too), and the useful corners not already in src are inserted; oidx[]
-// records where each dst entry was in src (kCorner + j for inserted corner
-// j).  Usefulness only shrinks as boxes are added and the later steps'
-// smallest box grows, so a useful corner never equals a point that left, and
-// a corner equal to a point still listed is not inserted.  With count_work
-// the other points of the reference's list are kept (unsorted) in dead[]: it
-// drops its strictly inside points and receives the points that leave the
-// live list without being strictly inside and the corners that are neither
-// useful nor already listed.  n_act <= cap on entry keeps every write within
-// the cap + 8 slots of a list.  One wave; returns the larger new list size.
-__device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t* oidx, uint32_t* dead,
-                            int& n_act, int& n_dead, const Box* box, int n_box, int x, int y, int z,
-                            int X2, int Y2, int Z2, int L, int W, int H, int xb, int yb, int cap,
-                            bool count_work, int wm, int dm, int hm) {
-    const int lane = lane_id();
-    // Corner j: bit 0 -> x side, bit 1 -> y, bit 2 -> z.  Lane j < 8 holds
-    // corner j's key in myk.
-    uint32_t ck[8];
-    uint32_t cdead = 0;
-    uint32_t myk = 0;
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-        const int jx = (j & 1) ? X2 : x, jy = (j & 2) ? Y2 : y, jz = (j & 4) ? Z2 : z;
-        ck[j] = xyz_key(jx, jy, jz, xb, yb);
-        if (lane == j) myk = ck[j];
-        if ((jx + wm > L) | (jy + dm > W) | (jz + hm > H)) cdead |= 1u << j;
-    }
-    // Each corner's lower bound in src (lane j < 8 keeps corner j's in lo)
-    // and whether src holds it.
-    int lo = 0;
-    uint32_t exists = 0;
-    for (int i0 = 0; i0 < n_act; i0 += kWave) {
-        const int i = i0 + lane;
-        const uint32_t v = i < n_act ? src[i] : 0xffffffffu;
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int c = __popcll(ballot(v < ck[j]));
-            if (lane == j) lo += c;
-            if (ballot(v == ck[j])) exists |= 1u << j;
-        }
-    }
-    // Corners whose smallest box meets a placed box (the new one included)
-    // are not useful.
-    for (int b0 = 0; b0 < n_box; b0 += kWave) {
-        const int bi = b0 + lane;
-        Box B = sentinel_box();
-        if (bi < n_box) B = box[bi];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int jx = (j & 1) ? X2 : x, jy = (j & 2) ? Y2 : y, jz = (j & 4) ? Z2 : z;
-            const bool in = (jx < B.x2) & (B.x < jx + wm) & (jy < B.y2) & (B.y < jy + dm) &
-                            (jz < B.z2) & (B.z < jz + hm);
-            if (ballot(in)) cdead |= 1u << j;
-        }
-    }
-    const uint32_t add = ~cdead & ~exists & 0xffu;   // useful corners that join the list
-
-    // Dead list: drop the strictly inside points in place (a point moves to
-    // an index no larger than its own, below the next 64-point chunk).
-    uint32_t eq_dead = 0;
-    int nd = n_dead;
-    if (count_work) {
-        int db = 0;
-        for (int i0 = 0; i0 < n_dead; i0 += kWave) {
-            const int i = i0 + lane;
-            const bool ok = i < n_dead;
-            const uint32_t v = ok ? dead[i] : 0u;
-            int px, py, pz;
-            key_xyz(v, xb, yb, px, py, pz);
-            const bool strict = (x < px) & (px < X2) & (y < py) & (py < Y2) & (z < pz) & (pz < Z2);
-            const bool keep = ok & !strict;
-#pragma unroll
-            for (int j = 0; j < 8; j++)
-                if (ballot(keep & (v == ck[j]))) eq_dead |= 1u << j;
-            const uint64_t km = ballot(keep);
-            if (keep) dead[db + rank_below(km)] = v;
-            db += __popcll(km);
-        }
-        nd = uni(db);
-        wave_sync();
-    }
-
-    // Live list: survivors go to dst after the inserted corners below them;
-    // lane j < 8 counts the points below corner j that leave (rb).
-    int base = 0, rb = 0;
-    for (int i0 = 0; i0 < n_act; i0 += kWave) {
-        const int i = i0 + lane;
-        const bool ok = i < n_act;
-        const uint32_t v = ok ? src[i] : 0u;
-        int px, py, pz;
-        key_xyz(v, xb, yb, px, py, pz);
-        const bool gone = ok & ((px + wm > L) | (py + dm > W) | (pz + hm > H) |
-                                ((px < X2) & (x < px + wm) & (py < Y2) & (y < py + dm) &
-                                 (pz < Z2) & (z < pz + hm)));
-        int shift = 0;
-#pragma unroll
-        for (int j = 0; j < 8; j++) shift += (((add >> j) & 1u) && ck[j] < v) ? 1 : 0;
-        const bool keep = ok & !gone;
-        const uint64_t km = ballot(keep);
-        if (keep) {
-            const int pos = base + rank_below(km) + shift;
-            dst[pos] = v;
-            oidx[pos] = (uint16_t)i;
-        }
-        base += __popcll(km);
-        const uint64_t gm = ballot(gone);
-        if (gm) {
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int c = __popcll(gm & ballot(v < ck[j]));
-                if (lane == j) rb += c;
-            }
-            if (count_work) {
-                const bool strict = (x < px) & (px < X2) & (y < py) & (py < Y2) & (z < pz) &
-                                    (pz < Z2);
-                const bool mv = gone & !strict;
-                const uint64_t mm = ballot(mv);
-                const int pos = nd + rank_below(mm);
-                if (mv && pos < cap + 8) dead[pos] = v;
-                nd += __popcll(mm);
-            }
-        }
-    }
-    base = uni(base);
-    if (lane < 8 && ((add >> lane) & 1u)) {
-        int r = 0;
-#pragma unroll
-        for (int j = 0; j < 8; j++) r += (((add >> j) & 1u) && ck[j] < myk) ? 1 : 0;
-        dst[lo - rb + r] = myk;
-        oidx[lo - rb + r] = (uint16_t)(kCorner + lane);
-    }
-    if (count_work) {
-        const uint32_t dadd = cdead & ~exists & ~eq_dead;
-        if (lane < 8 && ((dadd >> lane) & 1u)) {
-            const int pos = nd + __popc(dadd & ((1u << lane) - 1u));
-            if (pos < cap + 8) dead[pos] = myk;
-        }
-        nd += __popc(dadd);
-    }
-    wave_sync();
-    n_act = base + __popc(add);
-    n_dead = count_work ? uni(nd) : 0;
-    return max(n_act, n_dead);
-}
-
-// The pair states (cache entries) of the 8 corners of the box nb just
-// placed, for the orientations in ors (no of them), against all n_box boxes:
-// lane j * no + o.  One wave.
-__device__ void corner_states(const int* nb, uint32_t* cstate, const int* ors, int no,
-                              const Box* box, int n_box, int L, int W, int H, Box* cbox) {
-    const int lane = lane_id();
-    const int n = 8 * no;
-    const int j = lane / no, o = lane - (lane / no) * no;
-    int x = 0, y = 0, z = 0, w = 1, d = 1, h = 1;
-    if (lane < n) {
-        x = (j & 1) ? nb[3] : nb[0];
-        y = (j & 2) ? nb[4] : nb[1];
-        z = (j & 4) ? nb[5] : nb[2];
-        w = ors[3 * o];
-        d = ors[3 * o + 1];
-        h = ors[3 * o + 2];
-    }
-    const bool c2 = lane < n && (x + w <= L) && (y + d <= W) && (z + h <= H);
-    bool c12 = false;
-    uint32_t s = 0;
-    if (ballot(c2) != 0) {
-        const int xlo = wave_min(c2 ? x : 0x7fffffff);
-        const int xhi = wave_max(c2 ? x + w : -1);
-        const int ylo = wave_min(c2 ? y : 0x7fffffff);
-        const int yhi = wave_max(c2 ? y + d : -1);
-        const int zlo = wave_min(c2 ? z : 0x7fffffff);
-        const int zhi = wave_max(c2 ? z + h : -1);
-        c12 = pair_scan(x, y, z, w, d, h, c2, box, n_box, xlo, xhi, ylo, yhi, zlo, zhi, cbox, s);
-    }
-    if (lane < n) cstate[lane] = c12 ? s : kBlocked;
-}
 
 // ------------------------------------------------------------ decoder ----
 
@@ -430,24 +39,6 @@ struct StepShared {
     int cnt[2][8];  // feasible pairs found by each wave, by evaluated-step parity:
                     // a wave may start step k+1 while others still read step k
 };
-
-__device__ __forceinline__ void write_rows(int32_t* dst, const Box* box, int n) {
-    for (int t = lane_id(); t < 7 * n; t += kWave) {
-        const int i = t / 7, c = t - (t / 7) * 7;
-        const Box& B = box[i];
-        int v;
-        switch (c) {
-            case 0: v = B.type; break;
-            case 1: v = B.x; break;
-            case 2: v = B.y; break;
-            case 3: v = B.z; break;
-            case 4: v = B.x2 - B.x; break;
-            case 5: v = B.y2 - B.y; break;
-            default: v = B.z2 - B.z; break;
-        }
-        dst[t] = v;
-    }
-}
 
 // Persistent decoder: one block of NW waves decodes one (instance, seed)
 // for n_passes passes (plp_common.hpp:404-456 per pass, FSR and the D4'
@@ -916,6 +507,11 @@ hipError_t launch_decode(const DecodeArgs& a, int waves, int grid, int lds_bytes
         case 8: hipLaunchKernelGGL(decode_kernel<8>, dim3(grid), dim3(512), (size_t)lds_bytes, st, a); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st) {
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, v, n);
     return hipGetLastError();
 }
 

@@ -28,7 +28,7 @@ EXPORTS = [
     "plp_status_string", "plp_last_error", "plp_device_count", "plp_parse_instance",
     "plp_ctx_create", "plp_ctx_destroy", "plp_feas_check_all", "plp_feas_check_batch",
     "plp_fsr_eval", "plp_rng_script", "plp_plain_run", "plp_engine_create",
-    "plp_engine_destroy", "plp_engine_load", "plp_engine_reset", "plp_engine_run", "plp_engine_sync",
+    "plp_engine_destroy", "plp_engine_load", "plp_engine_load_blocks", "plp_engine_reset", "plp_engine_run", "plp_engine_sync",
     "plp_engine_max_items", "plp_engine_fetch", "plp_engine_last_ms", "plp_engine_describe",
     "plp_engine_autotune", "plp_server_create", "plp_server_destroy", "plp_server_submit",
     "plp_server_poll", "plp_server_inflight",
@@ -55,6 +55,19 @@ class Params(C.Structure):
 class EngineConfig(C.Structure):
     _fields_ = [("waves_per_decoder", C.c_int32), ("ep_cap", C.c_int32),
                 ("blocks_per_cu", C.c_int32), ("use_graph", C.c_int32), ("count_work", C.c_int32)]
+
+
+class BlockParams(C.Structure):
+    _fields_ = [("amax", C.c_int32), ("bmax", C.c_int32), ("kmax", C.c_int32),
+                ("topk", C.c_int32), ("pick", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+
+PICK = {"first": 0, "support": 1, "contact": 2}
+
+
+def block_params(amax=3, bmax=3, kmax=4, topk=8, pick="first") -> BlockParams:
+    """Block-engine options (reference defaults, blockproof/plp_block.cpp:297-300)."""
+    return BlockParams(amax, bmax, kmax, topk, PICK[pick] if isinstance(pick, str) else pick)
 
 
 class RowResult(C.Structure):
@@ -86,6 +99,8 @@ _sig = {
     "plp_engine_destroy": (None, [_P]),
     "plp_engine_load": (C.c_int, [_P, C.POINTER(InstanceDesc), C.c_int32, _P, _P, C.c_int32,
                                   C.POINTER(Params)]),
+    "plp_engine_load_blocks": (C.c_int, [_P, C.POINTER(InstanceDesc), C.c_int32, _P, _P,
+                                         C.c_int32, C.POINTER(Params), C.POINTER(BlockParams)]),
     "plp_engine_reset": (C.c_int, [_P]),
     "plp_engine_run": (C.c_int, [_P, C.c_int32]),
     "plp_engine_sync": (C.c_int, [_P]),
@@ -260,7 +275,9 @@ class Engine:
     def __del__(self):
         self.close()
 
-    def load(self, instances, dec_inst, dec_seed, prm: Params):
+    def load(self, instances, dec_inst, dec_seed, prm: Params, blocks: BlockParams = None):
+        """Item decoders (plp_engine_load), or block-building decoders when
+        blocks is given (plp_engine_load_blocks)."""
         insts = list(instances)
         self._keep = insts
         arr = (InstanceDesc * len(insts))(*[i.desc() for i in insts])
@@ -268,8 +285,12 @@ class Engine:
         ds = np.ascontiguousarray(np.asarray(dec_seed, np.uint64))
         self._arrs = (arr, di, ds)
         self.n_dec = len(di)
-        _check(_lib.plp_engine_load(self.h, arr, len(insts), _ptr(di), _ptr(ds), len(di),
-                                    C.byref(prm)))
+        if blocks is None:
+            _check(_lib.plp_engine_load(self.h, arr, len(insts), _ptr(di), _ptr(ds), len(di),
+                                        C.byref(prm)))
+        else:
+            _check(_lib.plp_engine_load_blocks(self.h, arr, len(insts), _ptr(di), _ptr(ds),
+                                               len(di), C.byref(prm), C.byref(blocks)))
 
     def reset(self):
         _check(_lib.plp_engine_reset(self.h))

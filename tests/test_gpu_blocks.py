@@ -1,0 +1,132 @@
+"""The gfx950 block-building decoder (block_kernel.hip) against the reference
+block engine (blockproof/plp_block.cpp): every fixture run of
+tests/golden/block_runs.json, and many decoders against the C oracle."""
+import json
+from collections import defaultdict
+
+import pytest
+
+from conftest import golden_path
+from oracle_lib import Oracle, load_text
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("n_boxes", "volume_cm3", "stability_score", "fallback", "n_passes", "seed",
+          "placements")
+
+
+@pytest.fixture(scope="module")
+def plp():
+    import plp_amd
+    assert plp_amd.device_count() >= 1, "no HIP device"
+    return plp_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(plp):
+    c = plp.Ctx(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle()
+
+
+def _params(plp, a):
+    return (plp.params(eps=a["eps"], theta=0.9 if a["theta"] is None else a["theta"],
+                       use_theta=a["theta"] is not None),
+            plp.block_params(a["amax"], a["bmax"], a["kmax"], a["topk"], a["pick"]))
+
+
+def test_every_reference_block_run(plp, ctx):
+    """One engine per parameter set, one decoder per fixture case."""
+    with open(golden_path("block_runs.json")) as f:
+        cases = json.load(f)
+    groups = defaultdict(list)
+    for c in cases:
+        a = c["args"]
+        groups[(a["passes"], a["eps"], a["theta"], a["amax"], a["bmax"], a["kmax"], a["topk"],
+                a["pick"])].append(c)
+    for key, cs in groups.items():
+        prm, bp = _params(plp, cs[0]["args"])
+        insts = [plp.Instance.from_text(load_text(c["instance"])) for c in cs]
+        eng = plp.Engine(ctx)
+        eng.load(insts, list(range(len(cs))), [c["args"]["seed"] for c in cs], prm, blocks=bp)
+        eng.run(key[0])
+        eng.sync()
+        rows, pl = eng.fetch()
+        for i, c in enumerate(cs):
+            assert rows[i].err == 0, c["case"]
+            got = plp.to_json_dict(rows[i], pl[i])
+            assert {k: got[k] for k in FIELDS} == {k: c["result"][k] for k in FIELDS}, c["case"]
+        eng.close()
+
+
+@pytest.mark.parametrize("wpd", [1, 2, 4])
+def test_many_block_decoders_vs_oracle(plp, ctx, orc, wpd):
+    """128 decoders of the 20-type instance, eps 0.3, 2 passes, first pick."""
+    text = load_text("synth/multi.txt")
+    o_inst = orc.parse(text)[0]
+    K = 128
+    eng = plp.Engine(ctx, waves_per_decoder=wpd)
+    eng.load([plp.Instance.from_text(text)], [0] * K, list(range(1, K + 1)),
+             plp.params(eps=0.3, theta=0.9), blocks=plp.block_params())
+    eng.run(2)
+    eng.sync()
+    rows, pl = eng.fetch()
+    for k in range(K):
+        want = orc.block_run(o_inst, seed=k + 1, passes=2, eps=0.3, theta=0.9)
+        got = plp.to_json_dict(rows[k], pl[k])
+        assert {x: got[x] for x in FIELDS} == {x: want[x] for x in FIELDS}, k
+
+
+@pytest.mark.parametrize("pick", ["support", "contact"])
+def test_contact_pick_rules_vs_oracle(plp, ctx, orc, pick):
+    text = load_text("data/instances/BR2_050.txt")
+    o_inst = orc.parse(text)[0]
+    K = 48
+    eng = plp.Engine(ctx)
+    eng.load([plp.Instance.from_text(text)], [0] * K, list(range(100, 100 + K)),
+             plp.params(eps=0.1, theta=0.9), blocks=plp.block_params(pick=pick, topk=4))
+    eng.run(1)
+    eng.sync()
+    rows, pl = eng.fetch()
+    for k in range(K):
+        want = orc.block_run(o_inst, seed=100 + k, passes=1, eps=0.1, theta=0.9, pick=pick, topk=4)
+        got = plp.to_json_dict(rows[k], pl[k])
+        assert {x: got[x] for x in FIELDS} == {x: want[x] for x in FIELDS}, k
+
+
+def test_mixed_block_batch_vs_oracle(plp, ctx, orc):
+    """Uneven instances (3..100 types) x 3 seeds in one launch."""
+    from plp_amd import synth
+    texts = synth.mixed_batch(12, seed0=515_000)
+    insts = [plp.Instance.from_text(t) for t in texts]
+    di = [i for i in range(len(texts)) for _ in range(3)]
+    ds = [7 + s for _ in range(len(texts)) for s in range(3)]
+    eng = plp.Engine(ctx)
+    eng.load(insts, di, ds, plp.params(eps=0.2, theta=0.9), blocks=plp.block_params(kmax=3))
+    eng.run(2)
+    eng.sync()
+    rows, pl = eng.fetch()
+    for k, (i, s) in enumerate(zip(di, ds)):
+        want = orc.block_run(orc.parse(texts[i])[0], seed=s, passes=2, eps=0.2, theta=0.9, kmax=3)
+        got = plp.to_json_dict(rows[k], pl[k])
+        assert {x: got[x] for x in FIELDS} == {x: want[x] for x in FIELDS}, k
+
+
+def test_block_capacity_escalation(plp, ctx, orc):
+    text = load_text("data/instances/BR1_050.txt")
+    eng = plp.Engine(ctx, ep_cap=16)
+    eng.load([plp.Instance.from_text(text)], [0] * 4, [1, 2, 3, 4], plp.params(eps=0.3, theta=0.9),
+             blocks=plp.block_params())
+    eng.run(2)
+    eng.sync()
+    rows, pl = eng.fetch()
+    assert eng.describe()["ep_cap"] >= max(r.ep_need for r in rows) > 16
+    for k in range(4):
+        want = orc.block_run(orc.parse(text)[0], seed=k + 1, passes=2, eps=0.3, theta=0.9)
+        got = plp.to_json_dict(rows[k], pl[k])
+        assert {x: got[x] for x in FIELDS} == {x: want[x] for x in FIELDS}, k
