@@ -13,7 +13,8 @@ Printed: one JSON line with the contract fields plus
   roofline      dominant kernel (decode_kernel) against the VALU issue roof
   cpu_baseline  the reference's g++ build (or the C oracle) on host cores
   extras        plain one-step-per-launch GPU form, the 256-instance batch,
-                the 100-type / 2048-decoder instance, Poisson serving
+                the 100-type / 2048-decoder instance, the block-building engine
+                (GPU and reference CPU), JSON emission, Poisson serving
 usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu] [--no-extras]
 """
 from __future__ import annotations
@@ -79,6 +80,31 @@ def cpu_baseline(text: str, threads: int, n: int, passes: int) -> dict | None:
             "sample": f"{n} decoders (seeds 1..{n}) x {passes} passes of the same 20-type "
                       f"instance, eps {EPS}, theta {THETA}; {out['seconds']:.2f} s wall on "
                       f"{threads} threads"}
+
+
+def block_cpu_baseline(text: str, threads: int, n: int = 1024, passes: int = 1200) -> dict | None:
+    """The reference block engine (blockproof/plp_block.cpp through
+    oracle/_ref/block_driver): n decoders x `passes` passes on host threads."""
+    drv = os.path.join(REPO, "oracle", "_ref", "block_driver")
+    if not os.path.exists(drv):
+        return None
+    inst_path = os.path.join(REPO, f"_bench_block_{os.getpid()}.txt")
+    with open(inst_path, "w") as f:
+        f.write(text)
+    try:
+        out = json.loads(subprocess.run(
+            [drv, "bench", inst_path, "--K", str(n), "--passes", str(passes), "--seed", "1",
+             "--eps", str(EPS), "--theta", str(THETA), "--threads", str(threads)],
+            check=True, capture_output=True, text=True, timeout=600).stdout)
+    except Exception as e:  # noqa: BLE001 -- reported, not fatal
+        log(f"block cpu baseline unavailable: {e}")
+        return None
+    finally:
+        os.remove(inst_path)
+    return {"value": round(out["packings_per_s"], 1), "unit": "packings/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"{n} decoders x {passes} passes, reference block options; "
+                      f"{out['seconds']:.2f} s wall on {threads} threads"}
 
 
 def serve(plp_amd, synth, seconds: float, rate: float, passes: int, max_inflight: int,
@@ -310,6 +336,25 @@ def main() -> int:
         extras["large_100types_2048_packings_per_s"] = round(3 * 2048 / (time.perf_counter() - t0), 1)
         extras["large_100types_2048_kernel_ms"] = round(e3.last_ms(), 3)
         e3.close()
+        # Block-building engine (blockproof/plp_block.cpp) on the same instance
+        # and seeds, reference block options, 4 passes per decoder.
+        eb = plp_amd.Engine(ctx, use_graph=True)
+        eb.load([inst], [0] * K_DEC, list(range(1, K_DEC + 1)), prm,
+                blocks=plp_amd.block_params())
+        eb.run(PASSES)
+        eb.sync()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            eb.reset()
+            eb.run(PASSES)
+            eb.sync()
+        extras["block_engine_packings_per_s"] = round(3 * K_DEC * PASSES /
+                                                      (time.perf_counter() - t0), 1)
+        extras["block_engine_kernel_ms"] = round(eb.last_ms(), 3)
+        eb.close()
+        if not a.no_cpu:
+            extras["block_engine_cpu_baseline"] = block_cpu_baseline(text, min(16, os.cpu_count()
+                                                                              or 1))
         # JSON wire format: one 4096-decoder request through `plp_gpu --serve`;
         # format_ms is the host time to write its 4096 reference-format rows.
         cli = os.path.join(REPO, "plp-gpu_amd", "build", "plp_gpu")

@@ -130,3 +130,30 @@ def test_block_capacity_escalation(plp, ctx, orc):
         want = orc.block_run(orc.parse(text)[0], seed=k + 1, passes=2, eps=0.3, theta=0.9)
         got = plp.to_json_dict(rows[k], pl[k])
         assert {x: got[x] for x in FIELDS} == {x: want[x] for x in FIELDS}, k
+
+
+BLOCK_CLI = __import__("os").path.join(__import__("conftest").REPO, "plp-gpu_amd", "build",
+                                       "plp_block")
+
+
+def test_block_cli_matches_reference_cli_runs():
+    """`plp_block ... --single-pass --json` prints the reference CLI's JSON
+    (time_s aside) for every fixture case the reference CLI ran."""
+    import subprocess
+    with open(golden_path("block_runs.json")) as f:
+        cases = [c for c in json.load(f) if c["reference_cli"] is not None][:12]
+    for c in cases:
+        a = c["args"]
+        args = [BLOCK_CLI, golden_path(c["instance"]), "--single-pass", "--json", "--seed",
+                str(a["seed"]), "--eps", repr(a["eps"]), "--pick", a["pick"], "--amax",
+                str(a["amax"]), "--bmax", str(a["bmax"]), "--kmax", str(a["kmax"]), "--topk",
+                str(a["topk"])] + (["--no-theta"] if a["theta"] is None else
+                                   ["--theta", repr(a["theta"])])
+        p = subprocess.run(args, capture_output=True, text=True, timeout=120)
+        assert p.returncode == 0, p.stderr
+        got = json.loads(p.stdout)
+        want = dict(c["reference_cli"])
+        got.pop("time_s")
+        want.pop("time_s")
+        assert got == want, c["case"]
+        assert p.stderr.startswith("fill_pct=")
