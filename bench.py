@@ -312,6 +312,7 @@ def main() -> int:
         e2 = plp_amd.Engine(ctx, use_graph=True)
         e2.load(insts, [i for i in range(256) for _ in range(16)],
                 [1 + s for _ in range(256) for s in range(16)], prm)
+        extras["batch_256x16_autotuned"], _ = e2.autotune(1)
         e2.run(1)
         e2.sync()
         t0 = time.perf_counter()
@@ -326,6 +327,7 @@ def main() -> int:
         e3 = plp_amd.Engine(ctx, use_graph=True)
         e3.load([plp_amd.Instance.from_text(synth.large_instance())], [0] * 2048,
                 list(range(1, 2049)), prm)
+        extras["large_100types_2048_autotuned"], _ = e3.autotune(1)
         e3.run(1)
         e3.sync()
         t0 = time.perf_counter()

@@ -200,7 +200,11 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                     nx2 = uni(NB.x2); ny2 = uni(NB.y2); nz2 = uni(NB.z2);
                 }
                 const int n_pairs = na_w * no;
-                const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)no - 1) / (uint64_t)no);
+                // e = p / no as umulhi(p, ceil(2^32 / no)): exact for p < 2^32 / no
+                // (no == 1 bypasses it)
+                const uint32_t magic = no == 2 ? 0x80000000u : no == 3 ? 0x55555556u
+                                     : no == 4 ? 0x40000000u : no == 5 ? 0x33333334u
+                                     : 0x2aaaaaabu;
                 const int n_chunks = (n_pairs + kWave - 1) / kWave;
                 int cnt_w = 0;
                 for (int c = wid; c < n_chunks; c += NW) {
