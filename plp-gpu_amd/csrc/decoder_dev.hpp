@@ -103,9 +103,28 @@ __device__ __forceinline__ bool pair_scan(int x, int y, int z, int w, int d, int
     return alive;
 }
 
-// C3 on top of C1 and C2 (plp_common.hpp:304-312), in the reference's FP64 form.
+// Division by an orientation count no in 1..6 without an integer divide:
+// p / no = umulhi(p, ceil(2^32 / no)), exact while p * (no - 2^32 mod no) < 2^32,
+// i.e. for every p below 2^32 / 6; no == 1 bypasses the product.
+__device__ __forceinline__ uint32_t orient_recip(int no) {
+    return no == 2 ? 0x80000000u : no == 3 ? 0x55555556u : no == 4 ? 0x40000000u
+         : no == 5 ? 0x33333334u : 0x2aaaaaabu;
+}
+__device__ __forceinline__ int div_orients(int p, int no, uint32_t recip) {
+    return no == 1 ? p : (int)__umulhi((uint32_t)p, recip);
+}
+
+// C3 on top of C1 and C2 (plp_common.hpp:304-312).  The reference rejects
+// when the FP64 quotient (double)s / (double)(w * d) < 0.5.  For integers
+// 0 <= s <= A = w * d with 1 <= A <= 2^53 (both exact in FP64) that verdict
+// equals 2 * s < A: when 2s < A the exact quotient is at most 0.5 - 1/(2A),
+// more than half an ulp (2^-55) below 0.5, so the correctly rounded quotient
+// stays below 0.5;
+// when 2s >= A it is at least 0.5 and rounds to at least 0.5.  Every box that
+// passes C2 has A <= L * W < 2^31 (the container range the engine accepts),
+// so the integer form gives the reference's verdict bit for bit.
 __device__ __forceinline__ bool pair_verdict(bool c12, int z, int w, int d, uint32_t s) {
-    return c12 && (z == 0 || !((double)s / (double)((int64_t)w * d) < 0.5));
+    return c12 && (z == 0 || 2ull * s >= (uint64_t)((uint32_t)w * (uint32_t)d));
 }
 
 // Pair-state cache entry: the support sum of a pair that passes C1 and C2,
@@ -237,6 +256,28 @@ static __device__ double fsr_score(const Box* __restrict__ box, int n, double mu
 // live list without being strictly inside and the corners that are neither
 // useful nor already listed.  n_act <= cap on entry keeps every write within
 // the cap + 8 slots of a list.  One wave; returns the larger new list size.
+// Corners j of the box [x,X2) x [y,Y2) x [z,Z2) (bit 0 of j -> x side, bit 1
+// -> y, bit 2 -> z) whose smallest later box [jx, jx+wm) x [jy, jy+dm) x
+// [jz, jz+hm) meets one of boxes[0..n_box): bit j of the result (wave-uniform).
+__device__ __forceinline__ uint32_t corner_kill(const Box* __restrict__ box, int n_box, int x, int y,
+                                                int z, int X2, int Y2, int Z2, int wm, int dm,
+                                                int hm) {
+    uint32_t kill = 0;
+    for (int b0 = 0; b0 < n_box; b0 += kWave) {
+        const int bi = b0 + lane_id();
+        Box B = sentinel_box();
+        if (bi < n_box) B = box[bi];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int jx = (j & 1) ? X2 : x, jy = (j & 2) ? Y2 : y, jz = (j & 4) ? Z2 : z;
+            const bool in = (jx < B.x2) & (B.x < jx + wm) & (jy < B.y2) & (B.y < jy + dm) &
+                            (jz < B.z2) & (B.z < jz + hm);
+            if (ballot(in)) kill |= 1u << j;
+        }
+    }
+    return kill;
+}
+
 static __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t* oidx, uint32_t* dead,
                             int& n_act, int& n_dead, const Box* box, int n_box, int x, int y, int z,
                             int X2, int Y2, int Z2, int L, int W, int H, int xb, int yb, int cap,
@@ -270,18 +311,7 @@ static __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t*
     }
     // Corners whose smallest box meets a placed box (the new one included)
     // are not useful.
-    for (int b0 = 0; b0 < n_box; b0 += kWave) {
-        const int bi = b0 + lane;
-        Box B = sentinel_box();
-        if (bi < n_box) B = box[bi];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int jx = (j & 1) ? X2 : x, jy = (j & 2) ? Y2 : y, jz = (j & 4) ? Z2 : z;
-            const bool in = (jx < B.x2) & (B.x < jx + wm) & (jy < B.y2) & (B.y < jy + dm) &
-                            (jz < B.z2) & (B.z < jz + hm);
-            if (ballot(in)) cdead |= 1u << j;
-        }
-    }
+    cdead |= corner_kill(box, n_box, x, y, z, X2, Y2, Z2, wm, dm, hm);
     const uint32_t add = ~cdead & ~exists & 0xffu;   // useful corners that join the list
 
     // Dead list: drop the strictly inside points in place (a point moves to
@@ -379,7 +409,7 @@ static __device__ void corner_states(const int* nb, uint32_t* cstate, const int*
                               const Box* box, int n_box, int L, int W, int H, Box* cbox) {
     const int lane = lane_id();
     const int n = 8 * no;
-    const int j = lane / no, o = lane - (lane / no) * no;
+    const int j = div_orients(lane, no, orient_recip(no)), o = lane - j * no;
     int x = 0, y = 0, z = 0, w = 1, d = 1, h = 1;
     if (lane < n) {
         x = (j & 1) ? nb[3] : nb[0];

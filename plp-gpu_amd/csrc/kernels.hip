@@ -200,17 +200,13 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                     nx2 = uni(NB.x2); ny2 = uni(NB.y2); nz2 = uni(NB.z2);
                 }
                 const int n_pairs = na_w * no;
-                // e = p / no as umulhi(p, ceil(2^32 / no)): exact for p < 2^32 / no
-                // (no == 1 bypasses it)
-                const uint32_t magic = no == 2 ? 0x80000000u : no == 3 ? 0x55555556u
-                                     : no == 4 ? 0x40000000u : no == 5 ? 0x33333334u
-                                     : 0x2aaaaaabu;
+                const uint32_t magic = orient_recip(no);
                 const int n_chunks = (n_pairs + kWave - 1) / kWave;
                 int cnt_w = 0;
                 for (int c = wid; c < n_chunks; c += NW) {
                     const int p = c * kWave + lane;
                     const bool valid = p < n_pairs;
-                    const int e = no == 1 ? p : (int)__umulhi((uint32_t)p, magic);
+                    const int e = div_orients(p, no, magic);
                     const int o = p - e * no;
                     int x = 0, y = 0, z = 0, w = 1, dd = 1, h = 1, oi = kNewPoint;
                     if (valid) {
@@ -298,7 +294,7 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                     const bool hit = ((m >> lane) & 1ull) && rank_below(m) == idx;
                     const int bit = __ffsll((unsigned long long)ballot(hit)) - 1;
                     const int p = c * kWave + bit;
-                    const int e = no == 1 ? p : (int)__umulhi((uint32_t)p, magic);
+                    const int e = div_orients(p, no, magic);
                     const int o = p - e * no;
                     key_xyz(uni(act[e]), xb, yb, x, y, z);
                     x = uni(x);
