@@ -42,11 +42,10 @@ __device__ __forceinline__ uint32_t xyz_key(int x, int y, int z, int xb, int yb)
 // [zlo, zhi) over that footprint (a box outside it can neither overlap nor
 // support any pair of the chunk), copies them into the wave's cbox[] window,
 // pads it with sentinels to a multiple of kBoxGroup and scans it in groups
-// whose LDS reads issue together.  The wave leaves as soon as every lane has
-// failed, and skips the support arithmetic of a box whose top is at no live
-// lane's level (both wave-uniform branches).  Areas fit 32 bits: L*W < 2^30
-// is a load-time precondition, and ox, oy < 2^21 so the 24-bit multiply is
-// exact.
+// whose LDS reads issue together, branch-free per box (overlap and support
+// both come from the three overlap extents).  The wave leaves as soon as
+// every lane has failed.  Areas fit 32 bits: L*W < 2^30 is a load-time
+// precondition, and ox, oy < 2^21 so the 24-bit multiply is exact.
 constexpr int kBoxGroup = 4;
 constexpr int kCullWindow = kWave + kBoxGroup;   // cbox[] entries per wave
 
@@ -85,15 +84,16 @@ __device__ __forceinline__ bool pair_scan(int x, int y, int z, int w, int d, int
             for (int u = 0; u < kBoxGroup; u++) B[u] = cbox[g + u];
 #pragma unroll
             for (int u = 0; u < kBoxGroup; u++) {
-                const bool ov = (x < B[u].x2) & (B[u].x < x2) & (y < B[u].y2) & (B[u].y < y2) &
-                                (z < B[u].z2) & (B[u].z < z2);
-                alive = alive & !ov;
-                if (ballot(alive & (B[u].z2 == z))) {
-                    const int ox = min(x2, B[u].x2) - max(x, B[u].x);
-                    const int oy = min(y2, B[u].y2) - max(y, B[u].y);
-                    const bool sup = (B[u].z2 == z) & (ox > 0) & (oy > 0);
-                    s += sup ? __umul24((uint32_t)ox, (uint32_t)oy) : 0u;
-                }
+                // overlap extents: the boxes meet strictly iff all three are
+                // positive; a box whose top is at z supports over ox * oy
+                // when the footprints meet
+                const int ox = min(x2, B[u].x2) - max(x, B[u].x);
+                const int oy = min(y2, B[u].y2) - max(y, B[u].y);
+                const int oz = min(z2, B[u].z2) - max(z, B[u].z);
+                const int oxy = min(ox, oy);
+                alive = alive & !(min(oxy, oz) > 0);
+                const bool sup = (B[u].z2 == z) & (oxy > 0);
+                s += __umul24((uint32_t)(sup ? ox : 0), (uint32_t)oy);
             }
             if (ballot(alive) == 0) break;
         }
@@ -119,10 +119,10 @@ __device__ __forceinline__ int div_orients(int p, int no, uint32_t recip) {
 // 0 <= s <= A = w * d with 1 <= A <= 2^53 (both exact in FP64) that verdict
 // equals 2 * s < A: when 2s < A the exact quotient is at most 0.5 - 1/(2A),
 // more than half an ulp (2^-55) below 0.5, so the correctly rounded quotient
-// stays below 0.5;
-// when 2s >= A it is at least 0.5 and rounds to at least 0.5.  Every box that
-// passes C2 has A <= L * W < 2^31 (the container range the engine accepts),
-// so the integer form gives the reference's verdict bit for bit.
+// stays below 0.5; when 2s >= A it is at least 0.5 and rounds to at least
+// 0.5.  Every box that passes C2 has A <= L * W < 2^31 (the container range
+// the engine accepts), so the integer form gives the reference's verdict bit
+// for bit.
 __device__ __forceinline__ bool pair_verdict(bool c12, int z, int w, int d, uint32_t s) {
     return c12 && (z == 0 || 2ull * s >= (uint64_t)((uint32_t)w * (uint32_t)d));
 }
