@@ -101,7 +101,7 @@ __device__ void bitonic_u32(uint32_t* key, int n2) {
 }  // namespace
 
 template <int NW>
-__global__ void __launch_bounds__(64 * NW) block_kernel(BlockArgs ba) {
+__global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
     const DecodeArgs& a = ba.d;
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ BlockShared sh;

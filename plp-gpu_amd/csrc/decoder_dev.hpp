@@ -148,17 +148,119 @@ __device__ __forceinline__ void cache_put(void* c, int i, uint32_t v, bool wide)
 }
 
 // ---------------------------------------------------------------- FSR ----
+__device__ __forceinline__ double read_lane_f64(double v, int src) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), src);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Contact area of box i's bottom face with the top face of box j (j's top at
+// i's bottom z > 0 and the footprints overlapping), else 0.
+__device__ __forceinline__ uint32_t contact_area(const Box& i, const Box& j) {
+    const int ox = min(i.x2, j.x2) - max(i.x, j.x);
+    const int oy = min(i.y2, j.y2) - max(i.y, j.y);
+    const bool c = (j.z2 == i.z) & (i.z > 0) & (ox > 0) & (oy > 0);
+    return c ? (uint32_t)ox * (uint32_t)oy : 0u;
+}
+
+// Box j precedes box i in the reference's visiting order (z desc, index asc).
+__device__ __forceinline__ int visits_before(const Box& j, int jdx, const Box& i, int idx) {
+    return ((j.z > i.z) | ((j.z == i.z) & (jdx < idx))) ? 1 : 0;
+}
+
+// fsr_score for n <= 2 * kWave boxes with everything in registers: lane i
+// holds boxes i and i + 64, their loads and contact totals.  The visiting
+// order comes from each box's rank (the number of boxes before it in
+// (z desc, index asc) order, scattered through ord[], n ints of LDS), and
+// the k loop fetches box k's data with lane reads.  Box j's load receives
+// additions only from the boxes whose bottom is at j's top, each through the
+// register of the lane that holds j, in the visiting order: the reference's
+// sequence of FP64 operations on every load[j].
+static __device__ double fsr_score_reg(const Box* __restrict__ box, int n, double mu, int* ord,
+                                       double* fsr_out) {
+    const int lane = lane_id();
+    const bool v0 = lane < n, v1 = lane + kWave < n;
+    const Box b0 = v0 ? box[lane] : sentinel_box();
+    const Box b1 = v1 ? box[lane + kWave] : sentinel_box();
+    // m = RHO * V / 1e6 (plp_common.hpp:360-362); load starts at m
+    auto mass = [](const Box& b) {
+        const int64_t v = (int64_t)(b.x2 - b.x) * (b.y2 - b.y) * (b.z2 - b.z);
+        return 50.0 * (double)v / 1.0e6;
+    };
+    const double m0 = v0 ? mass(b0) : 1.0, m1 = v1 ? mass(b1) : 1.0;
+    double l0 = m0, l1 = m1;
+    // contact totals (the boxes whose tops touch a box's bottom do not
+    // overlap one another, so a total is at most w * d < 2^30) and ranks
+    uint32_t t0 = 0, t1 = 0;
+    int r0 = 0, r1 = 0;
+    for (int j = 0; j < n; j++) {
+        const Box Bj = box[j];
+        t0 += contact_area(b0, Bj);
+        t1 += contact_area(b1, Bj);
+        r0 += visits_before(Bj, j, b0, lane);
+        r1 += visits_before(Bj, j, b1, lane + kWave);
+    }
+    if (v0) ord[r0] = lane;
+    if (v1) ord[r1] = lane + kWave;
+    wave_sync();
+    const int o0 = v0 ? ord[lane] : 0, o1 = v1 ? ord[lane + kWave] : 0;
+    // compute_fsr's k loop (plp_common.hpp:365-385)
+    for (int t = 0; t < n; t++) {
+        const int k = t < kWave ? __builtin_amdgcn_readlane(o0, t)
+                                : __builtin_amdgcn_readlane(o1, t - kWave);
+        const int kl = k & (kWave - 1);
+        const bool hi = k >= kWave;
+        const int bkz = hi ? __builtin_amdgcn_readlane(b1.z, kl) : __builtin_amdgcn_readlane(b0.z, kl);
+        if (bkz == 0) break;   // the rest are floor boxes: nothing supports them
+        const uint32_t total = hi ? (uint32_t)__builtin_amdgcn_readlane((int)t1, kl)
+                                  : (uint32_t)__builtin_amdgcn_readlane((int)t0, kl);
+        if (total == 0) continue;
+        const int bkx = hi ? __builtin_amdgcn_readlane(b1.x, kl) : __builtin_amdgcn_readlane(b0.x, kl);
+        const int bky = hi ? __builtin_amdgcn_readlane(b1.y, kl) : __builtin_amdgcn_readlane(b0.y, kl);
+        const int bkx2 = hi ? __builtin_amdgcn_readlane(b1.x2, kl) : __builtin_amdgcn_readlane(b0.x2, kl);
+        const int bky2 = hi ? __builtin_amdgcn_readlane(b1.y2, kl) : __builtin_amdgcn_readlane(b0.y2, kl);
+        const double lk = hi ? read_lane_f64(l1, kl) : read_lane_f64(l0, kl);
+        const double dt = (double)total;
+        // (double)a of the reference's 64-bit area a = ox * oy equals the FP64
+        // product of the two extents: both are exact (a < 2^42 < 2^53)
+        {
+            const int ox = min(bkx2, b0.x2) - max(bkx, b0.x);
+            const int oy = min(bky2, b0.y2) - max(bky, b0.y);
+            if ((b0.z2 == bkz) & (ox > 0) & (oy > 0))
+                l0 = l0 + lk * ((double)ox * (double)oy) / dt;
+        }
+        {
+            const int ox = min(bkx2, b1.x2) - max(bkx, b1.x);
+            const int oy = min(bky2, b1.y2) - max(bky, b1.y);
+            if ((b1.z2 == bkz) & (ox > 0) & (oy > 0))
+                l1 = l1 + lk * ((double)ox * (double)oy) / dt;
+        }
+    }
+    // FSR_i = ((mu / 0.3) * load_i) / m_i; score = #(FSR >= 1) / n
+    const double f0 = (mu / 0.3) * l0 / m0, f1 = (mu / 0.3) * l1 / m1;
+    if (fsr_out) {
+        if (v0) fsr_out[lane] = f0;
+        if (v1) fsr_out[lane + kWave] = f1;
+    }
+    const int c = __popcll(ballot(v0 && f0 >= 1.0)) + __popcll(ballot(v1 && f1 >= 1.0));
+    wave_sync();   // ord[] is free again
+    return (double)c / (double)n;
+}
+
 // compute_fsr + stability_score (plp_common.hpp:356-399) for the n boxes in
 // LDS, on one wave.  The reference adds load[k] * a / total into load[j]
-// while visiting k in (z desc, index asc) order; that order is produced by a
-// bitonic sort of 64-bit keys ((2^31-1-z) << 32 | index) in LDS, and the k
-// loop stays serial so every load[j] receives its additions in the
-// reference's order (bit-exact FP64).  Returns the score; per-box FSR goes to
-// fsr_out when non-null.
+// while visiting k in (z desc, index asc) order, and the k loop stays serial
+// so every load[j] receives its additions in the reference's order
+// (bit-exact FP64).  Up to 128 boxes the state stays in registers
+// (fsr_score_reg); beyond, loads and totals sit in LDS and the order comes
+// from a bitonic sort of 64-bit keys ((2^31-1-z) << 32 | index).  Returns
+// the score; per-box FSR goes to fsr_out when non-null.
 static __device__ double fsr_score(const Box* __restrict__ box, int n, double mu, double* load,
                             uint64_t* skey, uint32_t* tot, double* fsr_out) {
     const int lane = lane_id();
     if (n <= 0) return 0.0;
+    if (n <= 2 * kWave) return fsr_score_reg(box, n, mu, (int*)skey, fsr_out);
     const int n2 = next_pow2(n);
     for (int i = lane; i < n2; i += kWave)
         skey[i] = i < n ? (((uint64_t)(0x7fffffffu - (uint32_t)box[i].z) << 32) | (uint32_t)i)

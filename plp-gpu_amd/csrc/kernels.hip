@@ -59,8 +59,10 @@ struct StepShared {
 // Wave 0 alone runs the serial part of a step (RNG, pick, place, extreme-
 // point merge), FSR and the selection; the other waves wait at the block
 // barrier.
+// The bound asks for at most 128 VGPRs (four waves per SIMD, i.e. four
+// 4-wave decoders per CU).
 template <int NW>
-__global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
+__global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int s_ors[18 * NW];
     __shared__ StepShared sh;
@@ -343,6 +345,11 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
             // ---- pass end (wave 0): stability score and D4' selection
             if (wid == 0) {
                 const bool overflow = uni(sh.quit) != 0;
+                // the packing's score before the decoder state is read back
+                const bool scored = !overflow && uni(sst.err) == 0;
+                const int n = n_box;
+                double score = 0.0;
+                if (scored) score = fsr_score(box, n, a.mu, load, skey, stot, nullptr);
                 DecState st = sst;
                 if (overflow) st.err = 4;   // PLP_E_EPCAP
                 st.work_pair_box += work;
@@ -351,13 +358,11 @@ __global__ void __launch_bounds__(64 * NW) decode_kernel(DecodeArgs a) {
                 if (a.count_work) st.max_ep = max(st.max_ep, max_ep);
                 st.ep_need = max(st.ep_need, ep_need);
                 if (st.err == 0) {
-                    const int n = n_box;
                     int64_t vpart = 0;
                     for (int i = lane; i < n; i += kWave)
                         vpart += (int64_t)(box[i].x2 - box[i].x) * (box[i].y2 - box[i].y) *
                                  (box[i].z2 - box[i].z);
                     const int64_t vol = wave_sum_i64(vpart);
-                    const double score = fsr_score(box, n, a.mu, load, skey, stot, nullptr);
                     // D4' (plp_common.hpp:493-511), kept incrementally: the best
                     // of all packings and the best accepted one, strict > on
                     // (volume, n_boxes) so the earliest winner stays.
