@@ -155,6 +155,17 @@ __device__ __forceinline__ double read_lane_f64(double v, int src) {
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
+// The six coordinates of a box read from LDS (one 16-byte and one 8-byte
+// read; type and padding stay behind).
+__device__ __forceinline__ Box box_xyz(const Box& b) {
+    const int4 p = *(const int4*)&b.x;
+    const int2 q = *(const int2*)&b.y2;
+    Box r;
+    r.x = p.x; r.y = p.y; r.z = p.z; r.x2 = p.w; r.y2 = q.x; r.z2 = q.y;
+    r.type = 0; r.pad = 0;
+    return r;
+}
+
 // Contact area of box i's bottom face with the top face of box j (j's top at
 // i's bottom z > 0 and the footprints overlapping), else 0.
 __device__ __forceinline__ uint32_t contact_area(const Box& i, const Box& j) {
@@ -169,16 +180,17 @@ __device__ __forceinline__ int visits_before(const Box& j, int jdx, const Box& i
     return ((j.z > i.z) | ((j.z == i.z) & (jdx < idx))) ? 1 : 0;
 }
 
-// fsr_score for n <= 2 * kWave boxes with everything in registers: lane i
-// holds boxes i and i + 64, their loads and contact totals.  The visiting
-// order comes from each box's rank (the number of boxes before it in
-// (z desc, index asc) order, scattered through ord[], n ints of LDS), and
-// the k loop fetches box k's data with lane reads.  Box j's load receives
+// fsr_score for n <= 2 * kWave boxes with the loads in registers: lane i
+// holds boxes i and i + 64, their loads and contact totals (also written to
+// tot[]).  The visiting order comes from each box's rank (the number of
+// boxes before it in (z desc, index asc) order, scattered through ord[], n
+// ints of LDS); the k loop reads box k's coordinates and total from LDS and
+// its load from the lane that holds it.  Box j's load receives
 // additions only from the boxes whose bottom is at j's top, each through the
 // register of the lane that holds j, in the visiting order: the reference's
 // sequence of FP64 operations on every load[j].
 static __device__ double fsr_score_reg(const Box* __restrict__ box, int n, double mu, int* ord,
-                                       double* fsr_out) {
+                                       uint32_t* tot, double* fsr_out) {
     const int lane = lane_id();
     const bool v0 = lane < n, v1 = lane + kWave < n;
     const Box b0 = v0 ? box[lane] : sentinel_box();
@@ -194,33 +206,49 @@ static __device__ double fsr_score_reg(const Box* __restrict__ box, int n, doubl
     // overlap one another, so a total is at most w * d < 2^30) and ranks
     uint32_t t0 = 0, t1 = 0;
     int r0 = 0, r1 = 0;
-    for (int j = 0; j < n; j++) {
-        const Box Bj = box[j];
-        t0 += contact_area(b0, Bj);
-        t1 += contact_area(b1, Bj);
-        r0 += visits_before(Bj, j, b0, lane);
-        r1 += visits_before(Bj, j, b1, lane + kWave);
+    constexpr int kG = 2;   // boxes whose reads issue together
+    for (int j = 0; j < n; j += kG) {
+        Box B[kG];
+#pragma unroll
+        for (int u = 0; u < kG; u++) B[u] = j + u < n ? box_xyz(box[j + u]) : sentinel_box();
+#pragma unroll
+        for (int u = 0; u < kG; u++) {
+            t0 += contact_area(b0, B[u]);
+            t1 += contact_area(b1, B[u]);
+            r0 += visits_before(B[u], j + u, b0, lane);
+            r1 += visits_before(B[u], j + u, b1, lane + kWave);
+        }
     }
-    if (v0) ord[r0] = lane;
-    if (v1) ord[r1] = lane + kWave;
+    if (v0) {
+        ord[r0] = lane;
+        tot[lane] = t0;
+    }
+    if (v1) {
+        ord[r1] = lane + kWave;
+        tot[lane + kWave] = t1;
+    }
     wave_sync();
     const int o0 = v0 ? ord[lane] : 0, o1 = v1 ? ord[lane + kWave] : 0;
-    // compute_fsr's k loop (plp_common.hpp:365-385)
+    // compute_fsr's k loop (plp_common.hpp:365-385).  Box k's coordinates
+    // and total come from LDS one source ahead; its load from its lane.
+    int kn = __builtin_amdgcn_readlane(o0, 0);
+    Box Bn = box_xyz(box[kn]);
+    uint32_t totn = tot[kn];
     for (int t = 0; t < n; t++) {
-        const int k = t < kWave ? __builtin_amdgcn_readlane(o0, t)
-                                : __builtin_amdgcn_readlane(o1, t - kWave);
-        const int kl = k & (kWave - 1);
-        const bool hi = k >= kWave;
-        const int bkz = hi ? __builtin_amdgcn_readlane(b1.z, kl) : __builtin_amdgcn_readlane(b0.z, kl);
+        const int k = kn;
+        const int bkz = uni(Bn.z), bkx = uni(Bn.x), bky = uni(Bn.y), bkx2 = uni(Bn.x2),
+                  bky2 = uni(Bn.y2);
+        const uint32_t total = uni(totn);
+        if (t + 1 < n) {
+            kn = t + 1 < kWave ? __builtin_amdgcn_readlane(o0, t + 1)
+                               : __builtin_amdgcn_readlane(o1, t + 1 - kWave);
+            Bn = box_xyz(box[kn]);
+            totn = tot[kn];
+        }
         if (bkz == 0) break;   // the rest are floor boxes: nothing supports them
-        const uint32_t total = hi ? (uint32_t)__builtin_amdgcn_readlane((int)t1, kl)
-                                  : (uint32_t)__builtin_amdgcn_readlane((int)t0, kl);
         if (total == 0) continue;
-        const int bkx = hi ? __builtin_amdgcn_readlane(b1.x, kl) : __builtin_amdgcn_readlane(b0.x, kl);
-        const int bky = hi ? __builtin_amdgcn_readlane(b1.y, kl) : __builtin_amdgcn_readlane(b0.y, kl);
-        const int bkx2 = hi ? __builtin_amdgcn_readlane(b1.x2, kl) : __builtin_amdgcn_readlane(b0.x2, kl);
-        const int bky2 = hi ? __builtin_amdgcn_readlane(b1.y2, kl) : __builtin_amdgcn_readlane(b0.y2, kl);
-        const double lk = hi ? read_lane_f64(l1, kl) : read_lane_f64(l0, kl);
+        const int kl = k & (kWave - 1);
+        const double lk = k >= kWave ? read_lane_f64(l1, kl) : read_lane_f64(l0, kl);
         const double dt = (double)total;
         // (double)a of the reference's 64-bit area a = ox * oy equals the FP64
         // product of the two extents: both are exact (a < 2^42 < 2^53)
@@ -260,7 +288,7 @@ static __device__ double fsr_score(const Box* __restrict__ box, int n, double mu
                             uint64_t* skey, uint32_t* tot, double* fsr_out) {
     const int lane = lane_id();
     if (n <= 0) return 0.0;
-    if (n <= 2 * kWave) return fsr_score_reg(box, n, mu, (int*)skey, fsr_out);
+    if (n <= 2 * kWave) return fsr_score_reg(box, n, mu, (int*)skey, tot, fsr_out);
     const int n2 = next_pow2(n);
     for (int i = lane; i < n2; i += kWave)
         skey[i] = i < n ? (((uint64_t)(0x7fffffffu - (uint32_t)box[i].z) << 32) | (uint32_t)i)
