@@ -295,6 +295,27 @@ def test_engine_mixed_batch_vs_oracle(plp, ctx, orc, wpd):
                                                       "placements")}, d
 
 
+@pytest.mark.parametrize("wpd", [2, 4])
+def test_engine_many_types_vs_oracle(plp, ctx, orc, wpd):
+    """130 types of 2-3 copies (131 boxes in a packing, so FSR runs its LDS
+    form, beyond 128 boxes), 32 seeds, eps 0.3, 2 passes, against the oracle."""
+    from plp_amd import synth
+    text = synth.uniform_instance(2040, 130, 20, 90, 1.1)
+    inst = plp.Instance.from_text(text)
+    o_inst = orc.parse(text)[0]
+    K = 32
+    eng = plp.Engine(ctx, waves_per_decoder=wpd)
+    eng.load([inst], [0] * K, list(range(1, K + 1)), plp.params(eps=0.3, theta=0.9))
+    eng.run(2)
+    eng.sync()
+    rows, pl = eng.fetch()
+    assert max(r.n_boxes for r in rows) > 128
+    for k in range(K):
+        want = orc.run(o_inst, seed=k + 1, passes=2, eps=0.3, theta=0.9)
+        got = plp.to_json_dict(rows[k], pl[k])
+        assert got == {x: want[x] for x in got}, k
+
+
 def test_engine_autotune_keeps_results(plp, ctx, ref_runs):
     cases = [c for c in ref_runs if c["case"].startswith("multi_c2_s")]
     inst = inst_from(plp, cases[0]["instance"])
