@@ -247,6 +247,9 @@ int plp_server_poll(plp_server *srv, int64_t req_id, int32_t *done, plp_row_resu
                     int32_t *placements, int32_t placements_cap_rows, int32_t *max_items);
 /* Requests currently admitted. */
 int plp_server_inflight(plp_server *srv, int32_t *n);
+/* *done = 1 when the request's decode has finished (its results wait for
+ * plp_server_poll); does not release anything. */
+int plp_server_query(plp_server *srv, int64_t req_id, int32_t *done);
 
 #ifdef __cplusplus
 }

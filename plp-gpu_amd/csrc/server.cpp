@@ -160,6 +160,17 @@ extern "C" int plp_server_poll(plp_server* s, int64_t req_id, int32_t* done, plp
     return rc;
 }
 
+extern "C" int plp_server_query(plp_server* s, int64_t req_id, int32_t* done) {
+    if (!s || !done) return fail(PLP_E_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->req_slot.find(req_id);
+    if (it == s->req_slot.end()) return fail(PLP_E_ARG, "unknown request id");
+    const hipError_t q = hipEventQuery(s->slots[(size_t)it->second].done);
+    if (q != hipSuccess && q != hipErrorNotReady) return hip_fail(q, "request event");
+    *done = q == hipSuccess ? 1 : 0;
+    return PLP_OK;
+}
+
 extern "C" int plp_server_inflight(plp_server* s, int32_t* n) {
     if (!s || !n) return fail(PLP_E_ARG, "NULL argument");
     std::lock_guard<std::mutex> lock(s->mu);

@@ -31,7 +31,7 @@ EXPORTS = [
     "plp_engine_destroy", "plp_engine_load", "plp_engine_load_blocks", "plp_engine_reset", "plp_engine_run", "plp_engine_sync",
     "plp_engine_max_items", "plp_engine_fetch", "plp_engine_last_ms", "plp_engine_describe",
     "plp_engine_autotune", "plp_server_create", "plp_server_destroy", "plp_server_submit",
-    "plp_server_poll", "plp_server_inflight",
+    "plp_server_poll", "plp_server_inflight", "plp_server_query",
 ]
 
 
@@ -117,6 +117,7 @@ _sig = {
     "plp_server_poll": (C.c_int, [_P, C.c_int64, _I32P, C.POINTER(RowResult), _P, C.c_int32,
                                   _I32P]),
     "plp_server_inflight": (C.c_int, [_P, _I32P]),
+    "plp_server_query": (C.c_int, [_P, C.c_int64, _I32P]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(_lib, _name)
@@ -357,8 +358,13 @@ class Server:
         return rid.value
 
     def poll(self, rid: int, placements=False):
+        """None while the request runs; then (rows, placements or None), rows
+        being a ctypes array of n_dec RowResult."""
         n_dec, n_items = self._req[rid]
         done = C.c_int32()
+        _check(_lib.plp_server_query(self.h, rid, C.byref(done)))
+        if not done.value:
+            return None
         rows = (RowResult * n_dec)()
         mi = C.c_int32()
         cap = n_dec * max(n_items, 1)
@@ -370,7 +376,7 @@ class Server:
         if not done.value:
             return None
         del self._req[rid]
-        return list(rows), pl
+        return rows, pl
 
     def inflight(self) -> int:
         n = C.c_int32()
