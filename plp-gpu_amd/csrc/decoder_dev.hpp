@@ -122,9 +122,10 @@ __device__ __forceinline__ int div_orients(int p, int no, uint32_t recip) {
 // stays below 0.5; when 2s >= A it is at least 0.5 and rounds to at least
 // 0.5.  Every box that passes C2 has A <= L * W < 2^31 (the container range
 // the engine accepts), so the integer form gives the reference's verdict bit
-// for bit.
+// for bit.  Branch-free; when C2 fails the product (24-bit operands, exact
+// below 2^32) does not matter.
 __device__ __forceinline__ bool pair_verdict(bool c12, int z, int w, int d, uint32_t s) {
-    return c12 && (z == 0 || 2ull * s >= (uint64_t)((uint32_t)w * (uint32_t)d));
+    return c12 & ((z == 0) | (2ull * s >= (uint64_t)__umul24((uint32_t)w, (uint32_t)d)));
 }
 
 // Pair-state cache entry: the support sum of a pair that passes C1 and C2,
@@ -549,7 +550,7 @@ static __device__ void corner_states(const int* nb, uint32_t* cstate, const int*
         d = ors[3 * o + 1];
         h = ors[3 * o + 2];
     }
-    const bool c2 = lane < n && (x + w <= L) && (y + d <= W) && (z + h <= H);
+    const bool c2 = (lane < n) & (x + w <= L) & (y + d <= W) & (z + h <= H);
     bool c12 = false;
     uint32_t s = 0;
     if (ballot(c2) != 0) {

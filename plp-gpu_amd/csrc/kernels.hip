@@ -208,41 +208,45 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 for (int c = wid; c < n_chunks; c += NW) {
                     const int p = c * kWave + lane;
                     const bool valid = p < n_pairs;
-                    const int e = div_orients(p, no, magic);
-                    const int o = p - e * no;
-                    int x = 0, y = 0, z = 0, w = 1, dd = 1, h = 1, oi = kNewPoint;
-                    if (valid) {
-                        key_xyz(act[e], xb, yb, x, y, z);
-                        w = s_orw[3 * o];
-                        dd = s_orw[3 * o + 1];
-                        h = s_orw[3 * o + 2];
-                        if (inc) oi = oidx[e];
-                    }
-                    bool c12 = valid && (x + w <= L) && (y + dd <= W) && (z + h <= H);
+                    // lanes past the last pair read pair 0 (in bounds) and
+                    // are masked out of every result
+                    const int pv = valid ? p : 0;
+                    const int e = div_orients(pv, no, magic);
+                    const int o = pv - e * no;
+                    int x, y, z;
+                    key_xyz(act[e], xb, yb, x, y, z);
+                    const int w = s_orw[3 * o], dd = s_orw[3 * o + 1], h = s_orw[3 * o + 2];
+                    bool c12 = valid & (x + w <= L) & (y + dd <= W) & (z + h <= H);
                     uint32_t sup = 0;
                     bool scan = c12;
-                    if (inc && c12 && oi >= kCorner && oi != kNewPoint) {
-                        scan = false;
-                        const uint32_t cs = sh.cstate[(oi - kCorner) * no + o];
-                        c12 = cs != kBlocked;
-                        sup = cs;
-                    } else if (inc && c12 && oi != kNewPoint) {
-                        scan = false;
-                        const uint32_t old = cache_get(c_rd, oi * no + o, wide);
-                        const bool ov = (x < nx2) & (nx < x + w) & (y < ny2) & (ny < y + dd) &
-                                        (z < nz2) & (nz < z + h);
-                        if (old == kBlocked || ov) {
-                            c12 = false;
-                        } else {
-                            const int ox = min(x + w, nx2) - max(x, nx);
-                            const int oy = min(y + dd, ny2) - max(y, ny);
-                            sup = old + ((nz2 == z && ox > 0 && oy > 0)
-                                             ? __umul24((uint32_t)ox, (uint32_t)oy) : 0u);
+                    if (inc) {
+                        // a corner of the new box takes its precomputed state; a
+                        // point of the previous list its old state combined with
+                        // the new box; both read at in-bounds indices, selected
+                        const int oi = oidx[e];
+                        const bool corner = (oi >= kCorner) & (oi != kNewPoint);
+                        const bool cached = oi < kCorner;
+                        const uint32_t cs = sh.cstate[corner ? (oi - kCorner) * no + o : 0];
+                        const uint32_t old = cache_get(c_rd, cached ? oi * no + o : 0, wide);
+                        const int ox = min(x + w, nx2) - max(x, nx);
+                        const int oy = min(y + dd, ny2) - max(y, ny);
+                        const int oz = min(z + h, nz2) - max(z, nz);
+                        const int oxy = min(ox, oy);
+                        const bool ov = min(oxy, oz) > 0;
+                        const uint32_t add = __umul24((uint32_t)((nz2 == z) & (oxy > 0) ? ox : 0),
+                                                      (uint32_t)oy);
+                        const uint32_t st =
+                            corner ? cs : ((old == kBlocked) | ov ? kBlocked : old + add);
+                        const bool settled = c12 & (corner | cached);
+                        scan = c12 & !settled;
+                        if (settled) {
+                            c12 = st != kBlocked;
+                            sup = st;
                         }
                     }
                     // Settled lanes first, so that nothing of theirs stays
                     // live across the scan.
-                    uint64_t m = ballot(!scan && pair_verdict(c12, z, w, dd, sup));
+                    uint64_t m = ballot(!scan & pair_verdict(c12, z, w, dd, sup));
                     if (valid && !scan) cache_put(c_wr, p, c12 ? sup : kBlocked, wide);
                     if (ballot(scan) != 0) {
                         const int xlo = wave_min(scan ? x : 0x7fffffff);
@@ -255,7 +259,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                         const bool a2 = pair_scan(x, y, z, w, dd, h, scan, box, nb_w, xlo, xhi, ylo,
                                                   yhi, zlo, zhi, cbox, s2);
                         if (scan) cache_put(c_wr, p, a2 ? s2 : kBlocked, wide);
-                        m |= ballot(scan && pair_verdict(a2, z, w, dd, s2));
+                        m |= ballot(scan & pair_verdict(a2, z, w, dd, s2));
                     }
                     if (lane == 0) mask[c] = m;
                     cnt_w += __popcll(m);

@@ -391,15 +391,17 @@ int serve_main(int max_inflight, double mu_default) {
         }
         for (size_t i = 0; i < inflight.size();) {
             Request& r = inflight[i];
-            std::vector<plp_row_result> rows((size_t)r.n_dec);
-            std::vector<int32_t> pl((size_t)r.n_dec * (size_t)r.n_items * 7);
             int32_t done = 0, mi = 0;
-            rc = plp_server_poll(srv, r.rid, &done, rows.data(), pl.data(),
-                                 r.n_dec * r.n_items, &mi);
+            rc = plp_server_query(srv, r.rid, &done);   // result buffers only once done
             if (!done && rc == PLP_OK) {
                 i++;
                 continue;
             }
+            std::vector<plp_row_result> rows((size_t)r.n_dec);
+            std::vector<int32_t> pl((size_t)r.n_dec * (size_t)r.n_items * 7);
+            if (rc == PLP_OK)
+                rc = plp_server_poll(srv, r.rid, &done, rows.data(), pl.data(),
+                                     r.n_dec * r.n_items, &mi);
             progress = true;
             if (rc != PLP_OK) {
                 emit(error_line(r.id_raw, plp_last_error()));
