@@ -396,12 +396,12 @@ __device__ __forceinline__ uint32_t corner_kill(const Box* __restrict__ box, int
     uint32_t kill = 0;
     for (int b0 = 0; b0 < n_box; b0 += kWave) {
         const int bi = b0 + lane_id();
-        Box B = sentinel_box();
-        if (bi < n_box) B = box[bi];
+        const bool ok = bi < n_box;
+        const Box B = box[ok ? bi : b0];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int jx = (j & 1) ? X2 : x, jy = (j & 2) ? Y2 : y, jz = (j & 4) ? Z2 : z;
-            const bool in = (jx < B.x2) & (B.x < jx + wm) & (jy < B.y2) & (B.y < jy + dm) &
+            const bool in = ok & (jx < B.x2) & (B.x < jx + wm) & (jy < B.y2) & (B.y < jy + dm) &
                             (jz < B.z2) & (B.z < jz + hm);
             if (ballot(in)) kill |= 1u << j;
         }
@@ -432,7 +432,8 @@ static __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t*
     uint32_t exists = 0;
     for (int i0 = 0; i0 < n_act; i0 += kWave) {
         const int i = i0 + lane;
-        const uint32_t v = i < n_act ? src[i] : 0xffffffffu;
+        const uint32_t sv = src[i < n_act ? i : i0];
+        const uint32_t v = i < n_act ? sv : 0xffffffffu;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int c = __popcll(ballot(v < ck[j]));
@@ -484,7 +485,7 @@ static __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t*
                                  (pz < Z2) & (z < pz + hm)));
         int shift = 0;
 #pragma unroll
-        for (int j = 0; j < 8; j++) shift += (((add >> j) & 1u) && ck[j] < v) ? 1 : 0;
+        for (int j = 0; j < 8; j++) shift += (int)((add >> j) & 1u) & (int)(ck[j] < v);
         const bool keep = ok & !gone;
         const uint64_t km = ballot(keep);
         if (keep) {
@@ -541,15 +542,12 @@ static __device__ void corner_states(const int* nb, uint32_t* cstate, const int*
     const int lane = lane_id();
     const int n = 8 * no;
     const int j = div_orients(lane, no, orient_recip(no)), o = lane - j * no;
-    int x = 0, y = 0, z = 0, w = 1, d = 1, h = 1;
-    if (lane < n) {
-        x = (j & 1) ? nb[3] : nb[0];
-        y = (j & 2) ? nb[4] : nb[1];
-        z = (j & 4) ? nb[5] : nb[2];
-        w = ors[3 * o];
-        d = ors[3 * o + 1];
-        h = ors[3 * o + 2];
-    }
+    // lanes n..63 compute a corner too (j & 7, o < no: in bounds) and are
+    // masked out by c2
+    const int x = (j & 1) ? nb[3] : nb[0];
+    const int y = (j & 2) ? nb[4] : nb[1];
+    const int z = (j & 4) ? nb[5] : nb[2];
+    const int w = ors[3 * o], d = ors[3 * o + 1], h = ors[3 * o + 2];
     const bool c2 = (lane < n) & (x + w <= L) & (y + d <= W) & (z + h <= H);
     bool c12 = false;
     uint32_t s = 0;
