@@ -537,8 +537,9 @@ static __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t*
 // The pair states (cache entries) of the 8 corners of the box nb just
 // placed, for the orientations in ors (no of them), against all n_box boxes:
 // lane j * no + o.  One wave.
-static __device__ void corner_states(const int* nb, uint32_t* cstate, const int* ors, int no,
-                              const Box* box, int n_box, int L, int W, int H, Box* cbox) {
+static __device__ inline void corner_states(const int* nb, uint32_t* cstate, const int* ors,
+                                     int no, const Box* box, int n_box, int L, int W, int H,
+                                     Box* cbox) {
     const int lane = lane_id();
     const int n = 8 * no;
     const int j = div_orients(lane, no, orient_recip(no)), o = lane - j * no;
