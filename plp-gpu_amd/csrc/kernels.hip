@@ -233,10 +233,10 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                         const int oz = min(z + h, nz2) - max(z, nz);
                         const int oxy = min(ox, oy);
                         const bool ov = min(oxy, oz) > 0;
-                        const uint32_t add = __umul24((uint32_t)((nz2 == z) & (oxy > 0) ? ox : 0),
+                        const uint32_t add = __umul24((uint32_t)(((nz2 == z) & (oxy > 0)) ? ox : 0),
                                                       (uint32_t)oy);
                         const uint32_t st =
-                            corner ? cs : ((old == kBlocked) | ov ? kBlocked : old + add);
+                            corner ? cs : (((old == kBlocked) | ov) ? kBlocked : old + add);
                         const bool settled = c12 & (corner | cached);
                         scan = c12 & !settled;
                         if (settled) {
