@@ -23,8 +23,7 @@ PlainRunner::PlainRunner(plp_ctx* ctx, const plp::Instance& inst, uint64_t seed0
     seq_ = plp::item_sequence(inst_, p.order == 0 ? "volume" : "input");
     for (const plp::Item& it : inst_.items) orients_.push_back(plp::orientations(it));
     use_rng_ = !(p.first_fit || p.eps == 0.0);
-    for (int k = 0; k < K; k++) rows_.push_back(Row{plp::PyRng(seed0 + (uint64_t)k)});
-    for (int k = 0; k < K; k++) rows_[(size_t)k].seed = seed0 + (uint64_t)k;
+    for (int k = 0; k < K; k++) rows_.emplace_back(seed0 + (uint64_t)k);
 }
 
 int PlainRunner::run_pass() {

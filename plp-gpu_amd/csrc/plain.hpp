@@ -27,6 +27,7 @@ private:
         std::vector<std::array<int32_t, 7>> rows;
     };
     struct Row {
+        explicit Row(uint64_t s) : rng(s), seed(s) {}
         plp::PyRng rng;
         uint64_t seed = 0;
         std::vector<std::array<int32_t, 3>> cand;
