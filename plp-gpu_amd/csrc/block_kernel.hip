@@ -266,9 +266,10 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
                     int cnt_w = 0;
                     for (int c = wid; c < n_chunks; c += NW) {
                         const int e = c * kWave + lane;
-                        int x = 0, y = 0, z = 0;
-                        if (e < na_w) key_xyz(act[e], xb, yb, x, y, z);
-                        const bool c2 = e < na_w && x + bw <= L && y + bd <= W && z + bh <= H;
+                        const bool ok = e < na_w;
+                        int x, y, z;
+                        key_xyz(act[ok ? e : 0], xb, yb, x, y, z);   // in bounds; ok lanes count
+                        const bool c2 = ok & (x + bw <= L) & (y + bd <= W) & (z + bh <= H);
                         uint64_t m = 0;
                         if (ballot(c2) != 0) {
                             const int xlo = wave_min(c2 ? x : 0x7fffffff);

@@ -368,25 +368,6 @@ static __device__ double fsr_score(const Box* __restrict__ box, int n, double mu
     return (double)c / (double)n;
 }
 
-// Extreme-point update after placing the box [x, X2) x [y, Y2) x [z, Z2)
-// (plp_common.hpp:439-453): points strictly inside the box leave the list,
-// the box's 8 corners join it unless already present, and the list stays
-// sorted and unique.  The live list keeps only points where a box of the
-// later steps' smallest dimensions (wm, dm, hm, componentwise over every
-// orientation) fits inside the walls without overlapping a placed box: any
-// later box contains that one, so elsewhere nothing can ever be placed.  On
-// the live list src -> dst: the points whose smallest box meets the new box
-// or a wall leave (the strictly inside ones among them leave the reference's
-// list too), and the useful corners not already in src are inserted; oidx[]
-// records where each dst entry was in src (kCorner + j for inserted corner
-// j).  Usefulness only shrinks as boxes are added and the later steps'
-// smallest box grows, so a useful corner never equals a point that left, and
-// a corner equal to a point still listed is not inserted.  With count_work
-// the other points of the reference's list are kept (unsorted) in dead[]: it
-// drops its strictly inside points and receives the points that leave the
-// live list without being strictly inside and the corners that are neither
-// useful nor already listed.  n_act <= cap on entry keeps every write within
-// the cap + 8 slots of a list.  One wave; returns the larger new list size.
 // Corners j of the box [x,X2) x [y,Y2) x [z,Z2) (bit 0 of j -> x side, bit 1
 // -> y, bit 2 -> z) whose smallest later box [jx, jx+wm) x [jy, jy+dm) x
 // [jz, jz+hm) meets one of boxes[0..n_box): bit j of the result (wave-uniform).
@@ -409,12 +390,45 @@ __device__ __forceinline__ uint32_t corner_kill(const Box* __restrict__ box, int
     return kill;
 }
 
+// Corners whose key is in src[0..n): bit j of the result (wave-uniform).
+__device__ __forceinline__ uint32_t corner_present(const uint32_t* src, int n, const uint32_t ck[8]) {
+    uint32_t present = 0;
+    for (int i0 = 0; i0 < n; i0 += kWave) {
+        const int i = i0 + lane_id();
+        const uint32_t v = src[i < n ? i : i0];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (ballot((i < n) & (v == ck[j]))) present |= 1u << j;
+    }
+    return present;
+}
+
+// Extreme-point update after placing the box [x, X2) x [y, Y2) x [z, Z2)
+// (plp_common.hpp:439-453): points strictly inside the box leave the list,
+// the box's 8 corners join it unless already present, and the list stays
+// sorted and unique.  The live list keeps only points where a box of the
+// later steps' smallest dimensions (wm, dm, hm, componentwise over every
+// orientation) fits inside the walls without overlapping a placed box: any
+// later box contains that one, so elsewhere nothing can ever be placed.  On
+// the live list src -> dst: the points whose smallest box meets the new box
+// or a wall leave (the strictly inside ones among them leave the reference's
+// list too), and the useful corners are inserted, a point of src equal to
+// one giving way to it; oidx[] records where each dst entry was in src
+// (kCorner + j for inserted corner j).  Usefulness only shrinks as boxes are
+// added and the later steps' smallest box grows, so a useful corner never
+// equals a point that left, and the list stays a set.  With count_work
+// the other points of the reference's list are kept (unsorted) in dead[]: it
+// drops its strictly inside points and receives the points that leave the
+// live list without being strictly inside and the corners that are neither
+// useful nor already listed.  n_act <= cap on entry keeps every write within
+// the cap + 8 slots of a list.  One wave; returns the larger new list size.
 static __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t* oidx, uint32_t* dead,
                             int& n_act, int& n_dead, const Box* box, int n_box, int x, int y, int z,
                             int X2, int Y2, int Z2, int L, int W, int H, int xb, int yb, int cap,
                             bool count_work, int wm, int dm, int hm) {
     const int lane = lane_id();
-    // Corner j: bit 0 -> x side, bit 1 -> y, bit 2 -> z.  Lane j < 8 holds
+    // Corner j: bit 0 -> x side, bit 1 -> y, bit 2 -> z; with the key's field
+    // order (z, then y, then x) the keys increase with j.  Lane j < 8 holds
     // corner j's key in myk.
     uint32_t ck[8];
     uint32_t cdead = 0;
@@ -426,31 +440,20 @@ static __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t*
         if (lane == j) myk = ck[j];
         if ((jx + wm > L) | (jy + dm > W) | (jz + hm > H)) cdead |= 1u << j;
     }
-    // Each corner's lower bound in src (lane j < 8 keeps corner j's in lo)
-    // and whether src holds it.
-    int lo = 0;
-    uint32_t exists = 0;
-    for (int i0 = 0; i0 < n_act; i0 += kWave) {
-        const int i = i0 + lane;
-        const uint32_t sv = src[i < n_act ? i : i0];
-        const uint32_t v = i < n_act ? sv : 0xffffffffu;
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int c = __popcll(ballot(v < ck[j]));
-            if (lane == j) lo += c;
-            if (ballot(v == ck[j])) exists |= 1u << j;
-        }
-    }
     // Corners whose smallest box meets a placed box (the new one included)
-    // are not useful.
+    // are not useful; the useful ones are inserted, and a listed point equal
+    // to one gives way to it (same point: the list as a set and its order
+    // are the reference's; the point's pair states are then the corner's,
+    // computed against every box, instead of its old ones plus the new box).
     cdead |= corner_kill(box, n_box, x, y, z, X2, Y2, Z2, wm, dm, hm);
-    const uint32_t add = ~cdead & ~exists & 0xffu;   // useful corners that join the list
+    const uint32_t ins = ~cdead & 0xffu;
 
     // Dead list: drop the strictly inside points in place (a point moves to
     // an index no larger than its own, below the next 64-point chunk).
-    uint32_t eq_dead = 0;
+    uint32_t eq_dead = 0, present = 0;
     int nd = n_dead;
     if (count_work) {
+        present = corner_present(src, n_act, ck);
         int db = 0;
         for (int i0 = 0; i0 < n_dead; i0 += kWave) {
             const int i = i0 + lane;
@@ -472,8 +475,8 @@ static __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t*
     }
 
     // Live list: survivors go to dst after the inserted corners below them;
-    // lane j < 8 counts the points below corner j that leave (rb).
-    int base = 0, rb = 0;
+    // lane j < 8 counts the survivors below corner j (klt).
+    int base = 0, klt = 0;
     for (int i0 = 0; i0 < n_act; i0 += kWave) {
         const int i = i0 + lane;
         const bool ok = i < n_act;
@@ -483,10 +486,14 @@ static __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t*
         const bool gone = ok & ((px + wm > L) | (py + dm > W) | (pz + hm > H) |
                                 ((px < X2) & (x < px + wm) & (py < Y2) & (y < py + dm) &
                                  (pz < Z2) & (z < pz + hm)));
-        int shift = 0;
+        int shift = 0, eq = 0;
 #pragma unroll
-        for (int j = 0; j < 8; j++) shift += (int)((add >> j) & 1u) & (int)(ck[j] < v);
-        const bool keep = ok & !gone;
+        for (int j = 0; j < 8; j++) {
+            const int on = (int)((ins >> j) & 1u);
+            shift += on & (int)(ck[j] < v);
+            eq |= on & (int)(ck[j] == v);
+        }
+        const bool keep = ok & !gone & (eq == 0);
         const uint64_t km = ballot(keep);
         if (keep) {
             const int pos = base + rank_below(km) + shift;
@@ -494,34 +501,28 @@ static __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t*
             oidx[pos] = (uint16_t)i;
         }
         base += __popcll(km);
-        const uint64_t gm = ballot(gone);
-        if (gm) {
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int c = __popcll(gm & ballot(v < ck[j]));
-                if (lane == j) rb += c;
-            }
-            if (count_work) {
-                const bool strict = (x < px) & (px < X2) & (y < py) & (py < Y2) & (z < pz) &
-                                    (pz < Z2);
-                const bool mv = gone & !strict;
-                const uint64_t mm = ballot(mv);
-                const int pos = nd + rank_below(mm);
-                if (mv && pos < cap + 8) dead[pos] = v;
-                nd += __popcll(mm);
-            }
+        for (int j = 0; j < 8; j++) {
+            const int c = __popcll(km & ballot(v < ck[j]));
+            if (lane == j) klt += c;
+        }
+        if (count_work) {
+            const bool strict = (x < px) & (px < X2) & (y < py) & (py < Y2) & (z < pz) & (pz < Z2);
+            const bool mv = gone & !strict;
+            const uint64_t mm = ballot(mv);
+            const int pos = nd + rank_below(mm);
+            if (mv && pos < cap + 8) dead[pos] = v;
+            nd += __popcll(mm);
         }
     }
     base = uni(base);
-    if (lane < 8 && ((add >> lane) & 1u)) {
-        int r = 0;
-#pragma unroll
-        for (int j = 0; j < 8; j++) r += (((add >> j) & 1u) && ck[j] < myk) ? 1 : 0;
-        dst[lo - rb + r] = myk;
-        oidx[lo - rb + r] = (uint16_t)(kCorner + lane);
+    if (lane < 8 && ((ins >> lane) & 1u)) {
+        const int pos = klt + __popc(ins & ((1u << lane) - 1u));
+        dst[pos] = myk;
+        oidx[pos] = (uint16_t)(kCorner + lane);
     }
     if (count_work) {
-        const uint32_t dadd = cdead & ~exists & ~eq_dead;
+        const uint32_t dadd = cdead & ~present & ~eq_dead;
         if (lane < 8 && ((dadd >> lane) & 1u)) {
             const int pos = nd + __popc(dadd & ((1u << lane) - 1u));
             if (pos < cap + 8) dead[pos] = myk;
@@ -529,7 +530,7 @@ static __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t*
         nd += __popc(dadd);
     }
     wave_sync();
-    n_act = base + __popc(add);
+    n_act = base + __popc(ins);
     n_dead = count_work ? uni(nd) : 0;
     return max(n_act, n_dead);
 }
