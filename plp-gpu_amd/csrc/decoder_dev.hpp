@@ -374,20 +374,22 @@ static __device__ double fsr_score(const Box* __restrict__ box, int n, double mu
 __device__ __forceinline__ uint32_t corner_kill(const Box* __restrict__ box, int n_box, int x, int y,
                                                 int z, int X2, int Y2, int Z2, int wm, int dm,
                                                 int hm) {
+    // per lane: the corners its boxes kill, as the AND of per-axis masks
+    // (x side 0: corners 0x55, side 1: 0xaa; y: 0x33 / 0xcc; z: 0x0f / 0xf0)
     uint32_t kill = 0;
     for (int b0 = 0; b0 < n_box; b0 += kWave) {
         const int bi = b0 + lane_id();
         const bool ok = bi < n_box;
         const Box B = box[ok ? bi : b0];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int jx = (j & 1) ? X2 : x, jy = (j & 2) ? Y2 : y, jz = (j & 4) ? Z2 : z;
-            const bool in = ok & (jx < B.x2) & (B.x < jx + wm) & (jy < B.y2) & (B.y < jy + dm) &
-                            (jz < B.z2) & (B.z < jz + hm);
-            if (ballot(in)) kill |= 1u << j;
-        }
+        const uint32_t mx = (((x < B.x2) & (B.x < x + wm)) ? 0x55u : 0u) |
+                            (((X2 < B.x2) & (B.x < X2 + wm)) ? 0xaau : 0u);
+        const uint32_t my = (((y < B.y2) & (B.y < y + dm)) ? 0x33u : 0u) |
+                            (((Y2 < B.y2) & (B.y < Y2 + dm)) ? 0xccu : 0u);
+        const uint32_t mz = (((z < B.z2) & (B.z < z + hm)) ? 0x0fu : 0u) |
+                            (((Z2 < B.z2) & (B.z < Z2 + hm)) ? 0xf0u : 0u);
+        kill |= ok ? (mx & my & mz) : 0u;
     }
-    return kill;
+    return wave_or(kill);
 }
 
 // Corners whose key is in src[0..n): bit j of the result (wave-uniform).

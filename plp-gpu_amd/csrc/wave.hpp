@@ -85,4 +85,17 @@ __device__ __forceinline__ int wave_reduce_dpp(int v) {
 __device__ __forceinline__ int wave_min(int v) { return wave_reduce_dpp<false>(v); }
 __device__ __forceinline__ int wave_max(int v) { return wave_reduce_dpp<true>(v); }
 
+// Wave-wide bitwise OR, the same DPP pattern (v | v = v for rows a step
+// does not write).
+__device__ __forceinline__ uint32_t wave_or(uint32_t u) {
+    int v = (int)u;
+    v |= __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false);
+    v |= __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false);
+    v |= __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false);
+    v |= __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false);
+    v |= __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false);
+    v |= __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_readlane(v, 63);
+}
+
 }  // namespace plpk
