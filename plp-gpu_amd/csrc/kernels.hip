@@ -169,8 +169,29 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     n_steps_eval++;
                 }
                 // Same type, no placement since its last copy found nothing:
-                // the verdicts repeat, nothing is drawn.
-                if (ti == fail_ti) continue;
+                // the verdicts repeat, nothing is drawn -- for every further
+                // copy of the type that follows, skipped together.
+                if (ti == fail_ti) {
+                    int run = 1;
+                    for (;;) {
+                        const int q = s + run + lane;
+                        const bool same = (q < I.n_steps) && (s_step[q < I.n_steps ? q : s] == sv);
+                        const uint64_t other = ballot(!same);
+                        if (other) {
+                            run += __ffsll((unsigned long long)other) - 1;
+                            break;
+                        }
+                        run += kWave;
+                    }
+                    if (wid == 0) {
+                        if (a.count_work)
+                            work += (uint64_t)(run - 1) * (uint64_t)(n_act + n_dead) *
+                                    (uint64_t)no * (uint64_t)n_box;
+                        n_steps_eval += (uint32_t)(run - 1);
+                    }
+                    s += run - 1;
+                    continue;
+                }
                 int wm = 0, dm = 0, hm = 0;
                 if (wid == 0) {
                     live += (uint64_t)n_act * (uint64_t)no * (uint64_t)n_box;

@@ -153,8 +153,16 @@ __device__ __forceinline__ uint32_t mt_next(uint32_t* mt, int& mti) {
 }
 
 __device__ __forceinline__ double mt_random(uint32_t* mt, int& mti) {
-    uint32_t a = mt_next(mt, mti) >> 5;
-    uint32_t b = mt_next(mt, mti) >> 6;
+    uint32_t a, b;
+    if (mti + 2 <= 624) {   // both words without a twist: read them together
+        const uint32_t ya = mt[mti], yb = mt[mti + 1];
+        mti += 2;
+        a = mt_temper(uni(ya)) >> 5;
+        b = mt_temper(uni(yb)) >> 6;
+    } else {
+        a = mt_next(mt, mti) >> 5;
+        b = mt_next(mt, mti) >> 6;
+    }
     return ((double)a * 67108864.0 + (double)b) * (1.0 / 9007199254740992.0);
 }
 
