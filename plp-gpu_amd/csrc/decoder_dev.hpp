@@ -137,9 +137,12 @@ constexpr uint16_t kCorner = 0xfff0u;     // kCorner + j: corner j of the last b
 
 // Entries are 32-bit, or 16-bit with the support sum clamped to 65534 and
 // 0xffff for kBlocked (exact for C3 when every w*d <= kNarrowMaxArea).
+// One 32-bit read for either width (a 16-bit entry comes with its aligned
+// neighbour, which lies inside the cache or the region after it).
 __device__ __forceinline__ uint32_t cache_get(const void* c, int i, bool wide) {
-    if (wide) return ((const uint32_t*)c)[i];
-    const uint32_t v = ((const uint16_t*)c)[i];
+    const uint32_t w = *(const uint32_t*)((const unsigned char*)c + (wide ? 4 * i : 4 * (i >> 1)));
+    if (wide) return w;
+    const uint32_t v = (w >> (16 * (i & 1))) & 0xffffu;
     return v == 0xffffu ? kBlocked : v;
 }
 

@@ -237,6 +237,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     int x, y, z;
                     key_xyz(act[e], xb, yb, x, y, z);
                     const int w = s_orw[3 * o], dd = s_orw[3 * o + 1], h = s_orw[3 * o + 2];
+                    const int oi = oidx[e];   // read with the point (used by same-type steps)
                     bool c12 = valid & (x + w <= L) & (y + dd <= W) & (z + h <= H);
                     uint32_t sup = 0;
                     bool scan = c12;
@@ -244,7 +245,6 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                         // a corner of the new box takes its precomputed state; a
                         // point of the previous list its old state combined with
                         // the new box; both read at in-bounds indices, selected
-                        const int oi = oidx[e];
                         const bool corner = (oi >= kCorner) & (oi != kNewPoint);
                         const bool cached = oi < kCorner;
                         const uint32_t cs = sh.cstate[corner ? (oi - kCorner) * no + o : 0];
