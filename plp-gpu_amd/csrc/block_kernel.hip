@@ -431,14 +431,20 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
                         n += cnt_b;
                     }
                     wave_sync();
-                    const double score = fsr_score(items, n, a.mu, load, skey, stot, nullptr);
-                    if (st.n_all < 0 || vol > st.v_all || (vol == st.v_all && n > st.n_all)) {
+                    // D4' (plp_block.cpp:228-275): a packing that beats neither
+                    // best cannot be selected, so its score is never needed
+                    const bool better_all =
+                        st.n_all < 0 || vol > st.v_all || (vol == st.v_all && n > st.n_all);
+                    const bool better_acc =
+                        st.n_acc < 0 || vol > st.v_acc || (vol == st.v_acc && n > st.n_acc);
+                    const double score = better_all || better_acc
+                                             ? fsr_score(items, n, a.mu, load, skey, stot, nullptr)
+                                             : 0.0;
+                    if (better_all) {
                         st.n_all = n; st.v_all = vol; st.s_all = score;
                         write_rows(out_all, items, n);
                     }
-                    const bool accepted = !a.use_theta || score >= a.theta;
-                    if (accepted &&
-                        (st.n_acc < 0 || vol > st.v_acc || (vol == st.v_acc && n > st.n_acc))) {
+                    if (better_acc && (!a.use_theta || score >= a.theta)) {
                         st.n_acc = n; st.v_acc = vol; st.s_acc = score;
                         write_rows(out_acc, items, n);
                     }

@@ -370,11 +370,24 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
             // ---- pass end (wave 0): stability score and D4' selection
             if (wid == 0) {
                 const bool overflow = uni(sh.quit) != 0;
-                // the packing's score before the decoder state is read back
                 const bool scored = !overflow && uni(sst.err) == 0;
                 const int n = n_box;
+                int64_t vpart = 0;
+                for (int i = lane; i < n; i += kWave)
+                    vpart += (int64_t)(box[i].x2 - box[i].x) * (box[i].y2 - box[i].y) *
+                             (box[i].z2 - box[i].z);
+                const int64_t vol = wave_sum_i64(vpart);
+                // D4' (plp_common.hpp:493-511), kept incrementally: the best of
+                // all packings and the best accepted one, strict > on (volume,
+                // n_boxes) so the earliest winner stays.  A packing that beats
+                // neither cannot be selected, so its score is never needed.
+                const bool better_all = uni(sst.n_all) < 0 || vol > uni(sst.v_all) ||
+                                        (vol == uni(sst.v_all) && n > uni(sst.n_all));
+                const bool better_acc = uni(sst.n_acc) < 0 || vol > uni(sst.v_acc) ||
+                                        (vol == uni(sst.v_acc) && n > uni(sst.n_acc));
                 double score = 0.0;
-                if (scored) score = fsr_score(box, n, a.mu, load, skey, stot, nullptr);
+                if (scored && (better_all || better_acc))
+                    score = fsr_score(box, n, a.mu, load, skey, stot, nullptr);
                 DecState st = sst;
                 if (overflow) st.err = 4;   // PLP_E_EPCAP
                 st.work_pair_box += work;
@@ -383,21 +396,11 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 if (a.count_work) st.max_ep = max(st.max_ep, max_ep);
                 st.ep_need = max(st.ep_need, ep_need);
                 if (st.err == 0) {
-                    int64_t vpart = 0;
-                    for (int i = lane; i < n; i += kWave)
-                        vpart += (int64_t)(box[i].x2 - box[i].x) * (box[i].y2 - box[i].y) *
-                                 (box[i].z2 - box[i].z);
-                    const int64_t vol = wave_sum_i64(vpart);
-                    // D4' (plp_common.hpp:493-511), kept incrementally: the best
-                    // of all packings and the best accepted one, strict > on
-                    // (volume, n_boxes) so the earliest winner stays.
-                    if (st.n_all < 0 || vol > st.v_all || (vol == st.v_all && n > st.n_all)) {
+                    if (better_all) {
                         st.n_all = n; st.v_all = vol; st.s_all = score;
                         write_rows(out_all, box, n);
                     }
-                    const bool accepted = !a.use_theta || score >= a.theta;
-                    if (accepted &&
-                        (st.n_acc < 0 || vol > st.v_acc || (vol == st.v_acc && n > st.n_acc))) {
+                    if (better_acc && (!a.use_theta || score >= a.theta)) {
                         st.n_acc = n; st.v_acc = vol; st.s_acc = score;
                         write_rows(out_acc, box, n);
                     }
