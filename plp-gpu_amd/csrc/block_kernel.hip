@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
             wave_sync();
             if (uni(sh.quit) == 0 && a.use_rng) {
                 if (a.reset) {
-                    mt_seed_lane0(mt, a.dec_seed[d]);
+                    mt_seed_wave(mt, a.dec_seed[d]);
                 } else {
                     for (int i = lane; i < 624; i += kWave) mt[i] = gmt[i];
                     mti = uni((int)gmt[624]);

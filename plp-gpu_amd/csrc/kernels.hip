@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
             wave_sync();
             if (uni(sh.quit) == 0 && a.use_rng) {
                 if (a.reset) {
-                    mt_seed_lane0(mt, a.dec_seed[d]);
+                    mt_seed_wave(mt, a.dec_seed[d]);
                 } else {
                     for (int i = lane; i < 624; i += kWave) mt[i] = gmt[i];
                     mti = uni((int)gmt[624]);
@@ -506,7 +506,7 @@ __global__ void __launch_bounds__(64) fsr_kernel(FsrArgs f) {
 __global__ void __launch_bounds__(64) rng_script_kernel(RngArgs g) {
     __shared__ uint32_t mt[624];
     const int sd = blockIdx.x;
-    mt_seed_lane0(mt, g.seeds[sd]);
+    mt_seed_wave(mt, g.seeds[sd]);
     block_sync();
     int mti = 624;
     for (int i = 0; i < g.n_ops; i++) {

@@ -36,51 +36,74 @@ __device__ __forceinline__ uint32_t mt_mix(uint32_t cur, uint32_t nxt, uint32_t 
     return far ^ (y >> 1) ^ ((y & 1u) ? kMtMatrix : 0u);
 }
 
-// Serial CPython seeding by lane 0 (the recurrence is inherently serial);
-// the previous word is carried in a register so LDS reads stay off the
-// dependence chain.  Caller syncs the wave afterwards.
-__device__ inline void mt_seed_lane0(uint32_t* mt, uint64_t seed) {
-    if (lane_id() != 0) return;
-    uint32_t key[2];
-    int keyused;
-    if (seed == 0) {
-        key[0] = 0u;
-        key[1] = 0u;
-        keyused = 1;
-    } else {
-        key[0] = (uint32_t)seed;
-        key[1] = (uint32_t)(seed >> 32);
-        keyused = key[1] ? 2 : 1;
-    }
+// init_genrand(19650218): the state every seeding starts from, a constant.
+struct MtInitTable {
+    uint32_t w[640];   // 624 words, padded to ten 64-lane rows
+};
+
+constexpr MtInitTable mt_init_table() {
+    MtInitTable t{};
     uint32_t prev = 19650218u;
-    mt[0] = prev;
+    t.w[0] = prev;
     for (int i = 1; i < 624; i++) {
         prev = 1812433253u * (prev ^ (prev >> 30)) + (uint32_t)i;
-        mt[i] = prev;
+        t.w[i] = prev;
     }
-    prev = 19650218u;   // mt[i - 1] for i = 1
-    int i = 1, j = 0;
-    for (int k = 624; k; k--) {
-        prev = (mt[i] ^ ((prev ^ (prev >> 30)) * 1664525u)) + key[j] + (uint32_t)j;
-        mt[i] = prev;
-        i++;
-        j++;
-        if (i >= 624) {
-            mt[0] = prev;
-            i = 1;
+    return t;
+}
+
+static __constant__ const MtInitTable kMtInit = mt_init_table();
+
+// CPython seeding, all lanes of one wave.  init_by_array's two loops are a
+// serial recurrence through the previous word; the state words sit in ten
+// registers (lane l of row r holds word 64 r + l), so each step reads its
+// word and writes the result with a lane move, not an LDS round trip.  The
+// words visited are 1..623, 1 (first loop) and 2..623, 1 (second loop).
+// Caller syncs the wave afterwards.
+__device__ inline void mt_seed_wave(uint32_t* mt, uint64_t seed) {
+    const int lane = lane_id();
+    uint32_t v[10];
+#pragma unroll
+    for (int r = 0; r < 10; r++) v[r] = kMtInit.w[64 * r + lane];
+    // key = the 32-bit chunks of the seed (one chunk below 2^32, seed 0 gives
+    // [0]); the first loop adds key[j] + j with j cycling over the chunks
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const uint32_t kodd = k1 ? k1 + 1u : k0;
+    uint32_t prev = 19650218u;   // word 0 of the initial state
+    int n = 0;
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const int hi = r == 9 ? 624 - 576 : kWave;
+        for (int q = r == 0 ? 1 : 0; q < hi; q++) {
+            const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)v[r], q);
+            prev = (x ^ ((prev ^ (prev >> 30)) * 1664525u)) + ((n & 1) ? kodd : k0);
+            n++;
+            v[r] = lane == q ? prev : v[r];
         }
-        if (j >= keyused) j = 0;
     }
-    for (int k = 623; k; k--) {
-        prev = (mt[i] ^ ((prev ^ (prev >> 30)) * 1566083941u)) - (uint32_t)i;
-        mt[i] = prev;
-        i++;
-        if (i >= 624) {
-            mt[0] = prev;
-            i = 1;
+    {   // 624th step: word 1 again (n = 623, odd)
+        const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)v[0], 1);
+        prev = (x ^ ((prev ^ (prev >> 30)) * 1664525u)) + kodd;
+        v[0] = lane == 1 ? prev : v[0];
+    }
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const int hi = r == 9 ? 624 - 576 : kWave;
+        for (int q = r == 0 ? 2 : 0; q < hi; q++) {
+            const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)v[r], q);
+            prev = (x ^ ((prev ^ (prev >> 30)) * 1566083941u)) - (uint32_t)(kWave * r + q);
+            v[r] = lane == q ? prev : v[r];
         }
     }
-    mt[0] = 0x80000000u;
+    {   // last step: word 1
+        const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)v[0], 1);
+        prev = (x ^ ((prev ^ (prev >> 30)) * 1566083941u)) - 1u;
+        v[0] = lane == 1 ? prev : v[0];
+    }
+    v[0] = lane == 0 ? 0x80000000u : v[0];
+#pragma unroll
+    for (int r = 0; r < 10; r++)
+        if (kWave * r + lane < 624) mt[kWave * r + lane] = v[r];
 }
 
 // Regenerate the block: all lanes of one wave.
