@@ -30,15 +30,35 @@ namespace plpk {
 // that ends a placement step; every wave reads them after it, and the next
 // write comes after the following step's verdict barrier.
 struct StepShared {
+    alignas(16) int cnt[2][8];   // feasible pairs found by each wave, by evaluated-step
+                                 // parity: a wave may start step k+1 while others
+                                 // still read step k
     int item;       // queue item of the block
     int quit;       // the decoder stopped (capacity error): leave the pass loop
     int n_act;      // live extreme points after the last placement
     int n_box;      // placed boxes after the last placement
     int nb[6];      // the box just placed: x, y, z, x2, y2, z2
     uint32_t cstate[48];   // pair states of its 8 corners x this type's orientations
-    int cnt[2][8];  // feasible pairs found by each wave, by evaluated-step parity:
-                    // a wave may start step k+1 while others still read step k
 };
+
+// Sum of the NW per-wave counts, read in 16-byte pieces (one LDS round trip).
+template <int NW>
+__device__ __forceinline__ int sum_counts(const int* c) {
+    if constexpr (NW == 1) {
+        return c[0];
+    } else if constexpr (NW == 2) {
+        const int2 v = *(const int2*)c;
+        return v.x + v.y;
+    } else {
+        int t = 0;
+#pragma unroll
+        for (int i = 0; i < NW; i += 4) {
+            const int4 v = *(const int4*)(c + i);
+            t += v.x + v.y + v.z + v.w;
+        }
+        return t;
+    }
+}
 
 // Persistent decoder: one block of NW waves decodes one (instance, seed)
 // for n_passes passes (plp_common.hpp:404-456 per pass, FSR and the D4'
@@ -238,6 +258,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     key_xyz(act[e], xb, yb, x, y, z);
                     const int w = s_orw[3 * o], dd = s_orw[3 * o + 1], h = s_orw[3 * o + 2];
                     const int oi = oidx[e];   // read with the point (used by same-type steps)
+                    asm volatile("" ::"v"(oi));   // keeps the read beside the point's, not in the branch
                     bool c12 = valid & (x + w <= L) & (y + dd <= W) & (z + h <= H);
                     uint32_t sup = 0;
                     bool scan = c12;
@@ -289,10 +310,15 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 ev++;
                 if (lane == 0) sh.cnt[par][wid] = cnt_w;
                 block_sync();
-                int cnt = 0;
-#pragma unroll
-                for (int w2 = 0; w2 < NW; w2++) cnt += sh.cnt[par][w2];
-                cnt = uni(cnt);
+                // wave 0 reads its next random() words and the first mask
+                // with the counts: one LDS round trip for the three
+                MtPeek pk{0u, 0u};
+                uint64_t m0 = 0;
+                if (wid == 0) {
+                    if (a.use_rng) pk = mt_peek2(mt, mti);
+                    m0 = mask[0];
+                }
+                const int cnt = uni(sum_counts<NW>(sh.cnt[par]));
                 if (cnt == 0) {
                     fail_ti = ti;     // this copy is skipped; no RNG use
                     continue;         // nothing changed: no barrier needed
@@ -306,11 +332,11 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     // 1 - eps (plp_common.hpp:427-435).
                     int idx = 0;
                     if (a.use_rng) {
-                        if (mt_random(mt, mti) >= 1.0 - a.eps) idx = mt_randrange(mt, mti, cnt);
+                        if (mt_random(mt, mti, pk) >= 1.0 - a.eps) idx = mt_randrange(mt, mti, cnt);
                     }
                     // The idx-th feasible pair.
                     int c = 0;
-                    uint64_t m = uni(mask[0]);
+                    uint64_t m = uni(m0);
                     for (;;) {
                         const int pc = __popcll(m);
                         if (idx < pc) break;

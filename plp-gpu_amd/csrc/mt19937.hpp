@@ -175,18 +175,33 @@ __device__ __forceinline__ uint32_t mt_next(uint32_t* mt, int& mti) {
     return mt_temper(y);
 }
 
-__device__ __forceinline__ double mt_random(uint32_t* mt, int& mti) {
+// The two words the next random() takes when no regeneration falls inside
+// it (mti <= 622; otherwise two in-bounds words that go unused), so that a
+// caller can read them together with other LDS data.
+struct MtPeek {
     uint32_t a, b;
-    if (mti + 2 <= 624) {   // both words without a twist: read them together
-        const uint32_t ya = mt[mti], yb = mt[mti + 1];
+};
+
+__device__ __forceinline__ MtPeek mt_peek2(const uint32_t* mt, int mti) {
+    const int i = mti <= 622 ? mti : 622;
+    return MtPeek{mt[i], mt[i + 1]};
+}
+
+__device__ __forceinline__ double mt_random(uint32_t* mt, int& mti, MtPeek pk) {
+    uint32_t a, b;
+    if (mti + 2 <= 624) {   // both words without a regeneration
         mti += 2;
-        a = mt_temper(uni(ya)) >> 5;
-        b = mt_temper(uni(yb)) >> 6;
+        a = mt_temper(uni(pk.a)) >> 5;
+        b = mt_temper(uni(pk.b)) >> 6;
     } else {
         a = mt_next(mt, mti) >> 5;
         b = mt_next(mt, mti) >> 6;
     }
     return ((double)a * 67108864.0 + (double)b) * (1.0 / 9007199254740992.0);
+}
+
+__device__ __forceinline__ double mt_random(uint32_t* mt, int& mti) {
+    return mt_random(mt, mti, mt_peek2(mt, mti));
 }
 
 __device__ __forceinline__ uint32_t mt_getrandbits(uint32_t* mt, int& mti, int k) {
