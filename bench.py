@@ -232,7 +232,6 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     rows, _ = eng.fetch(placements=False)
     assert all(r.err == 0 for r in rows)
-    live = sum(r.work_live_pair_box for r in rows)     # the tests this algorithm needs
     shape = eng.describe()
     # The reference's pair x box tests of the same step, from one untimed run
     # of an engine that also keeps the dead extreme points (same results).
@@ -245,6 +244,7 @@ def main() -> int:
     assert [(r.n_boxes, r.volume, r.score) for r in crows] == \
         [(r.n_boxes, r.volume, r.score) for r in rows]
     work = sum(r.work_pair_box for r in crows)
+    live = sum(r.work_live_pair_box for r in crows)    # the tests this algorithm needs
     packings = K_DEC * PASSES * a.steps
     value = packings / elapsed
     ms_launch = sum(kernel_ms) / len(kernel_ms)

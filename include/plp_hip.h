@@ -147,7 +147,8 @@ typedef struct {
     uint64_t seed;
     uint64_t work_pair_box;  /* sum over steps of n_cand * n_ors * n_boxes (brute-force AABB tests);
                                 engine rows: only with count_work, else 0 */
-    uint64_t work_live_pair_box;  /* the same over the steps and live points the kernel evaluates */
+    uint64_t work_live_pair_box;  /* the same over the steps and live points the kernel evaluates;
+                                     engine rows: only with count_work, else 0 */
     int32_t max_ep;          /* largest extreme-point list of the reference (engine: count_work) */
     int32_t ep_need;         /* engine: the extreme-point capacity this decoder needed */
 } plp_row_result;

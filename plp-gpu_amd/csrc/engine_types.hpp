@@ -34,6 +34,7 @@ struct DecState {
     uint64_t work_pair_box;  // brute-force AABB tests so far (sum of n_cand * n_ors * n_box);
                              // counted only with count_work
     uint64_t work_live;      // the same over the steps and live points this kernel evaluates
+                             // (count_work only)
     uint32_t work_steps;     // steps that evaluated feasibility
     uint32_t max_ep;         // largest reference extreme-point list (count_work only)
     uint32_t ep_need;        // largest list this decoder held (the capacity it needs)

@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 }
                 int wm = 0, dm = 0, hm = 0;
                 if (wid == 0) {
-                    live += (uint64_t)n_act * (uint64_t)no * (uint64_t)n_box;
+                    if (a.count_work) live += (uint64_t)n_act * (uint64_t)no * (uint64_t)n_box;
                     const int32_t* mn = a.step_min + 4 * (size_t)(I.step_off + s);
                     wm = mn[0];
                     dm = mn[1];

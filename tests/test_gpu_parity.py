@@ -266,12 +266,11 @@ def test_engine_many_decoders_vs_oracle(plp, ctx, orc, wpd):
         want = orc.run(o_inst, seed=k + 1, passes=2, eps=0.01, theta=0.9)
         got = plp.to_json_dict(rows[k], pl[k])
         assert got == {x: want[x] for x in got}, k
-        assert 0 < rows[k].work_live_pair_box, k
         if count:
             assert rows[k].work_pair_box == want["pair_box"], k
-            assert rows[k].work_live_pair_box <= rows[k].work_pair_box, k
+            assert 0 < rows[k].work_live_pair_box <= rows[k].work_pair_box, k
         else:
-            assert rows[k].work_pair_box == 0, k
+            assert rows[k].work_pair_box == 0 and rows[k].work_live_pair_box == 0, k
 
 
 @pytest.mark.parametrize("wpd", [1, 4])
