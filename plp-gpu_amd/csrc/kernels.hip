@@ -26,17 +26,23 @@ namespace plpk {
 
 // ------------------------------------------------------------ decoder ----
 
-// Block-wide scalars.  Wave 0 writes quit / n_act / n_box before the barrier
-// that ends a placement step; every wave reads them after it, and the next
-// write comes after the following step's verdict barrier.
+// One evaluated step as wave 0 publishes it: the step index (-1: the pass is
+// over), its type | n_ors << 24, whether the previous evaluated step had the
+// same type, and the live points and boxes it starts from.
+struct alignas(16) StepDesc {
+    int s, sv, inc, n_act, n_box, pad0, pad1, pad2;
+};
+
+// Block-wide scalars.  Wave 0 writes step before the barrier that opens the
+// step (the one ending the previous step); every wave reads it before the
+// step's verdict barrier, and the next write comes after that barrier.
 struct StepShared {
     alignas(16) int cnt[2][8];   // feasible pairs found by each wave, by evaluated-step
                                  // parity: a wave may start step k+1 while others
                                  // still read step k
+    StepDesc step;  // the step being evaluated, published by wave 0
     int item;       // queue item of the block
     int quit;       // the decoder stopped (capacity error): leave the pass loop
-    int n_act;      // live extreme points after the last placement
-    int n_box;      // placed boxes after the last placement
     int nb[6];      // the box just placed: x, y, z, x2, y2, z2
     uint32_t cstate[48];   // pair states of its 8 corners x this type's orientations
 };
@@ -171,27 +177,34 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 }
                 for (int i = lane; i < a.box_cap + 8; i += kWave) box[i] = sentinel_box();
             }
-            block_sync();
-            // Every wave follows the step sequence itself: the memo decision
-            // needs only the type, the last failure and whether a placement
-            // happened, which all waves learn from the shared counts.
-            int nb_w = 0, na_w = 1;
+            // Wave 0 walks the step sequence (the memo decision and the step
+            // table need no other wave) and, before the barrier that opens
+            // an evaluated step, publishes it in sh.step; the other waves
+            // only follow the published steps.
             int ev = 0;     // evaluated steps so far (same count in every wave)
             int cur = 0;    // live list in use: 0 -> ep_a, 1 -> ep_b (every wave)
-            int prev_ti = -1;   // type of the previous evaluated step
-            for (int s = 0; s < I.n_steps; s++) {
-                const int sv = uni(s_step[s]);
-                const int ti = sv & 0xffffff, no = sv >> 24;
-                if (no == 0) continue;      // type with no allowed orientation
-                if (wid == 0) {
+            int s = 0;          // wave 0: the step being evaluated
+            int prev_ti = -1;   // wave 0: type of the previous evaluated step
+            // Wave 0: the next step from s0 on that is evaluated (a type with
+            // orientations, not skipped by the memo), published with the list
+            // and box counts; -1 ends the pass.  The memo: same type, no
+            // placement since its last copy found nothing -- the verdicts
+            // repeat and nothing is drawn, for every further copy of the type
+            // that follows, skipped together.
+            auto advance = [&](int s0, bool stop) {
+                s = s0;
+                int sv = 0;
+                while (!stop && s < I.n_steps) {
+                    sv = uni(s_step[s]);
+                    const int ti = sv & 0xffffff, no = sv >> 24;
+                    if (no == 0) {      // type with no allowed orientation
+                        s++;
+                        continue;
+                    }
                     if (a.count_work)
                         work += (uint64_t)(n_act + n_dead) * (uint64_t)no * (uint64_t)n_box;
                     n_steps_eval++;
-                }
-                // Same type, no placement since its last copy found nothing:
-                // the verdicts repeat, nothing is drawn -- for every further
-                // copy of the type that follows, skipped together.
-                if (ti == fail_ti) {
+                    if (ti != fail_ti) break;
                     int run = 1;
                     for (;;) {
                         const int q = s + run + lane;
@@ -203,19 +216,39 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                         }
                         run += kWave;
                     }
-                    if (wid == 0) {
-                        if (a.count_work)
-                            work += (uint64_t)(run - 1) * (uint64_t)(n_act + n_dead) *
-                                    (uint64_t)no * (uint64_t)n_box;
-                        n_steps_eval += (uint32_t)(run - 1);
-                    }
-                    s += run - 1;
-                    continue;
+                    if (a.count_work)
+                        work += (uint64_t)(run - 1) * (uint64_t)(n_act + n_dead) *
+                                (uint64_t)no * (uint64_t)n_box;
+                    n_steps_eval += (uint32_t)(run - 1);
+                    s += run;
                 }
+                const bool more = !stop && s < I.n_steps;
+                const int ti = sv & 0xffffff;
+                if (lane == 0) {
+                    StepDesc sd;
+                    sd.s = more ? s : -1;
+                    sd.sv = sv;
+                    sd.inc = more && ti == prev_ti;
+                    sd.n_act = n_act;
+                    sd.n_box = n_box;
+                    sh.step = sd;
+                }
+                if (more) prev_ti = ti;
+            };
+            if (wid == 0) advance(0, false);
+            block_sync();
+            for (;;) {
+                const StepDesc sd = sh.step;
+                const int s_cur = uni(sd.s);
+                if (s_cur < 0) break;
+                const int sv = uni(sd.sv);
+                const int ti = sv & 0xffffff, no = sv >> 24;
+                const bool inc = uni(sd.inc) != 0;
+                const int na_w = uni(sd.n_act), nb_w = uni(sd.n_box);
                 int wm = 0, dm = 0, hm = 0;
                 if (wid == 0) {
                     if (a.count_work) live += (uint64_t)n_act * (uint64_t)no * (uint64_t)n_box;
-                    const int32_t* mn = a.step_min + 4 * (size_t)(I.step_off + s);
+                    const int32_t* mn = a.step_min + 4 * (size_t)(I.step_off + s_cur);
                     wm = mn[0];
                     dm = mn[1];
                     hm = mn[2];
@@ -232,8 +265,6 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 // the same type, a pair of a point that was in the previous
                 // list takes its cached state and tests the new box alone;
                 // the other pairs scan the boxes.
-                const bool inc = ti == prev_ti;
-                prev_ti = ti;
                 const unsigned char* c_rd = cache + ((ev + 1) & 1) * cache_bytes;
                 unsigned char* c_wr = cache + (ev & 1) * cache_bytes;
                 int nx = 0, ny = 0, nz = 0, nx2 = -1, ny2 = -1, nz2 = -1;
@@ -320,8 +351,13 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 }
                 const int cnt = uni(sum_counts<NW>(sh.cnt[par]));
                 if (cnt == 0) {
-                    fail_ti = ti;     // this copy is skipped; no RNG use
-                    continue;         // nothing changed: no barrier needed
+                    // this copy is skipped; no RNG use, nothing changed
+                    if (wid == 0) {
+                        fail_ti = ti;
+                        advance(s_cur + 1, false);
+                    }
+                    block_sync();
+                    continue;
                 }
                 fail_ti = -1;
 
@@ -368,6 +404,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     wave_sync();
                 }
                 if (NW > 1) block_sync();   // every wave sees the new box
+                bool quit = false;
                 // ---- extreme-point merge (wave 0) and, meanwhile, the pair
                 // states of the new box's corners for this type (last wave),
                 // which the next step takes when it has the same type.
@@ -378,18 +415,13 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                                                   uni(hm));
                     max_ep = max(max_ep, (uint32_t)(n_act + n_dead));
                     ep_need = max(ep_need, (uint32_t)need);
-                    if (lane == 0) {
-                        sh.quit = need > a.ep_cap;   // PLP_E_EPCAP
-                        sh.n_act = n_act;
-                        sh.n_box = n_box;
-                    }
+                    quit = need > a.ep_cap;   // PLP_E_EPCAP
+                    if (lane == 0) sh.quit = quit;
                 }
                 if (wid == NW - 1)
                     corner_states(sh.nb, sh.cstate, s_orw, no, box, nb_w + 1, L, W, H, cbox);
+                if (wid == 0) advance(s_cur + 1, quit);
                 block_sync();
-                if (uni(sh.quit) != 0) break;
-                na_w = uni(sh.n_act);
-                nb_w = uni(sh.n_box);
                 cur ^= 1;
             }
 
