@@ -122,10 +122,13 @@ __device__ __forceinline__ int div_orients(int p, int no, uint32_t recip) {
 // stays below 0.5; when 2s >= A it is at least 0.5 and rounds to at least
 // 0.5.  Every box that passes C2 has A <= L * W < 2^31 (the container range
 // the engine accepts), so the integer form gives the reference's verdict bit
-// for bit.  Branch-free; when C2 fails the product (24-bit operands, exact
-// below 2^32) does not matter.
+// for bit.  It runs in 32 bits: s <= A (boxes whose top is at z do not
+// overlap one another, so their overlaps with the footprint are disjoint
+// parts of it; a clamped cache sum is smaller still), hence 2s < 2^32.
+// Branch-free; when C1 or C2 fails neither the product (24-bit operands,
+// exact below 2^32) nor the sum matters.
 __device__ __forceinline__ bool pair_verdict(bool c12, int z, int w, int d, uint32_t s) {
-    return c12 & ((z == 0) | (2ull * s >= (uint64_t)__umul24((uint32_t)w, (uint32_t)d)));
+    return c12 & ((z == 0) | (2u * s >= __umul24((uint32_t)w, (uint32_t)d)));
 }
 
 // Pair-state cache entry: the support sum of a pair that passes C1 and C2,
@@ -139,15 +142,19 @@ constexpr uint16_t kCorner = 0xfff0u;     // kCorner + j: corner j of the last b
 // 0xffff for kBlocked (exact for C3 when every w*d <= kNarrowMaxArea).
 // One 32-bit read for either width (a 16-bit entry comes with its aligned
 // neighbour, which lies inside the cache or the region after it).
-__device__ __forceinline__ uint32_t cache_get(const void* c, int i, bool wide) {
-    const uint32_t w = *(const uint32_t*)((const unsigned char*)c + (wide ? 4 * i : 4 * (i >> 1)));
-    if (wide) return w;
-    const uint32_t v = (w >> (16 * (i & 1))) & 0xffffu;
-    return v == 0xffffu ? kBlocked : v;
+template <bool WIDE>
+__device__ __forceinline__ uint32_t cache_get(const void* c, int i) {
+    if constexpr (WIDE) {
+        return ((const uint32_t*)c)[i];
+    } else {
+        const uint32_t v = ((const uint16_t*)c)[i];
+        return v == 0xffffu ? kBlocked : v;
+    }
 }
 
-__device__ __forceinline__ void cache_put(void* c, int i, uint32_t v, bool wide) {
-    if (wide) ((uint32_t*)c)[i] = v;
+template <bool WIDE>
+__device__ __forceinline__ void cache_put(void* c, int i, uint32_t v) {
+    if constexpr (WIDE) ((uint32_t*)c)[i] = v;
     else ((uint16_t*)c)[i] = (uint16_t)(v == kBlocked ? 0xffffu : min(v, 0xfffeu));
 }
 
@@ -548,7 +555,7 @@ static __device__ inline void corner_states(const int* nb, uint32_t* cstate, con
                                      Box* cbox) {
     const int lane = lane_id();
     const int n = 8 * no;
-    const int j = div_orients(lane, no, orient_recip(no)), o = lane - j * no;
+    const int j = div_orients(lane, no, orient_recip(no)), o = lane - __mul24(j, no);
     // lanes n..63 compute a corner too (j & 7, o < no: in bounds) and are
     // masked out by c2
     const int x = (j & 1) ? nb[3] : nb[0];

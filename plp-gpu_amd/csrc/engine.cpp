@@ -82,7 +82,7 @@ int dyn_lds(plp_engine* g, int ep_cap) {
 int static_lds(plp_engine* g) {
     int st = 0;
     hipError_t e = g->blocks ? plpk::block_static_lds(g->wpd, &st)
-                             : plpk::decode_static_lds(g->wpd, &st);
+                             : plpk::decode_static_lds(g->wpd, g->wide != 0, &st);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         st = 4096;
@@ -104,7 +104,7 @@ int shape(plp_engine* g, int ep_cap, int blocks_per_cu_req, int n_work) {
                                      " extreme points) exceeds the block LDS limit");
     int occ = 0;
     hipError_t e = g->blocks ? plpk::block_occupancy(g->wpd, lds, &occ)
-                             : plpk::decode_occupancy(g->wpd, lds, &occ);
+                             : plpk::decode_occupancy(g->wpd, g->wide != 0, lds, &occ);
     if (e != hipSuccess) return hip_fail(e, "decoder occupancy");
     if (occ < 1) return fail(PLP_E_EPCAP, "the decoder cannot be resident with this LDS size");
     int bpc = blocks_per_cu_req > 0 ? std::min(blocks_per_cu_req, occ) : occ;

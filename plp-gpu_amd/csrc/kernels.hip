@@ -86,8 +86,9 @@ __device__ __forceinline__ int sum_counts(const int* c) {
 // point merge), FSR and the selection; the other waves wait at the block
 // barrier.
 // The bound asks for at most 128 VGPRs (four waves per SIMD, i.e. four
-// 4-wave decoders per CU).
-template <int NW>
+// 4-wave decoders per CU).  WIDE selects 32-bit pair-state cache entries
+// (a.wide_cache), else 16-bit ones.
+template <int NW, bool WIDE>
 __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int s_ors[18 * NW];
@@ -100,8 +101,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
     uint32_t* const dead = (uint32_t*)(lds + lay.off_dead);
     uint16_t* const oidx = (uint16_t*)(lds + lay.off_oidx);
     unsigned char* const cache = lds + lay.off_cache;
-    const bool wide = a.wide_cache != 0;
-    const int cache_bytes = lay.n_cache * (wide ? 4 : 2);
+    const int cache_bytes = lay.n_cache * (WIDE ? 4 : 2);
     uint64_t* mask = (uint64_t*)(lds + lay.off_mask);
     double* load = (double*)(lds + lay.off_load);
     uint64_t* skey = (uint64_t*)(lds + lay.off_key);
@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     // are masked out of every result
                     const int pv = valid ? p : 0;
                     const int e = div_orients(pv, no, magic);
-                    const int o = pv - e * no;
+                    const int o = pv - __mul24(e, no);
                     int x, y, z;
                     key_xyz(act[e], xb, yb, x, y, z);
                     const int w = s_orw[3 * o], dd = s_orw[3 * o + 1], h = s_orw[3 * o + 2];
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                         const bool corner = (oi >= kCorner) & (oi != kNewPoint);
                         const bool cached = oi < kCorner;
                         const uint32_t cs = sh.cstate[corner ? (oi - kCorner) * no + o : 0];
-                        const uint32_t old = cache_get(c_rd, cached ? oi * no + o : 0, wide);
+                        const uint32_t old = cache_get<WIDE>(c_rd, cached ? oi * no + o : 0);
                         const int ox = min(x + w, nx2) - max(x, nx);
                         const int oy = min(y + dd, ny2) - max(y, ny);
                         const int oz = min(z + h, nz2) - max(z, nz);
@@ -320,7 +320,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     // Settled lanes first, so that nothing of theirs stays
                     // live across the scan.
                     uint64_t m = ballot(!scan & pair_verdict(c12, z, w, dd, sup));
-                    if (valid && !scan) cache_put(c_wr, p, c12 ? sup : kBlocked, wide);
+                    if (valid && !scan) cache_put<WIDE>(c_wr, p, c12 ? sup : kBlocked);
                     if (ballot(scan) != 0) {
                         const int xlo = wave_min(scan ? x : 0x7fffffff);
                         const int xhi = wave_max(scan ? x + w : -1);
@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                         uint32_t s2 = 0;
                         const bool a2 = pair_scan(x, y, z, w, dd, h, scan, box, nb_w, xlo, xhi, ylo,
                                                   yhi, zlo, zhi, cbox, s2);
-                        if (scan) cache_put(c_wr, p, a2 ? s2 : kBlocked, wide);
+                        if (scan) cache_put<WIDE>(c_wr, p, a2 ? s2 : kBlocked);
                         m |= ballot(scan & pair_verdict(a2, z, w, dd, s2));
                     }
                     if (lane == 0) mask[c] = m;
@@ -384,7 +384,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     const int bit = __ffsll((unsigned long long)ballot(hit)) - 1;
                     const int p = c * kWave + bit;
                     const int e = div_orients(p, no, magic);
-                    const int o = p - e * no;
+                    const int o = p - __mul24(e, no);
                     key_xyz(uni(act[e]), xb, yb, x, y, z);
                     x = uni(x);
                     y = uni(y);
@@ -588,17 +588,24 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int n) {
 
 // ------------------------------------------------------------ launchers --
 
-hipError_t launch_decode(const DecodeArgs& a, int waves, int grid, int lds_bytes,
-                         hipStream_t st) {
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, a.work_counter, 0u, 1);
+template <bool WIDE>
+static hipError_t launch_decode_w(const DecodeArgs& a, int waves, int grid, int lds_bytes,
+                                  hipStream_t st) {
     switch (waves) {
-        case 1: hipLaunchKernelGGL(decode_kernel<1>, dim3(grid), dim3(64), (size_t)lds_bytes, st, a); break;
-        case 2: hipLaunchKernelGGL(decode_kernel<2>, dim3(grid), dim3(128), (size_t)lds_bytes, st, a); break;
-        case 4: hipLaunchKernelGGL(decode_kernel<4>, dim3(grid), dim3(256), (size_t)lds_bytes, st, a); break;
-        case 8: hipLaunchKernelGGL(decode_kernel<8>, dim3(grid), dim3(512), (size_t)lds_bytes, st, a); break;
+        case 1: hipLaunchKernelGGL((decode_kernel<1, WIDE>), dim3(grid), dim3(64), (size_t)lds_bytes, st, a); break;
+        case 2: hipLaunchKernelGGL((decode_kernel<2, WIDE>), dim3(grid), dim3(128), (size_t)lds_bytes, st, a); break;
+        case 4: hipLaunchKernelGGL((decode_kernel<4, WIDE>), dim3(grid), dim3(256), (size_t)lds_bytes, st, a); break;
+        case 8: hipLaunchKernelGGL((decode_kernel<8, WIDE>), dim3(grid), dim3(512), (size_t)lds_bytes, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_decode(const DecodeArgs& a, int waves, int grid, int lds_bytes,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, a.work_counter, 0u, 1);
+    return a.wide_cache ? launch_decode_w<true>(a, waves, grid, lds_bytes, st)
+                        : launch_decode_w<false>(a, waves, grid, lds_bytes, st);
 }
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st) {
@@ -621,41 +628,52 @@ hipError_t launch_rng_script(const RngArgs& g, int n_seeds, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int NW>
+template <int NW, bool WIDE>
 static hipError_t occupancy_of(int lds_bytes, int* blocks_per_cu) {
-    hipError_t e = hipFuncSetAttribute((const void*)decode_kernel<NW>,
+    hipError_t e = hipFuncSetAttribute((const void*)decode_kernel<NW, WIDE>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
     if (e != hipSuccess) return e;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, decode_kernel<NW>, 64 * NW,
-                                                        (size_t)lds_bytes);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, decode_kernel<NW, WIDE>,
+                                                        64 * NW, (size_t)lds_bytes);
 }
 
-template <int NW>
+template <int NW, bool WIDE>
 static hipError_t static_lds_of(int* bytes) {
     hipFuncAttributes at;
-    hipError_t e = hipFuncGetAttributes(&at, (const void*)decode_kernel<NW>);
+    hipError_t e = hipFuncGetAttributes(&at, (const void*)decode_kernel<NW, WIDE>);
     if (e == hipSuccess) *bytes = (int)at.sharedSizeBytes;
     return e;
 }
 
-hipError_t decode_static_lds(int waves, int* bytes) {
+template <bool WIDE>
+static hipError_t decode_static_lds_w(int waves, int* bytes) {
     switch (waves) {
-        case 1: return static_lds_of<1>(bytes);
-        case 2: return static_lds_of<2>(bytes);
-        case 4: return static_lds_of<4>(bytes);
-        case 8: return static_lds_of<8>(bytes);
+        case 1: return static_lds_of<1, WIDE>(bytes);
+        case 2: return static_lds_of<2, WIDE>(bytes);
+        case 4: return static_lds_of<4, WIDE>(bytes);
+        case 8: return static_lds_of<8, WIDE>(bytes);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t decode_occupancy(int waves, int lds_bytes, int* blocks_per_cu) {
+hipError_t decode_static_lds(int waves, bool wide, int* bytes) {
+    return wide ? decode_static_lds_w<true>(waves, bytes) : decode_static_lds_w<false>(waves, bytes);
+}
+
+template <bool WIDE>
+static hipError_t decode_occupancy_w(int waves, int lds_bytes, int* blocks_per_cu) {
     switch (waves) {
-        case 1: return occupancy_of<1>(lds_bytes, blocks_per_cu);
-        case 2: return occupancy_of<2>(lds_bytes, blocks_per_cu);
-        case 4: return occupancy_of<4>(lds_bytes, blocks_per_cu);
-        case 8: return occupancy_of<8>(lds_bytes, blocks_per_cu);
+        case 1: return occupancy_of<1, WIDE>(lds_bytes, blocks_per_cu);
+        case 2: return occupancy_of<2, WIDE>(lds_bytes, blocks_per_cu);
+        case 4: return occupancy_of<4, WIDE>(lds_bytes, blocks_per_cu);
+        case 8: return occupancy_of<8, WIDE>(lds_bytes, blocks_per_cu);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t decode_occupancy(int waves, bool wide, int lds_bytes, int* blocks_per_cu) {
+    return wide ? decode_occupancy_w<true>(waves, lds_bytes, blocks_per_cu)
+                : decode_occupancy_w<false>(waves, lds_bytes, blocks_per_cu);
 }
 
 hipError_t fsr_prepare(int lds_bytes) {

@@ -315,6 +315,34 @@ def test_engine_many_types_vs_oracle(plp, ctx, orc, wpd):
         assert got == {x: want[x] for x in got}, k
 
 
+@pytest.mark.parametrize("wpd", [1, 4])
+def test_engine_large_footprints_vs_oracle(plp, ctx, orc, wpd):
+    """Footprints up to 620 x 590 cm (w * d > 131068, so the pair-state cache
+    holds 32-bit support sums instead of clamped 16-bit ones) in a
+    1300 x 1000 x 900 container, 48 seeds, eps 0.3, 2 passes."""
+    import random
+    rr = random.Random(2051)
+    types = []
+    for i in range(10):
+        dims = [rr.randint(180, 620) for _ in range(3)]
+        types.append(f"{i + 1} {dims[0]} 1 {dims[1]} {rr.randint(0, 1)} {dims[2]} 1 {rr.randint(1, 4)}")
+    text = "1\n1 2051\n1300 1000 900\n10\n" + "\n".join(types) + "\n"
+    o_inst = orc.parse(text)[0]
+    assert max(a * b for t in o_inst[3] for a, b in ((t[0], t[2]), (t[0], t[4]), (t[2], t[4]))) > 131068
+    K = 48
+    eng = plp.Engine(ctx, waves_per_decoder=wpd)
+    eng.load([plp.Instance.from_text(text)], [0] * K, list(range(1, K + 1)),
+             plp.params(eps=0.3, theta=0.9))
+    eng.run(2)
+    eng.sync()
+    rows, pl = eng.fetch()
+    assert min(r.n_boxes for r in rows) > 1
+    for k in range(K):
+        want = orc.run(o_inst, seed=k + 1, passes=2, eps=0.3, theta=0.9)
+        got = plp.to_json_dict(rows[k], pl[k])
+        assert got == {x: want[x] for x in got}, k
+
+
 def test_engine_autotune_keeps_results(plp, ctx, ref_runs):
     cases = [c for c in ref_runs if c["case"].startswith("multi_c2_s")]
     inst = inst_from(plp, cases[0]["instance"])
