@@ -128,7 +128,7 @@ __device__ __forceinline__ int div_orients(int p, int no, uint32_t recip) {
 // Branch-free; when C1 or C2 fails neither the product (24-bit operands,
 // exact below 2^32) nor the sum matters.
 __device__ __forceinline__ bool pair_verdict(bool c12, int z, int w, int d, uint32_t s) {
-    return c12 & ((z == 0) | (2u * s >= __umul24((uint32_t)w, (uint32_t)d)));
+    return c12 & ((z == 0) | (2u * s >= (uint32_t)__umul24((uint32_t)w, (uint32_t)d)));
 }
 
 // Pair-state cache entry: the support sum of a pair that passes C1 and C2,

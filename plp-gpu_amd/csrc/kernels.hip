@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
     const int lane = lane_id();
     const int wid = (int)(threadIdx.x >> 6);
     Box* cbox = (Box*)(lds + lay.off_rel) + wid * kCullWindow;   // this wave's culled window
-    int* s_orw = s_ors + 18 * wid;                                   // this wave's orientations
+    const int orw = 18 * wid;   // this wave's orientations: s_ors[orw ..]
 
     for (;;) {
         if (threadIdx.x == 0) sh.item = (int)atomicAdd(a.work_counter, 1u);
@@ -129,7 +129,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
             const int t = steps[i];
             s_step[i] = t | (a.ors_tab[(size_t)(I.ors_off + t) * kOrsRow] << 24);
         }
-        int cur_ti = -1;   // type whose orientations this wave holds in s_orw
+        int cur_ti = -1;   // type whose orientations this wave holds in s_ors[orw ..]
 
         // ---- decoder state: in LDS (sst) between pass ends; wave 0
         int mti = 624;
@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 }
                 if (ti != cur_ti) {
                     const int32_t* orow = a.ors_tab + (size_t)(I.ors_off + ti) * kOrsRow;
-                    if (lane < 3 * no) s_orw[lane] = orow[1 + lane];
+                    if (lane < 3 * no) s_ors[orw + lane] = orow[1 + lane];
                     cur_ti = ti;
                 }
                 wave_sync();
@@ -287,12 +287,15 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     const int o = pv - __mul24(e, no);
                     int x, y, z;
                     key_xyz(act[e], xb, yb, x, y, z);
-                    const int w = s_orw[3 * o], dd = s_orw[3 * o + 1], h = s_orw[3 * o + 2];
+                    const int w = s_ors[orw + 3 * o], dd = s_ors[orw + 3 * o + 1],
+                              h = s_ors[orw + 3 * o + 2];
                     const int oi = oidx[e];   // read with the point (used by same-type steps)
                     asm volatile("" ::"v"(oi));   // keeps the read beside the point's, not in the branch
-                    bool c12 = valid & (x + w <= L) & (y + dd <= W) & (z + h <= H);
-                    uint32_t sup = 0;
-                    bool scan = c12;
+                    const bool c12 = valid & (x + w <= L) & (y + dd <= W) & (z + h <= H);
+                    // settled: the pair's state st (support sum or kBlocked) is
+                    // known without a box scan
+                    uint32_t st = kBlocked;
+                    bool settled = false;
                     if (inc) {
                         // a corner of the new box takes its precomputed state; a
                         // point of the previous list its old state combined with
@@ -308,19 +311,14 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                         const bool ov = min(oxy, oz) > 0;
                         const uint32_t add = __umul24((uint32_t)(((nz2 == z) & (oxy > 0)) ? ox : 0),
                                                       (uint32_t)oy);
-                        const uint32_t st =
-                            corner ? cs : (((old == kBlocked) | ov) ? kBlocked : old + add);
-                        const bool settled = c12 & (corner | cached);
-                        scan = c12 & !settled;
-                        if (settled) {
-                            c12 = st != kBlocked;
-                            sup = st;
-                        }
+                        st = corner ? cs : (((old == kBlocked) | ov) ? kBlocked : old + add);
+                        settled = c12 & (corner | cached);
                     }
+                    const bool scan = c12 & !settled;
                     // Settled lanes first, so that nothing of theirs stays
-                    // live across the scan.
-                    uint64_t m = ballot(!scan & pair_verdict(c12, z, w, dd, sup));
-                    if (valid && !scan) cache_put<WIDE>(c_wr, p, c12 ? sup : kBlocked);
+                    // live across the scan; a lane failing C2 is blocked.
+                    uint64_t m = ballot(settled & pair_verdict(st != kBlocked, z, w, dd, st));
+                    if (valid && !scan) cache_put<WIDE>(c_wr, p, settled ? st : kBlocked);
                     if (ballot(scan) != 0) {
                         const int xlo = wave_min(scan ? x : 0x7fffffff);
                         const int xhi = wave_max(scan ? x + w : -1);
@@ -334,12 +332,12 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                         if (scan) cache_put<WIDE>(c_wr, p, a2 ? s2 : kBlocked);
                         m |= ballot(scan & pair_verdict(a2, z, w, dd, s2));
                     }
-                    if (lane == 0) mask[c] = m;
+                    mask[c] = m;   // every lane writes the same (uniform) mask
                     cnt_w += __popcll(m);
                 }
                 const int par = ev & 1;
                 ev++;
-                if (lane == 0) sh.cnt[par][wid] = cnt_w;
+                sh.cnt[par][wid] = cnt_w;   // every lane: the same value, no exec mask change
                 block_sync();
                 // wave 0 reads its next random() words and the first mask
                 // with the counts: one LDS round trip for the three
@@ -389,9 +387,9 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     x = uni(x);
                     y = uni(y);
                     z = uni(z);
-                    X2 = x + uni(s_orw[3 * o]);
-                    Y2 = y + uni(s_orw[3 * o + 1]);
-                    Z2 = z + uni(s_orw[3 * o + 2]);
+                    X2 = x + uni(s_ors[orw + 3 * o]);
+                    Y2 = y + uni(s_ors[orw + 3 * o + 1]);
+                    Z2 = z + uni(s_ors[orw + 3 * o + 2]);
                     if (lane == 0) {
                         Box nbx;
                         nbx.x = x; nbx.y = y; nbx.z = z; nbx.x2 = X2; nbx.y2 = Y2; nbx.z2 = Z2;
@@ -419,7 +417,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     if (lane == 0) sh.quit = quit;
                 }
                 if (wid == NW - 1)
-                    corner_states(sh.nb, sh.cstate, s_orw, no, box, nb_w + 1, L, W, H, cbox);
+                    corner_states(sh.nb, sh.cstate, s_ors + orw, no, box, nb_w + 1, L, W, H, cbox);
                 if (wid == 0) advance(s_cur + 1, quit);
                 block_sync();
                 cur ^= 1;
