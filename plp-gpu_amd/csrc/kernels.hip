@@ -82,9 +82,10 @@ __device__ __forceinline__ int sum_counts(const int* c) {
 // brute-force work, can be reported.
 // Every wave evaluates 64-pair chunks c = wave, wave + NW, ...; each chunk's
 // verdicts become one ballot mask in LDS and the pick is the idx-th set bit.
-// Wave 0 alone runs the serial part of a step (RNG, pick, place, extreme-
-// point merge), FSR and the selection; the other waves wait at the block
-// barrier.
+// Wave 0 alone walks the step table (memo included) and publishes each
+// evaluated step, and runs the serial part of a step (RNG, pick, place,
+// extreme-point merge), FSR and the selection; the other waves follow the
+// published steps and wait at the block barriers.
 // The bound asks for at most 128 VGPRs (four waves per SIMD, i.e. four
 // 4-wave decoders per CU).  WIDE selects 32-bit pair-state cache entries
 // (a.wide_cache), else 16-bit ones.
