@@ -37,12 +37,12 @@ namespace plpk {
 namespace {
 
 struct BlockShared {
+    alignas(16) int cnt[2][8];   // feasible points found by each wave, by evaluated-test parity
     int item;       // queue item of the block
     int quit;       // the decoder stopped (capacity error)
     int n_act;      // live extreme points
     int n_box;      // placed blocks
     int n_plan;     // blocks planned by the current sweep
-    int cnt[2][8];  // feasible points found by each wave, by evaluated-test parity
 };
 
 // Componentwise smallest orientation over the types with copies left.
@@ -283,17 +283,14 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
                                                        ylo, yhi, zlo, zhi, cbox, sup);
                             m = ballot(pair_verdict(c12, z, bw, bd, sup));
                         }
-                        if (lane == 0) mask[c] = m;
+                        mask[c] = m;   // every lane writes the same (uniform) mask
                         cnt_w += __popcll(m);
                     }
                     const int par = ev & 1;
                     ev++;
-                    if (lane == 0) sh.cnt[par][wid] = cnt_w;
+                    sh.cnt[par][wid] = cnt_w;   // every lane: the same value
                     block_sync();
-                    int cnt = 0;
-#pragma unroll
-                    for (int w2 = 0; w2 < NW; w2++) cnt += sh.cnt[par][w2];
-                    cnt = uni(cnt);
+                    const int cnt = uni(sum_counts<NW>(sh.cnt[par]));
                     if (cnt == 0) {
                         fail_s = s;
                         continue;

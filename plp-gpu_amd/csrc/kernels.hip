@@ -47,25 +47,6 @@ struct StepShared {
     uint32_t cstate[48];   // pair states of its 8 corners x this type's orientations
 };
 
-// Sum of the NW per-wave counts, read in 16-byte pieces (one LDS round trip).
-template <int NW>
-__device__ __forceinline__ int sum_counts(const int* c) {
-    if constexpr (NW == 1) {
-        return c[0];
-    } else if constexpr (NW == 2) {
-        const int2 v = *(const int2*)c;
-        return v.x + v.y;
-    } else {
-        int t = 0;
-#pragma unroll
-        for (int i = 0; i < NW; i += 4) {
-            const int4 v = *(const int4*)(c + i);
-            t += v.x + v.y + v.z + v.w;
-        }
-        return t;
-    }
-}
-
 // Persistent decoder: one block of NW waves decodes one (instance, seed)
 // for n_passes passes (plp_common.hpp:404-456 per pass, FSR and the D4'
 // selection after each), then takes the next queue item.

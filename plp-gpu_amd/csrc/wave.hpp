@@ -67,6 +67,26 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Sum of the NW per-wave counts c[0..NW) (16-byte aligned), read in 16-byte
+// pieces: one LDS round trip.
+template <int NW>
+__device__ __forceinline__ int sum_counts(const int* c) {
+    if constexpr (NW == 1) {
+        return c[0];
+    } else if constexpr (NW == 2) {
+        const int2 v = *(const int2*)c;
+        return v.x + v.y;
+    } else {
+        int t = 0;
+#pragma unroll
+        for (int i = 0; i < NW; i += 4) {
+            const int4 v = *(const int4*)(c + i);
+            t += v.x + v.y + v.z + v.w;
+        }
+        return t;
+    }
+}
+
 // Wave-wide min / max through DPP lane moves (VALU latency, no LDS round
 // trip): quad swaps, half-row and row mirrors reduce each 16-lane row, then
 // row_bcast:15 and row_bcast:31 fold the rows into lane 63.  Rows a DPP step
