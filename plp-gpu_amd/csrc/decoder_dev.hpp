@@ -47,6 +47,14 @@ __device__ __forceinline__ uint32_t xyz_key(int x, int y, int z, int xb, int yb)
 // every lane has failed.  Areas fit 32 bits: L*W < 2^30 is a load-time
 // precondition, and ox, oy < 2^21 so the 24-bit multiply is exact.
 constexpr int kBoxGroup = 4;
+
+// a * b + c for a, b < 2^24 in one VALU op (the compiler otherwise pairs two
+// products into one three-way add, three ops per two boxes instead of two).
+__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
 constexpr int kCullWindow = kWave + kBoxGroup;   // cbox[] entries per wave
 
 __device__ __forceinline__ Box sentinel_box() {
@@ -64,8 +72,9 @@ __device__ __forceinline__ bool pair_scan(int x, int y, int z, int w, int d, int
     const int lane = lane_id();
     const int x2 = x + w, y2 = y + d, z2 = z + h;
     uint32_t s = 0;
+    uint64_t am = ballot(alive);   // lanes still passing C1, as a scalar mask
     for (int top = n_box - 1; top >= 0; top -= kWave) {
-        if (ballot(alive) == 0) break;
+        if (am == 0) break;
         const int b = top - lane;
         Box C = sentinel_box();
         if (b >= 0) C = boxes[b];
@@ -91,16 +100,16 @@ __device__ __forceinline__ bool pair_scan(int x, int y, int z, int w, int d, int
                 const int oy = min(y2, B[u].y2) - max(y, B[u].y);
                 const int oz = min(z2, B[u].z2) - max(z, B[u].z);
                 const int oxy = min(ox, oy);
-                alive = alive & !(min(oxy, oz) > 0);
+                am &= ~ballot(min(oxy, oz) > 0);
                 const bool sup = (B[u].z2 == z) & (oxy > 0);
-                s += __umul24((uint32_t)(sup ? ox : 0), (uint32_t)oy);
+                s = mad_u24((uint32_t)(sup ? ox : 0), (uint32_t)oy, s);
             }
-            if (ballot(alive) == 0) break;
+            if (am == 0) break;
         }
         wave_sync();   // the next window overwrites cbox[]
     }
     s_out = s;
-    return alive;
+    return (am >> lane) & 1ull;
 }
 
 // Division by an orientation count no in 1..6 without an integer divide:
