@@ -89,32 +89,33 @@ __device__ __forceinline__ int sum_counts(const int* c) {
 
 // Wave-wide min / max through DPP lane moves (VALU latency, no LDS round
 // trip): quad swaps, half-row and row mirrors reduce each 16-lane row, then
-// row_bcast:15 and row_bcast:31 fold the rows into lane 63.  Rows a DPP step
-// does not write keep their own value (old = v), which min/max leaves intact.
+// row_bcast:15 and row_bcast:31 fold the rows into lane 63.  Lanes a DPP step
+// does not write take `old`, the operation's identity (INT_MAX for min,
+// INT_MIN for max, 0 for or), so each step folds into one DPP-source VALU op.
 template <bool kMax>
 __device__ __forceinline__ int wave_reduce_dpp(int v) {
+    constexpr int id = kMax ? INT32_MIN : INT32_MAX;
     auto op = [](int a, int b) { return kMax ? max(a, b) : min(a, b); };
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));  // row_mirror
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));  // row_bcast:15
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x140, 0xF, 0xF, false));  // row_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x143, 0xC, 0xF, false));  // row_bcast:31
     return __builtin_amdgcn_readlane(v, 63);
 }
 __device__ __forceinline__ int wave_min(int v) { return wave_reduce_dpp<false>(v); }
 __device__ __forceinline__ int wave_max(int v) { return wave_reduce_dpp<true>(v); }
 
-// Wave-wide bitwise OR, the same DPP pattern (v | v = v for rows a step
-// does not write).
+// Wave-wide bitwise OR, the same DPP pattern (identity 0).
 __device__ __forceinline__ uint32_t wave_or(uint32_t u) {
     int v = (int)u;
-    v |= __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false);
-    v |= __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false);
-    v |= __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false);
-    v |= __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false);
-    v |= __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false);
-    v |= __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false);
+    v |= __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+    v |= __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+    v |= __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);
+    v |= __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);
+    v |= __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);
+    v |= __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);
     return (uint32_t)__builtin_amdgcn_readlane(v, 63);
 }
 
