@@ -7,6 +7,7 @@ from collections import defaultdict
 import pytest
 
 from conftest import golden_path
+from edge_cases import EDGE_TEXTS
 from oracle_lib import Oracle, load_text
 
 pytestmark = pytest.mark.gpu
@@ -115,6 +116,30 @@ def test_mixed_block_batch_vs_oracle(plp, ctx, orc):
         want = orc.block_run(orc.parse(texts[i])[0], seed=s, passes=2, eps=0.2, theta=0.9, kmax=3)
         got = plp.to_json_dict(rows[k], pl[k])
         assert {x: got[x] for x in FIELDS} == {x: want[x] for x in FIELDS}, k
+
+
+@pytest.mark.parametrize("eps", [0.0, 1.0])
+def test_block_edge_instances_vs_oracle(plp, ctx, orc, eps):
+    """The decoder's degenerate instances (nothing fits, one item, types
+    without an allowed orientation, an exact tiling) through the block engine,
+    seeds 0, 1 and 2^64 - 1; eps 1.0 shuffles every plan and picks every shape
+    at random."""
+    names = list(EDGE_TEXTS)
+    seeds = [0, 1, 2**64 - 1]
+    di = [i for i in range(len(names)) for _ in seeds]
+    ds = [s for _ in names for s in seeds]
+    eng = plp.Engine(ctx)
+    eng.load([plp.Instance.from_text(EDGE_TEXTS[n]) for n in names], di, ds,
+             plp.params(eps=eps, theta=0.9), blocks=plp.block_params())
+    eng.run(2)
+    eng.sync()
+    rows, pl = eng.fetch()
+    for k, (i, s) in enumerate(zip(di, ds)):
+        assert rows[k].err == 0, (names[i], s)
+        want = orc.block_run(orc.parse(EDGE_TEXTS[names[i]])[0], seed=s, passes=2, eps=eps,
+                             theta=0.9)
+        got = plp.to_json_dict(rows[k], pl[k])
+        assert {x: got[x] for x in FIELDS} == {x: want[x] for x in FIELDS}, (names[i], s)
 
 
 def test_block_capacity_escalation(plp, ctx, orc):

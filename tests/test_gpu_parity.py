@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from conftest import golden_path
+from edge_cases import EDGE_TEXTS
 from oracle_lib import Oracle, load_text
 
 pytestmark = pytest.mark.gpu
@@ -341,6 +342,27 @@ def test_engine_large_footprints_vs_oracle(plp, ctx, orc, wpd):
         want = orc.run(o_inst, seed=k + 1, passes=2, eps=0.3, theta=0.9)
         got = plp.to_json_dict(rows[k], pl[k])
         assert got == {x: want[x] for x in got}, k
+
+
+@pytest.mark.parametrize("eps", [0.01, 1.0])
+def test_engine_edge_instances_vs_oracle(plp, ctx, orc, eps):
+    """Degenerate instances (nothing fits, one item, types without an allowed
+    orientation, exact tilings) with seeds 0, 1 and 2^64 - 1, in one
+    multi-instance launch; eps 1.0 draws a random pick at every placement."""
+    names = list(EDGE_TEXTS)
+    seeds = [0, 1, 2**64 - 1]
+    dec_inst = [i for i in range(len(names)) for _ in seeds]
+    dec_seed = [s for _ in names for s in seeds]
+    eng = plp.Engine(ctx)
+    eng.load([plp.Instance.from_text(EDGE_TEXTS[n]) for n in names], dec_inst, dec_seed,
+             plp.params(eps=eps, theta=0.9))
+    eng.run(2)
+    eng.sync()
+    rows, pl = eng.fetch()
+    for d, (i, s) in enumerate(zip(dec_inst, dec_seed)):
+        assert rows[d].err == 0, (names[i], s)
+        want = orc.run(orc.parse(EDGE_TEXTS[names[i]])[0], seed=s, passes=2, eps=eps, theta=0.9)
+        assert plp.to_json_dict(rows[d], pl[d]) == ref_fields(want), (names[i], s)
 
 
 def test_engine_autotune_keeps_results(plp, ctx, ref_runs):
