@@ -435,7 +435,7 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
                     const bool better_acc =
                         st.n_acc < 0 || vol > st.v_acc || (vol == st.v_acc && n > st.n_acc);
                     const double score = better_all || better_acc
-                                             ? fsr_score(items, n, a.mu, load, skey, stot, nullptr)
+                                             ? fsr_score<4>(items, n, a.mu, load, skey, stot, nullptr)
                                              : 0.0;
                     if (better_all) {
                         st.n_all = n; st.v_all = vol; st.s_all = score;

@@ -200,32 +200,54 @@ __device__ __forceinline__ int visits_before(const Box& j, int jdx, const Box& i
     return ((j.z > i.z) | ((j.z == i.z) & (jdx < idx))) ? 1 : 0;
 }
 
-// fsr_score for n <= 2 * kWave boxes with the loads in registers: lane i
-// holds boxes i and i + 64, their loads and contact totals (also written to
-// tot[]).  The visiting order comes from each box's rank (the number of
+// fsr_score for n <= R * kWave boxes with the loads in registers: lane i
+// holds boxes i, i + 64, ..., their loads and contact totals (also written
+// to tot[]).  The visiting order comes from each box's rank (the number of
 // boxes before it in (z desc, index asc) order, scattered through ord[], n
 // ints of LDS); the k loop reads box k's coordinates and total from LDS and
-// its load from the lane that holds it.  Box j's load receives
-// additions only from the boxes whose bottom is at j's top, each through the
-// register of the lane that holds j, in the visiting order: the reference's
-// sequence of FP64 operations on every load[j].
+// its load from the lane that holds it.  Box j's load receives additions
+// only from the boxes whose bottom is at j's top, each through the register
+// of the lane that holds j, in the visiting order: the reference's sequence
+// of FP64 operations on every load[j].
+template <int R>
+__device__ __forceinline__ int read_lane_of(const int (&v)[R], int q, int src) {
+#pragma unroll
+    for (int i = 0; i < R - 1; i++)
+        if (q == i) return __builtin_amdgcn_readlane(v[i], src);
+    return __builtin_amdgcn_readlane(v[R - 1], src);
+}
+template <int R>
+__device__ __forceinline__ double read_lane_of(const double (&v)[R], int q, int src) {
+#pragma unroll
+    for (int i = 0; i < R - 1; i++)
+        if (q == i) return read_lane_f64(v[i], src);
+    return read_lane_f64(v[R - 1], src);
+}
+
+template <int R>
 static __device__ double fsr_score_reg(const Box* __restrict__ box, int n, double mu, int* ord,
                                        uint32_t* tot, double* fsr_out) {
     const int lane = lane_id();
-    const bool v0 = lane < n, v1 = lane + kWave < n;
-    const Box b0 = v0 ? box[lane] : sentinel_box();
-    const Box b1 = v1 ? box[lane + kWave] : sentinel_box();
     // m = RHO * V / 1e6 (plp_common.hpp:360-362); load starts at m
     auto mass = [](const Box& b) {
         const int64_t v = (int64_t)(b.x2 - b.x) * (b.y2 - b.y) * (b.z2 - b.z);
         return 50.0 * (double)v / 1.0e6;
     };
-    const double m0 = v0 ? mass(b0) : 1.0, m1 = v1 ? mass(b1) : 1.0;
-    double l0 = m0, l1 = m1;
+    bool v[R];
+    Box b[R];
+    double l[R];
+    uint32_t t[R];
+    int r[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        v[q] = lane + q * kWave < n;
+        b[q] = v[q] ? box[lane + q * kWave] : sentinel_box();
+        l[q] = v[q] ? mass(b[q]) : 1.0;
+        t[q] = 0;
+        r[q] = 0;
+    }
     // contact totals (the boxes whose tops touch a box's bottom do not
     // overlap one another, so a total is at most w * d < 2^30) and ranks
-    uint32_t t0 = 0, t1 = 0;
-    int r0 = 0, r1 = 0;
     constexpr int kG = 2;   // boxes whose reads issue together
     for (int j = 0; j < n; j += kG) {
         Box B[kG];
@@ -233,65 +255,61 @@ static __device__ double fsr_score_reg(const Box* __restrict__ box, int n, doubl
         for (int u = 0; u < kG; u++) B[u] = j + u < n ? box_xyz(box[j + u]) : sentinel_box();
 #pragma unroll
         for (int u = 0; u < kG; u++) {
-            t0 += contact_area(b0, B[u]);
-            t1 += contact_area(b1, B[u]);
-            r0 += visits_before(B[u], j + u, b0, lane);
-            r1 += visits_before(B[u], j + u, b1, lane + kWave);
+#pragma unroll
+            for (int q = 0; q < R; q++) {
+                t[q] += contact_area(b[q], B[u]);
+                r[q] += visits_before(B[u], j + u, b[q], lane + q * kWave);
+            }
         }
     }
-    if (v0) {
-        ord[r0] = lane;
-        tot[lane] = t0;
-    }
-    if (v1) {
-        ord[r1] = lane + kWave;
-        tot[lane + kWave] = t1;
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        if (v[q]) {
+            ord[r[q]] = lane + q * kWave;
+            tot[lane + q * kWave] = t[q];
+        }
     }
     wave_sync();
-    const int o0 = v0 ? ord[lane] : 0, o1 = v1 ? ord[lane + kWave] : 0;
+    int o[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) o[q] = v[q] ? ord[lane + q * kWave] : 0;
     // compute_fsr's k loop (plp_common.hpp:365-385).  Box k's coordinates
     // and total come from LDS one source ahead; its load from its lane.
-    int kn = __builtin_amdgcn_readlane(o0, 0);
+    int kn = __builtin_amdgcn_readlane(o[0], 0);
     Box Bn = box_xyz(box[kn]);
     uint32_t totn = tot[kn];
-    for (int t = 0; t < n; t++) {
+    for (int tt = 0; tt < n; tt++) {
         const int k = kn;
         const int bkz = uni(Bn.z), bkx = uni(Bn.x), bky = uni(Bn.y), bkx2 = uni(Bn.x2),
                   bky2 = uni(Bn.y2);
         const uint32_t total = uni(totn);
-        if (t + 1 < n) {
-            kn = t + 1 < kWave ? __builtin_amdgcn_readlane(o0, t + 1)
-                               : __builtin_amdgcn_readlane(o1, t + 1 - kWave);
+        if (tt + 1 < n) {
+            kn = read_lane_of(o, (tt + 1) >> 6, (tt + 1) & (kWave - 1));
             Bn = box_xyz(box[kn]);
             totn = tot[kn];
         }
         if (bkz == 0) break;   // the rest are floor boxes: nothing supports them
         if (total == 0) continue;
-        const int kl = k & (kWave - 1);
-        const double lk = k >= kWave ? read_lane_f64(l1, kl) : read_lane_f64(l0, kl);
+        const double lk = read_lane_of(l, k >> 6, k & (kWave - 1));
         const double dt = (double)total;
         // (double)a of the reference's 64-bit area a = ox * oy equals the FP64
         // product of the two extents: both are exact (a < 2^42 < 2^53)
-        {
-            const int ox = min(bkx2, b0.x2) - max(bkx, b0.x);
-            const int oy = min(bky2, b0.y2) - max(bky, b0.y);
-            if ((b0.z2 == bkz) & (ox > 0) & (oy > 0))
-                l0 = l0 + lk * ((double)ox * (double)oy) / dt;
-        }
-        {
-            const int ox = min(bkx2, b1.x2) - max(bkx, b1.x);
-            const int oy = min(bky2, b1.y2) - max(bky, b1.y);
-            if ((b1.z2 == bkz) & (ox > 0) & (oy > 0))
-                l1 = l1 + lk * ((double)ox * (double)oy) / dt;
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            const int ox = min(bkx2, b[q].x2) - max(bkx, b[q].x);
+            const int oy = min(bky2, b[q].y2) - max(bky, b[q].y);
+            if ((b[q].z2 == bkz) & (ox > 0) & (oy > 0))
+                l[q] = l[q] + lk * ((double)ox * (double)oy) / dt;
         }
     }
     // FSR_i = ((mu / 0.3) * load_i) / m_i; score = #(FSR >= 1) / n
-    const double f0 = (mu / 0.3) * l0 / m0, f1 = (mu / 0.3) * l1 / m1;
-    if (fsr_out) {
-        if (v0) fsr_out[lane] = f0;
-        if (v1) fsr_out[lane + kWave] = f1;
+    int c = 0;
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        const double f = (mu / 0.3) * l[q] / (v[q] ? mass(b[q]) : 1.0);   // m recomputed: same value
+        if (fsr_out && v[q]) fsr_out[lane + q * kWave] = f;
+        c += __popcll(ballot(v[q] && f >= 1.0));
     }
-    const int c = __popcll(ballot(v0 && f0 >= 1.0)) + __popcll(ballot(v1 && f1 >= 1.0));
     wave_sync();   // ord[] is free again
     return (double)c / (double)n;
 }
@@ -300,15 +318,19 @@ static __device__ double fsr_score_reg(const Box* __restrict__ box, int n, doubl
 // LDS, on one wave.  The reference adds load[k] * a / total into load[j]
 // while visiting k in (z desc, index asc) order, and the k loop stays serial
 // so every load[j] receives its additions in the reference's order
-// (bit-exact FP64).  Up to 128 boxes the state stays in registers
+// (bit-exact FP64).  Up to 64 * kRegBoxes boxes (kRegBoxes 2 or 4: the
+// registers a kernel can spare at its pass end) the state stays in registers
 // (fsr_score_reg); beyond, loads and totals sit in LDS and the order comes
 // from a bitonic sort of 64-bit keys ((2^31-1-z) << 32 | index).  Returns
 // the score; per-box FSR goes to fsr_out when non-null.
+template <int kRegBoxes = 2>
 static __device__ double fsr_score(const Box* __restrict__ box, int n, double mu, double* load,
                             uint64_t* skey, uint32_t* tot, double* fsr_out) {
     const int lane = lane_id();
     if (n <= 0) return 0.0;
-    if (n <= 2 * kWave) return fsr_score_reg(box, n, mu, (int*)skey, tot, fsr_out);
+    if (n <= 2 * kWave) return fsr_score_reg<2>(box, n, mu, (int*)skey, tot, fsr_out);
+    if (kRegBoxes >= 4 && n <= 4 * kWave)
+        return fsr_score_reg<4>(box, n, mu, (int*)skey, tot, fsr_out);
     const int n2 = next_pow2(n);
     for (int i = lane; i < n2; i += kWave)
         skey[i] = i < n ? (((uint64_t)(0x7fffffffu - (uint32_t)box[i].z) << 32) | (uint32_t)i)

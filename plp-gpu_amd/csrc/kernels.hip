@@ -535,7 +535,7 @@ __global__ void __launch_bounds__(64) fsr_kernel(FsrArgs f) {
         box[i] = B;
     }
     block_sync();
-    const double sc = fsr_score(box, n, f.mu, load, skey, stot,
+    const double sc = fsr_score<4>(box, n, f.mu, load, skey, stot,
                                 f.fsr_out ? f.fsr_out + r0 : nullptr);
     if (lane_id() == 0) f.score_out[pk] = sc;
 }
