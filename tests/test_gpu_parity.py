@@ -401,6 +401,31 @@ def test_plain_form_reference_cases(plp, ctx, ref_runs):
         assert plp.to_json_dict(row, pl) == ref_fields(c["result"]), c["case"]
 
 
+def test_fsr_eval_sizes_around_register_limits(plp, ctx, orc):
+    """FSR of packings of about 110-150 boxes from the 130-type instance (decoder
+    output, varied support structures) and of towers of 128, 129, 200, 256
+    and 257 boxes: the register forms (two and four boxes per lane) and the
+    LDS form, bit for bit against the oracle."""
+    from plp_amd import synth
+    text = synth.uniform_instance(2040, 130, 20, 90, 1.1)
+    eng = plp.Engine(ctx)
+    eng.load([plp.Instance.from_text(text)], [0] * 8, list(range(1, 9)),
+             plp.params(eps=0.5, theta=0.9))
+    eng.run(1)
+    eng.sync()
+    rows, pl = eng.fetch()
+    packs = [pl[k][:rows[k].n_boxes] for k in range(8)]
+    for n in (128, 129, 200, 256, 257):
+        packs.append(np.array([[0, 0, 0, 10 * i, 10, 10, 10] for i in range(n)], np.int32))
+    assert max(len(p) for p in packs[:8]) > 128
+    for mu in (0.2, 0.5):
+        fsr, score = ctx.fsr_eval(packs, mu=mu)
+        for j, p in enumerate(packs):
+            placed7 = [[r[1], r[2], r[3], r[4], r[5], r[6], r[0]] for r in p]
+            assert list(fsr[j][:len(p)]) == list(orc.compute_fsr(placed7, mu)), (j, len(p))
+            assert score[j] == orc.stability_score(placed7, mu), (j, len(p))
+
+
 def test_engine_fsr_beyond_reference_cap(plp, ctx, orc):
     """300 stacked boxes: FSR over more boxes than the reference device
     kernel's cap of 256 (plp_gpu.cu:1121-1146), bit-exact against the oracle
