@@ -196,8 +196,8 @@ def main() -> int:
         return 0
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: decoders never exchange data, so a multi-GPU run is "
-              "N independent copies of this job (see DESIGN.md); run with --gpus 1",
-              file=sys.stderr)
+              "N independent copies of this job, each GPU decoding its own share of the seeds; "
+              "run with --gpus 1", file=sys.stderr)
         return 2
     import plp_amd
     from plp_amd import synth
