@@ -460,12 +460,15 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
             }
             if (lane == 0) a.state[d] = sst;
         }
+        // work_counter[1] counts the decoders that end with an error, so the
+        // host reads one word, not every state, when none did
+        if (wid == 0 && lane == 0 && sst.err != 0) atomicAdd(a.work_counter + 1, 1u);
         block_sync();
     }
 }
 
 hipError_t launch_block(const BlockArgs& b, int waves, int grid, int lds_bytes, hipStream_t st) {
-    hipError_t e = launch_fill_u32(b.d.work_counter, 0u, 1, st);
+    hipError_t e = launch_fill_u32(b.d.work_counter, 0u, 2, st);
     if (e != hipSuccess) return e;
     switch (waves) {
         case 1: hipLaunchKernelGGL(block_kernel<1>, dim3(grid), dim3(64), (size_t)lds_bytes, st, b); break;

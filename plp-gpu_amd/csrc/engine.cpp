@@ -46,6 +46,7 @@ struct plp_engine {
     std::vector<int32_t> h_dec_inst;
     std::vector<uint64_t> h_seed;
     std::vector<DecState> h_state;
+    uint32_t* h_failed = nullptr;   // pinned: decoders of the last run that ended in error
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.0f;
     hipGraph_t graph = nullptr;
@@ -238,9 +239,15 @@ extern "C" int plp_engine_create(plp_ctx* ctx, const plp_engine_config* cfg, plp
     }
     int rc = g->d_counter.reserve(64, "queue head");
     if (rc != PLP_OK) {
-        delete g;
+        plp_engine_destroy(g);
         return rc;
     }
+    if ((e = hipHostMalloc((void**)&g->h_failed, 64, hipHostMallocDefault)) != hipSuccess) {
+        g->h_failed = nullptr;
+        plp_engine_destroy(g);
+        return hip_fail(e, "pinned error count");
+    }
+    *g->h_failed = 0;
     *out = g;
     return PLP_OK;
 }
@@ -257,6 +264,7 @@ extern "C" void plp_engine_destroy(plp_engine* g) {
         b->release();
     if (g->ev0) (void)hipEventDestroy(g->ev0);
     if (g->ev1) (void)hipEventDestroy(g->ev1);
+    if (g->h_failed) (void)hipHostFree(g->h_failed);
     delete g;
 }
 
@@ -581,6 +589,10 @@ extern "C" int plp_engine_run(plp_engine* g, int32_t n_passes) {
     int rc = launch_run(g, n_passes, reset);
     if (rc != PLP_OK) return rc;
     if ((e = hipEventRecord(g->ev1, g->ctx->stream)) != hipSuccess) return hip_fail(e, "event");
+    // the error count rides the stream to pinned memory; sync reads it
+    e = hipMemcpyAsync(g->h_failed, (const uint32_t*)g->d_counter.p + 1, sizeof(uint32_t),
+                       hipMemcpyDeviceToHost, g->ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "D2H error count");
     g->passes_done = reset ? n_passes : g->passes_done + n_passes;
     g->need_reset = false;
     g->states_fresh = false;
@@ -600,6 +612,7 @@ extern "C" int plp_engine_sync(plp_engine* g) {
     }
     g->last_ms = ms;
     if (g->passes_done == 0) return PLP_OK;
+    if (*g->h_failed == 0) return PLP_OK;   // no decoder failed: states stay on the device
     int rc = read_states(g);
     if (rc != PLP_OK) return rc;
     // Capacity escalation: a decoder whose extreme-point list outgrew the
@@ -726,6 +739,7 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
     g->need_reset = true;
     int rc = plp_engine_run(g, n_passes);
     if (rc == PLP_OK) rc = plp_engine_sync(g);
+    if (rc == PLP_OK && !g->states_fresh) rc = read_states(g);
     if (rc != PLP_OK) return rc;
     int need = 1;
     for (const DecState& s : g->h_state) need = std::max(need, (int)s.ep_need);

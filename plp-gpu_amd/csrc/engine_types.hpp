@@ -68,7 +68,7 @@ struct DecodeArgs {
     int32_t count_work;      // 1: also keep the dead points, to count the reference's work
     int32_t wide_cache;      // 1: 32-bit pair-state cache (some w*d > kNarrowMaxArea)
     double eps, theta, mu;
-    uint32_t* work_counter;  // persistent-grid work queue head
+    uint32_t* work_counter;  // [0] persistent-grid work queue head, [1] decoders ending in error
 };
 
 // Dynamic LDS layout of one decoder block.  The pass-time region (two

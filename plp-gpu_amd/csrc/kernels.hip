@@ -457,6 +457,9 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
             }
             if (lane == 0) a.state[d] = sst;
         }
+        // work_counter[1] counts the decoders that end with an error, so the
+        // host reads one word, not every state, when none did
+        if (wid == 0 && lane == 0 && sst.err != 0) atomicAdd(a.work_counter + 1, 1u);
         block_sync();
     }
 }
@@ -583,7 +586,7 @@ static hipError_t launch_decode_w(const DecodeArgs& a, int waves, int grid, int 
 
 hipError_t launch_decode(const DecodeArgs& a, int waves, int grid, int lds_bytes,
                          hipStream_t st) {
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, a.work_counter, 0u, 1);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, a.work_counter, 0u, 2);
     return a.wide_cache ? launch_decode_w<true>(a, waves, grid, lds_bytes, st)
                         : launch_decode_w<false>(a, waves, grid, lds_bytes, st);
 }
