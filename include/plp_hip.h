@@ -204,8 +204,14 @@ int plp_engine_reset(plp_engine *eng);
 
 /* Enqueue n_passes more passes for every decoder (asynchronous). */
 int plp_engine_run(plp_engine *eng, int32_t n_passes);
-/* Wait for the engine's work; returns PLP_E_EPCAP if any decoder overflowed
- * (the affected rows carry err; see plp_engine_config.ep_cap). */
+/* Wait for the engine's work.  The host reads one word, the count of
+ * decoders that ended in an error; only when it is non-zero are the decoder
+ * states copied over, and decoders whose extreme-point list outgrew the
+ * capacity are rerun from their seeds at the largest capacity a block holds
+ * (results do not depend on it).  Returns PLP_E_EPCAP if a decoder needs
+ * more than that (the affected rows carry err; see
+ * plp_engine_config.ep_cap).  Replaces the error check after the launch at
+ * plp_gpu.cu:1417-1422. */
 int plp_engine_sync(plp_engine *eng);
 /* Largest placement count of any decoder (row stride for fetch). */
 int plp_engine_max_items(plp_engine *eng, int32_t *max_items);
