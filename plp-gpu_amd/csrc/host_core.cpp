@@ -3,102 +3,162 @@
 
 #include <algorithm>
 #include <charconv>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
-#include <sstream>
+#include <iterator>
+#include <string_view>
 
 namespace plp {
 
 namespace {
 
-std::vector<std::vector<std::string>> tokenize_lines(const std::string& text) {
-    std::vector<std::vector<std::string>> out;
-    std::istringstream in(text);
-    std::string line;
-    while (std::getline(in, line)) {
-        std::istringstream ls(line);
-        std::vector<std::string> t;
-        std::string w;
-        while (ls >> w) t.push_back(w);
-        if (!t.empty()) out.push_back(std::move(t));
+// The thpack text read as records: each record is one line that holds at
+// least one field, its fields being the runs of non-whitespace characters
+// (the blank and whitespace-only lines carry nothing and are passed over).
+// One record of look-ahead, so the grammar can ask whether more follow.
+class Records {
+public:
+    explicit Records(std::string_view text) : rest_(text) { advance(); }
+    bool more() const { return have_; }
+    const std::vector<std::string_view>& peek() const { return cur_; }
+    // Current record, then move past it.
+    std::vector<std::string_view> take() {
+        std::vector<std::string_view> r;
+        r.swap(cur_);
+        advance();
+        return r;
     }
-    return out;
+
+private:
+    static bool blank(char c) {
+        return c == ' ' || c == '\t' || c == '\n' || c == '\v' || c == '\f' || c == '\r';
+    }
+    void advance() {
+        have_ = false;
+        cur_.clear();
+        while (!have_ && !rest_.empty()) {
+            const size_t eol = rest_.find('\n');
+            std::string_view line = rest_.substr(0, eol);
+            rest_ = eol == std::string_view::npos ? std::string_view() : rest_.substr(eol + 1);
+            size_t i = 0;
+            while (i < line.size()) {
+                while (i < line.size() && blank(line[i])) i++;
+                const size_t start = i;
+                while (i < line.size() && !blank(line[i])) i++;
+                if (i > start) cur_.push_back(line.substr(start, i - start));
+            }
+            have_ = !cur_.empty();
+        }
+    }
+    std::string_view rest_;
+    std::vector<std::string_view> cur_;
+    bool have_ = false;
+};
+
+[[noreturn]] void reject(const std::string& why) { throw InstanceFormatError(why); }
+
+// A decimal integer field, read the way C's strtoll(s, &end, 10) reads it
+// (optional sign, digits, saturation at the long long range); anything else
+// in the field is an error naming the field and its role.
+long long field_value(std::string_view f, const char* role) {
+    f = f.substr(0, f.find('\0'));   // strtoll stops at a NUL byte as at the field's end
+    size_t i = 0;
+    const bool neg = i < f.size() && f[i] == '-';
+    if (i < f.size() && (f[i] == '-' || f[i] == '+')) i++;
+    const size_t first_digit = i;
+    unsigned long long mag = 0;
+    bool saturated = false;
+    const unsigned long long limit =
+        neg ? (unsigned long long)LLONG_MAX + 1ull : (unsigned long long)LLONG_MAX;
+    for (; i < f.size() && f[i] >= '0' && f[i] <= '9'; i++) {
+        const unsigned digit = (unsigned)(f[i] - '0');
+        if (saturated || mag > (limit - digit) / 10) saturated = true;
+        else mag = mag * 10 + digit;
+    }
+    if (i == first_digit || i != f.size())
+        reject("not an integer: " + std::string(f) + " (" + role + ")");
+    if (saturated) mag = limit;
+    return neg ? (long long)(0ull - mag) : (long long)mag;
 }
 
-long long to_int(const std::string& s, const char* what) {
-    char* end = nullptr;
-    long long v = std::strtoll(s.c_str(), &end, 10);
-    if (end == s.c_str() || *end != '\0')
-        throw InstanceFormatError(std::string("not an integer: ") + s + " (" + what + ")");
-    return v;
+bool is_flag(std::string_view f) { return f == "0" || f == "1"; }
+
+// The record the grammar expects next: present and with a field count that
+// the rule accepts, else the rule's message.
+template <class Fits>
+std::vector<std::string_view> expect(Records& rec, Fits fits, const char* absent,
+                                     const char* wrong_shape) {
+    if (!rec.more()) reject(absent);
+    if (!fits(rec.peek().size())) reject(wrong_shape);
+    return rec.take();
+}
+
+// One item line: "k l lf w wf h hf qty", k counting from 1.
+Item item_record(Records& rec, long long k) {
+    const auto f = expect(rec, [](size_t n) { return n == 8; }, "instance truncated",
+                          "item line must have 8 tokens");
+    if (field_value(f[0], "item index") != k) reject("item index must be sequential");
+    long long dim[3];
+    for (int a = 0; a < 3; a++) dim[a] = field_value(f[1 + 2 * a], "item dimension");
+    if (dim[0] <= 0 || dim[1] <= 0 || dim[2] <= 0) reject("item dimensions must be positive");
+    if (!is_flag(f[2]) || !is_flag(f[4]) || !is_flag(f[6]))
+        reject("orientation flags must be 0/1");
+    const long long qty = field_value(f[7], "item qty");
+    if (qty < 0) reject("item qty must be >= 0");
+    return Item{(int)dim[0], (int)dim[1], (int)dim[2], f[2] == "1", f[4] == "1", f[6] == "1",
+                (int)qty};
+}
+
+// One problem: "id [seed]", "L W H", "n", then n item lines.
+Instance problem_records(Records& rec) {
+    expect(rec, [](size_t n) { return n <= 2; }, "malformed problem line", "malformed problem line");
+    const auto box = expect(rec, [](size_t n) { return n == 3; },
+                            "container line must have 3 dimensions",
+                            "container line must have 3 dimensions");
+    long long side[3];
+    for (int a = 0; a < 3; a++) side[a] = field_value(box[(size_t)a], "container dimension");
+    if (side[0] <= 0 || side[1] <= 0 || side[2] <= 0)
+        reject("container dimensions must be positive");
+    const auto count = expect(rec, [](size_t n) { return n == 1; },
+                              "type-count line must be a single integer",
+                              "type-count line must be a single integer");
+    const long long n_types = field_value(count[0], "type count");
+    Instance inst;
+    inst.L = (int)side[0];
+    inst.W = (int)side[1];
+    inst.H = (int)side[2];
+    for (long long k = 1; k <= n_types; k++) inst.items.push_back(item_record(rec, k));
+    return inst;
 }
 
 }  // namespace
 
-// Format and errors of parser.py:67-139; token rules of plp_common.hpp:68-141.
+// Grammar of the thpack "original" format with the reference's error
+// messages (plp/parser.py:67-139; messages and check order as
+// gpu/plp_common.hpp:68-141 reports them): a count line whose first field is
+// the number of problems P >= 1, P problems, nothing after them.
 std::vector<Instance> parse(const std::string& text) {
-    auto toks = tokenize_lines(text);
-    if (toks.empty()) throw InstanceFormatError("empty file");
-    long long P = to_int(toks[0][0], "instance count");
-    if (P <= 0 || toks.size() < 2) throw InstanceFormatError("instance count must be >= 1");
+    Records rec{std::string_view(text)};
+    if (!rec.more()) reject("empty file");
+    const long long n_problems = field_value(rec.take()[0], "instance count");
+    if (n_problems <= 0 || !rec.more()) reject("instance count must be >= 1");
     std::vector<Instance> out;
-    size_t pos = 1;
-    for (long long p = 0; p < P; p++) {
-        if (pos >= toks.size() || toks[pos].size() > 2)
-            throw InstanceFormatError("malformed problem line");
-        pos++;
-        if (pos >= toks.size() || toks[pos].size() != 3)
-            throw InstanceFormatError("container line must have 3 dimensions");
-        long long L = to_int(toks[pos][0], "container dimension");
-        long long W = to_int(toks[pos][1], "container dimension");
-        long long H = to_int(toks[pos][2], "container dimension");
-        if (L <= 0 || W <= 0 || H <= 0)
-            throw InstanceFormatError("container dimensions must be positive");
-        pos++;
-        if (pos >= toks.size() || toks[pos].size() != 1)
-            throw InstanceFormatError("type-count line must be a single integer");
-        long long n = to_int(toks[pos][0], "type count");
-        pos++;
-        Instance inst;
-        inst.L = (int)L;
-        inst.W = (int)W;
-        inst.H = (int)H;
-        for (long long k = 1; k <= n; k++) {
-            if (pos >= toks.size()) throw InstanceFormatError("instance truncated");
-            const auto& t = toks[pos++];
-            if (t.size() != 8) throw InstanceFormatError("item line must have 8 tokens");
-            if (to_int(t[0], "item index") != k)
-                throw InstanceFormatError("item index must be sequential");
-            long long dl = to_int(t[1], "item dimension");
-            long long dw = to_int(t[3], "item dimension");
-            long long dh = to_int(t[5], "item dimension");
-            if (dl <= 0 || dw <= 0 || dh <= 0)
-                throw InstanceFormatError("item dimensions must be positive");
-            for (int f : {2, 4, 6})
-                if (t[f] != "0" && t[f] != "1")
-                    throw InstanceFormatError("orientation flags must be 0/1");
-            long long qty = to_int(t[7], "item qty");
-            if (qty < 0) throw InstanceFormatError("item qty must be >= 0");
-            inst.items.push_back({(int)dl, (int)dw, (int)dh, t[2] == "1", t[4] == "1",
-                                  t[6] == "1", (int)qty});
-        }
-        out.push_back(std::move(inst));
-    }
-    if (pos != toks.size()) throw InstanceFormatError("trailing data after last instance");
+    for (long long p = 0; p < n_problems; p++) out.push_back(problem_records(rec));
+    if (rec.more()) reject("trailing data after last instance");
     return out;
 }
 
 Instance load_instance(const std::string& path) {
-    std::ifstream f(path);
-    if (!f) throw InstanceFormatError("cannot open " + path);
-    std::ostringstream ss;
-    ss << f.rdbuf();
-    auto v = parse(ss.str());
-    if (v.size() != 1)
-        throw InstanceFormatError("expected exactly 1 instance, got " + std::to_string(v.size()));
-    return v[0];
+    std::ifstream in(path, std::ios::binary);
+    if (!in) reject("cannot open " + path);
+    const std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    std::vector<Instance> all = parse(text);
+    if (all.size() != 1)
+        reject("expected exactly 1 instance, got " + std::to_string(all.size()));
+    return std::move(all.front());
 }
 
 std::vector<std::array<int, 3>> orientations(const Item& it) {
