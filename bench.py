@@ -1,21 +1,30 @@
 #!/usr/bin/env python3
 """Decoded-packings/s benchmark of the MI355X pallet-loading decoder.
 
-Workload (one GPU): the 20-type instance (587x233x220, dims U[25,115], stand
-flags p=0.7, ~1.2x container volume) decoded by K = 1024 device-resident
-decoders seeded 1..1024 with eps-greedy eps = 0.01, theta = 0.9; one step is
-P = 4 passes of every decoder (4096 packings) with D4' selection on the
-device.  Instance tables and decoder state are resident in HBM before the
-timed region; every step resets the decoders, runs, synchronises and checks
-the per-decoder error flags.
+Headline workload (one GPU): 256 synthetic instances (3..100 item types
+each, dims U[25,115], ~1.2x the 587x233x220 container) x 16 seeds = 4096
+device-resident decoders in one persistent launch captured in a hipGraph,
+launch shape autotuned (waves per decoder x extreme-point capacity x blocks
+per CU), eps-greedy eps = 0.01, theta = 0.9, one pass per decoder.  Instance
+tables and decoder state are resident in HBM before the timed region.  One
+step = reset + run + synchronise + plp_engine_fetch of every decoder's row
+and placements into host arrays (the decoders write them into pinned host
+memory while they run).  After timing, the digest of the last step's
+outputs must equal the one tests/golden/fullsize_digests.json pins (the C
+oracle and the reference's g++ build agree on every decoder there).
 
 Printed: one JSON line with the contract fields plus
-  roofline      dominant kernel (decode_kernel) against the VALU issue roof
-  cpu_baseline  the reference's g++ build (or the C oracle) on host cores
-  extras        plain one-step-per-launch GPU form, the 256-instance batch,
-                the 100-type / 2048-decoder instance, the block-building engine
-                (GPU and reference CPU), JSON emission, Poisson serving
-usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu] [--no-extras]
+  roofline      decode_kernel against the VALU issue roof: the live
+                pair x box tests it evaluates (counted on the device) x 10
+                lane-ops per test / kernel time; the reference's brute-force
+                test count over the same decoders is work_avoided times that
+  cpu_baseline  the reference's g++ build (oracle/_ref/ref_driver) on the
+                same 4096 decoders over the host cores of this job
+  extras        the 20-type instance x 1024 decoders x 4 passes, the 100-type
+                instance (2048 decoders) in 587x233x220 and in 880x349x330,
+                the plain one-step-per-launch GPU form, the block-building
+                engine (GPU and reference CPU), JSON emission, Poisson serving
+usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu] [--no-extras]
 """
 from __future__ import annotations
 
@@ -24,62 +33,90 @@ import json
 import os
 import subprocess
 import sys
+import tempfile
 import time
+
+import numpy as np
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "plp-gpu_amd"))
 
-K_DEC = 1024
-PASSES = 4
+HEADLINE = "mixed_256x16"
 EPS, THETA = 0.01, 0.9
-# Algorithmic unit: one pair x box test of the reference predicate
-# (plp_common.hpp:297-321) over every (candidate, orientation, placed box) the
-# reference's check_all receives (plp_common.hpp:418), at 10 lane-ops per test
-# (C1 6 compares + ANDs; C3 1 compare + 2 min + 2 max + 2 sub + 1 mul + 1 add).
+# One pair x box test of the reference predicate (plp_common.hpp:297-321):
+# C1 6 compares + ANDs; C3 1 compare + 2 min + 2 max + 2 sub + 1 mul + 1 add.
 OPS_PER_TEST = 10
 # MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz VALU issue = 39.3 T lane-ops/s
 # (= FP64 vector 78.6 TFLOP/s counting an FMA as two flops); HBM3E 8 TB/s.
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 HBM_PEAK_GBS = 8000.0
+PINNED = os.path.join(REPO, "tests", "golden", "fullsize_digests.json")
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(text: str, threads: int, n: int, passes: int) -> dict | None:
-    """The reference g++ build (oracle/_ref/ref_driver) or, when it was not
-    built, the C oracle: n decoders x `passes` passes of the same workload
-    (1024 x 12 on 16 threads and 64 x 4 on one are each about 5-10 s)."""
-    inst_path = os.path.join(REPO, f"_bench_multi_{os.getpid()}.txt")
-    with open(inst_path, "w") as f:
-        f.write(text)
+def host_cores() -> dict:
+    """CPUs this job may run on: the affinity mask, the cgroup quota if any,
+    and nproc as the OS reports it."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()
+            if q != "max":
+                quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    usable = aff if quota is None else max(1, min(aff, int(quota)))
+    return {"nproc": os.cpu_count(), "affinity": aff, "cgroup_quota": quota, "usable": usable}
+
+
+def cpu_baseline(texts, seeds_per_inst: int, threads: int, label: str) -> dict | None:
+    """The reference g++ build (oracle/_ref/ref_driver bench: each instance
+    decoded by seeds 1..seeds_per_inst, one pass) on `threads` threads; where
+    it was not built, the C oracle (tests/oracle_lib.py) on a thread pool."""
     ref = os.path.join(REPO, "oracle", "_ref", "ref_driver")
-    port = os.path.join(REPO, "oracle", "build", "plp_oracle")
+    n_dec = len(texts) * seeds_per_inst
     try:
         if os.path.exists(ref):
-            cmd = [ref, "bench", inst_path, "--K", str(n), "--passes", str(passes), "--seed", "1",
-                   "--eps", str(EPS), "--theta", str(THETA), "--threads", str(threads)]
-            kind = "reference"
+            tmp = tempfile.mkdtemp(prefix="plp_bench_", dir=REPO)
+            paths = []
+            try:
+                for i, t in enumerate(texts):
+                    paths.append(os.path.join(tmp, f"i{i:04d}.txt"))
+                    with open(paths[-1], "w") as f:
+                        f.write(t)
+                out = json.loads(subprocess.run(
+                    [ref, "bench", *paths, "--K", str(seeds_per_inst), "--passes", "1", "--seed",
+                     "1", "--eps", str(EPS), "--theta", str(THETA), "--threads", str(threads)],
+                    check=True, capture_output=True, text=True, timeout=600).stdout)
+            finally:
+                for pth in paths:
+                    os.remove(pth)
+                os.rmdir(tmp)
+            kind, secs = "reference", out["seconds"]
         else:
-            if not os.path.exists(port):
-                subprocess.run(["make", "-C", os.path.join(REPO, "oracle")], check=True,
-                               capture_output=True)
-            cmd = [port, inst_path, "--K", str(n), "--passes", str(passes), "--seed", "1", "--eps",
-                   str(EPS), "--theta", str(THETA), "--threads", str(threads), "--bench"]
-            kind = "port"
-        out = json.loads(subprocess.run(cmd, check=True, capture_output=True, text=True,
-                                        timeout=600).stdout)
+            from concurrent.futures import ThreadPoolExecutor
+            sys.path.insert(0, os.path.join(REPO, "tests"))
+            from oracle_lib import Oracle
+            orc = Oracle()
+            insts = [orc.parse(t)[0] for t in texts]
+            jobs = [(i, s) for i in range(len(texts)) for s in range(1, seeds_per_inst + 1)]
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(threads) as ex:
+                list(ex.map(lambda j: orc.run(insts[j[0]], seed=j[1], passes=1, eps=EPS,
+                                              theta=THETA), jobs))
+            kind, secs = "port", time.perf_counter() - t0
     except Exception as e:  # noqa: BLE001 -- a missing CPU baseline is reported, not fatal
         log(f"cpu_baseline unavailable: {e}")
         return None
-    finally:
-        os.remove(inst_path)
-    return {"value": round(out["packings_per_s"], 3), "unit": "packings/s", "cores": threads,
+    return {"value": round(n_dec / secs, 2), "unit": "packings/s", "cores": threads,
             "kind": kind,
-            "sample": f"{n} decoders (seeds 1..{n}) x {passes} passes of the same 20-type "
-                      f"instance, eps {EPS}, theta {THETA}; {out['seconds']:.2f} s wall on "
-                      f"{threads} threads"}
+            "sample": f"{label}: {len(texts)} instances x seeds 1..{seeds_per_inst} "
+                      f"({n_dec} decoders, one pass, eps {EPS}, theta {THETA}); {secs:.2f} s "
+                      f"wall on {threads} threads"}
 
 
 def block_cpu_baseline(text: str, threads: int, n: int = 1024, passes: int = 1200) -> dict | None:
@@ -169,16 +206,78 @@ def serve(plp_amd, synth, seconds: float, rate: float, passes: int, max_inflight
             "max_ms": round(1e3 * lat[-1], 2)}
 
 
+def load_pinned() -> dict:
+    with open(PINNED) as f:
+        return json.load(f)["workloads"]
+
+
+def timed_workload(plp_amd, synth, ctx, name: str, steps: int, warmup: int, shape: str = "",
+                   blocks=None) -> dict:
+    """Decoded packings/s of a named workload: autotuned (or fixed) launch
+    shape, graph-captured launch; each timed step resets, runs, synchronises
+    and fetches every decoder's row and placements into preallocated host
+    arrays.  Returns the figures, the engine and the last step's outputs."""
+    texts, dec_inst, seeds, passes = synth.workload(name)
+    insts = [plp_amd.Instance.from_text(t) for t in texts]
+    prm = plp_amd.params(eps=EPS, theta=THETA)
+    if shape:
+        wpd, cap, bpc = (int(v) for v in shape.split(","))
+        eng = plp_amd.Engine(ctx, ep_cap=cap, blocks_per_cu=bpc, use_graph=True,
+                             waves_per_decoder=wpd)
+        eng.load(insts, dec_inst, seeds, prm, blocks=blocks)
+        tuned = {"waves_per_decoder": wpd, "ep_cap": cap, "blocks_per_cu": bpc, "fixed": True}
+    else:
+        eng = plp_amd.Engine(ctx, use_graph=True)
+        eng.load(insts, dec_inst, seeds, prm, blocks=blocks)
+        log(f"{name}: tuning launch shape")
+        tuned, tune_ms = eng.autotune(passes)
+        log(f"{name}: autotune {tuned} ({tune_ms:.3f} ms per launch)")
+    n = len(seeds)
+    rows = (plp_amd.RowResult * n)()
+    pl = np.zeros((n, eng.max_items(), 7), np.int32)
+    for _ in range(warmup):
+        eng.reset()
+        eng.run(passes)
+        eng.sync()
+        eng.fetch(out=(rows, pl))
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.reset()
+        eng.run(passes)
+        eng.sync()
+        eng.fetch(out=(rows, pl))
+        kernel_ms.append(eng.last_ms())
+    elapsed = time.perf_counter() - t0
+    assert all(r.err == 0 for r in rows)
+    digest, _ = plp_amd.results_digest(rows, pl)
+    return {"name": name, "texts": texts, "insts": insts, "dec_inst": dec_inst, "seeds": seeds,
+            "passes": passes, "eng": eng, "tuned": tuned, "steps": steps,
+            "packings_per_s": n * passes * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+            "kernel_ms": sum(kernel_ms) / len(kernel_ms), "digest": digest}
+
+
+def parity_note(res: dict, pinned: dict) -> dict:
+    pin = pinned.get(res["name"])
+    ok = pin is not None and pin["digest"] == res["digest"]
+    if not ok:
+        raise SystemExit(f"{res['name']}: timed outputs differ from the pinned digest "
+                         f"({res['digest']} vs {pin and pin['digest']})")
+    return {"digest": res["digest"], "matches_pinned": True, "decoders": len(res["seeds"])}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--workload", default=HEADLINE,
+                    help="headline workload (plp_amd.synth.workload names)")
     ap.add_argument("--shape", default="",
                     help="fixed launch shape WPD,EP_CAP,BLOCKS_PER_CU instead of autotuning")
-    ap.add_argument("--serve-seconds", type=float, default=10.0,
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="length of the Poisson serving run in extras (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-inflight", type=int, default=8)
@@ -202,58 +301,33 @@ def main() -> int:
     import plp_amd
     from plp_amd import synth
 
-    text = synth.multi_instance()
-    inst = plp_amd.Instance.from_text(text)
+    pinned = load_pinned()
     ctx = plp_amd.Ctx(0)
     prm = plp_amd.params(eps=EPS, theta=THETA)
-    if a.shape:
-        wpd, cap, bpc = (int(v) for v in a.shape.split(","))
-        eng = plp_amd.Engine(ctx, ep_cap=cap, blocks_per_cu=bpc, use_graph=True,
-                             waves_per_decoder=wpd)
-        eng.load([inst], [0] * K_DEC, list(range(1, K_DEC + 1)), prm)
-        tuned = {"waves_per_decoder": wpd, "ep_cap": cap, "blocks_per_cu": bpc, "fixed": True}
-    else:
-        eng = plp_amd.Engine(ctx, use_graph=True)
-        eng.load([inst], [0] * K_DEC, list(range(1, K_DEC + 1)), prm)
-        log("tuning launch shape")
-        tuned, tune_ms = eng.autotune(PASSES)
-        log(f"autotune: {tuned} ({tune_ms:.2f} ms/pass)")
-    for _ in range(a.warmup):
-        eng.reset()
-        eng.run(PASSES)
-        eng.sync()
-    kernel_ms = []
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        eng.reset()
-        eng.run(PASSES)
-        eng.sync()
-        kernel_ms.append(eng.last_ms())
-    elapsed = time.perf_counter() - t0
-    rows, _ = eng.fetch(placements=False)
-    assert all(r.err == 0 for r in rows)
+    head = timed_workload(plp_amd, synth, ctx, a.workload, a.steps, a.warmup, a.shape)
+    parity = parity_note(head, pinned)
+    eng = head["eng"]
     shape = eng.describe()
-    # The reference's pair x box tests of the same step, from one untimed run
-    # of an engine that also keeps the dead extreme points (same results).
+    n_dec, passes = len(head["seeds"]), head["passes"]
+    # The pair x box tests of the same decoders, from one untimed run of an
+    # engine that also keeps the dead extreme points (same results): the live
+    # tests this kernel evaluates, and the reference's brute-force count.
     ec = plp_amd.Engine(ctx, waves_per_decoder=shape["waves_per_decoder"], count_work=True)
-    ec.load([inst], [0] * K_DEC, list(range(1, K_DEC + 1)), prm)
-    ec.run(PASSES)
+    ec.load(head["insts"], head["dec_inst"], head["seeds"], prm)
+    ec.run(passes)
     ec.sync()
-    crows, _ = ec.fetch(placements=False)
+    crows, cpl = ec.fetch()
     ec.close()
-    assert [(r.n_boxes, r.volume, r.score) for r in crows] == \
-        [(r.n_boxes, r.volume, r.score) for r in rows]
-    work = sum(r.work_pair_box for r in crows)
-    live = sum(r.work_live_pair_box for r in crows)    # the tests this algorithm needs
-    packings = K_DEC * PASSES * a.steps
-    value = packings / elapsed
-    ms_launch = sum(kernel_ms) / len(kernel_ms)
-    achieved = work * OPS_PER_TEST / (ms_launch * 1e-3) / 1e12
-    # Per-launch hardware counters of the same launch shape, from the
+    assert plp_amd.results_digest(crows, cpl)[0] == head["digest"]
+    ref_tests = sum(r.work_pair_box for r in crows)
+    live = sum(r.work_live_pair_box for r in crows)
+    ms_launch = head["kernel_ms"]
+    achieved = live * OPS_PER_TEST / (ms_launch * 1e-3) / 1e12
+    # Per-launch hardware counters of this workload and launch shape, from the
     # committed rocprofv3 summary (FETCH_SIZE / WRITE_SIZE scaled by the copy
     # calibration; SQ_INSTS_VALU wave instructions x 64 lanes).
     prof = {}
-    pfile = os.path.join(REPO, "profiles", "decode_kernel_counters.json")
+    pfile = os.path.join(REPO, "profiles", f"decode_kernel_{a.workload}_counters.json")
     if os.path.exists(pfile):
         with open(pfile) as f:
             prof = json.load(f)
@@ -261,29 +335,35 @@ def main() -> int:
     executed = None
     if prof.get("valu_insts_per_launch"):
         executed = prof["valu_insts_per_launch"] * 64 / (ms_launch * 1e-3) / 1e12
+    texts = head["texts"]
     line = {
         "metric": "decoded packings/s at bit-parity vs plain GPU & g++ host; p50/p99 latency",
-        "value": round(value, 1),
+        "value": round(head["packings_per_s"], 1),
         "unit": "packings/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(1e3 * elapsed / a.steps, 3),
+        "ms_per_step": round(head["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64+i32",
         "data": "synthetic",
-        "config": {"workload": "20-type instance 587x233x220, K=1024 decoders seeds 1..1024, "
-                               "eps 0.01, theta 0.9, 4 passes per step",
-                   "decoders": K_DEC, "passes_per_step": PASSES, "parallelism": "none (1 GPU)",
-                   "launch": shape, "autotuned": tuned},
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
-                     "unit": "Tlane-op/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
-                     "traffic": traffic,
-                     "kernel": "decode_kernel", "kernel_ms": round(ms_launch, 3),
-                     "reference_pair_box_tests_per_launch": work, "ops_per_test": OPS_PER_TEST,
-                     "live_pair_box_tests_per_launch": live,
+        "config": {"workload": f"{a.workload}: {len(texts)} instance(s), {n_dec} decoders "
+                               f"(plp_amd.synth.workload), eps {EPS}, theta {THETA}, "
+                               f"{passes} pass(es) per decoder per step, one persistent "
+                               f"hipGraph launch; step = reset + run + sync + fetch of every "
+                               f"row and placement",
+                   "decoders": n_dec, "passes_per_step": passes, "parallelism": "none (1 GPU)",
+                   "launch": shape, "autotuned": head["tuned"]},
+        "parity": parity,
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3),
+                     "peak": round(VALU_PEAK_TOPS, 2), "unit": "Tlane-op/s",
+                     "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": traffic,
+                     "kernel": "decode_kernel", "kernel_ms": round(ms_launch, 4),
+                     "live_pair_box_tests_per_launch": live, "ops_per_test": OPS_PER_TEST,
+                     "reference_pair_box_tests_per_launch": ref_tests,
+                     "work_avoided": round(ref_tests / max(live, 1), 2),
                      "executed_valu_tops": None if executed is None else round(executed, 3),
                      "executed_valu_frac": None if executed is None
                      else round(executed / VALU_PEAK_TOPS, 4),
@@ -291,72 +371,57 @@ def main() -> int:
                      else round(traffic / (ms_launch * 1e-3) / 1e9 / HBM_PEAK_GBS, 6),
                      "counters_from": prof.get("source")},
     }
+    extras = {}
+    cores = host_cores()
     if not a.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline on {threads} threads")
-        line["cpu_baseline"] = cpu_baseline(text, threads, n=K_DEC, passes=12)
-        single = cpu_baseline(text, 1, n=64, passes=4)
+        threads = min(16, cores["usable"])
+        log(f"cpu baseline on {threads} threads ({cores})")
+        ins_per, seeds_per = (len(texts), len(head["seeds"]) // len(texts))
+        line["cpu_baseline"] = cpu_baseline(texts, seeds_per, threads, a.workload)
+        if line["cpu_baseline"] is not None:
+            line["cpu_baseline"]["host"] = cores
+        single = cpu_baseline(texts[:max(1, ins_per // 16)], seeds_per, 1, a.workload)
         if single is not None:
-            line.setdefault("extras", {})["cpu_single_core"] = single
+            extras["cpu_single_core"] = single
+    eng.close()
     if not a.no_extras:
-        extras = line.get("extras", {})
-        # Plain one-step-per-launch form, one decoder per launch sequence.
+        for name in ("twenty_1024x4", "hundred_2048", "hundred_fit_2048"):
+            if name == a.workload:
+                continue
+            r = timed_workload(plp_amd, synth, ctx, name, 5 if name != "hundred_fit_2048" else 3,
+                               1)
+            extras[name] = {"packings_per_s": round(r["packings_per_s"], 1),
+                            "ms_per_step": round(r["ms_per_step"], 3),
+                            "kernel_ms": round(r["kernel_ms"], 3),
+                            "launch": r["eng"].describe(), "autotuned": r["tuned"],
+                            "parity": parity_note(r, pinned)}
+            r["eng"].close()
+        # Plain one-step-per-launch form: the host decode loop with one
+        # feasibility launch per item step (the reference's CudaBackend).
+        text = synth.multi_instance()
+        inst = plp_amd.Instance.from_text(text)
         n_plain = 8
         t0 = time.perf_counter()
-        for s in range(n_plain):
-            ctx.plain_run(inst, 1 + s, 1, prm)
+        for s_ in range(n_plain):
+            ctx.plain_run(inst, 1 + s_, 1, prm)
         extras["plain_gpu_packings_per_s"] = round(n_plain / (time.perf_counter() - t0), 2)
-        # 256 uneven instances x 16 seeds in one persistent launch.
-        texts = synth.mixed_batch(256)
-        insts = [plp_amd.Instance.from_text(t) for t in texts]
-        e2 = plp_amd.Engine(ctx, use_graph=True)
-        e2.load(insts, [i for i in range(256) for _ in range(16)],
-                [1 + s for _ in range(256) for s in range(16)], prm)
-        extras["batch_256x16_autotuned"], _ = e2.autotune(1)
-        e2.run(1)
-        e2.sync()
-        t0 = time.perf_counter()
-        for _ in range(3):
-            e2.reset()
-            e2.run(1)
-            e2.sync()
-        extras["batch_256x16_packings_per_s"] = round(3 * 4096 / (time.perf_counter() - t0), 1)
-        extras["batch_256x16_kernel_ms"] = round(e2.last_ms(), 3)
-        e2.close()
-        # 100 types, ~600 boxes, 2048 decoders, one pass.
-        e3 = plp_amd.Engine(ctx, use_graph=True)
-        e3.load([plp_amd.Instance.from_text(synth.large_instance())], [0] * 2048,
-                list(range(1, 2049)), prm)
-        extras["large_100types_2048_autotuned"], _ = e3.autotune(1)
-        e3.run(1)
-        e3.sync()
-        t0 = time.perf_counter()
-        for _ in range(3):
-            e3.reset()
-            e3.run(1)
-            e3.sync()
-        extras["large_100types_2048_packings_per_s"] = round(3 * 2048 / (time.perf_counter() - t0), 1)
-        extras["large_100types_2048_kernel_ms"] = round(e3.last_ms(), 3)
-        e3.close()
-        # Block-building engine (blockproof/plp_block.cpp) on the same instance
-        # and seeds, reference block options, 4 passes per decoder.
+        # Block-building engine (blockproof/plp_block.cpp) on the 20-type
+        # instance, reference block options, 4 passes per decoder.
         eb = plp_amd.Engine(ctx, use_graph=True)
-        eb.load([inst], [0] * K_DEC, list(range(1, K_DEC + 1)), prm,
-                blocks=plp_amd.block_params())
-        eb.run(PASSES)
+        eb.load([inst], [0] * 1024, list(range(1, 1025)), prm, blocks=plp_amd.block_params())
+        eb.run(4)
         eb.sync()
         t0 = time.perf_counter()
         for _ in range(3):
             eb.reset()
-            eb.run(PASSES)
+            eb.run(4)
             eb.sync()
-        extras["block_engine_packings_per_s"] = round(3 * K_DEC * PASSES /
-                                                      (time.perf_counter() - t0), 1)
+            eb.fetch()
+        extras["block_engine_packings_per_s"] = round(3 * 1024 * 4 / (time.perf_counter() - t0), 1)
         extras["block_engine_kernel_ms"] = round(eb.last_ms(), 3)
         eb.close()
         if not a.no_cpu:
-            extras["block_engine_cpu_baseline"] = block_cpu_baseline(text, min(16, os.cpu_count()
-                                                                              or 1))
+            extras["block_engine_cpu_baseline"] = block_cpu_baseline(text, min(16, cores["usable"]))
         # JSON wire format: one 4096-decoder request through `plp_gpu --serve`;
         # format_ms is the host time to write its 4096 reference-format rows.
         cli = os.path.join(REPO, "plp-gpu_amd", "build", "plp_gpu")
@@ -373,9 +438,15 @@ def main() -> int:
         if a.serve_seconds > 0:
             extras["serving"] = serve(plp_amd, synth, a.serve_seconds, a.serve_rate, 1,
                                       a.serve_inflight)
-        line["extras"] = extras
+    v = line["value"]
+    if extras.get("plain_gpu_packings_per_s"):
+        extras["x_plain_gpu"] = round(v / extras["plain_gpu_packings_per_s"], 1)
+    if line.get("cpu_baseline"):
+        extras["x_cpu_all_cores"] = round(v / line["cpu_baseline"]["value"], 1)
+    if extras.get("cpu_single_core"):
+        extras["x_cpu_single_core"] = round(v / extras["cpu_single_core"]["value"], 1)
+    line["extras"] = extras
     print(json.dumps(line), flush=True)
-    eng.close()
     ctx.close()
     return 0
 

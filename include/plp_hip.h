@@ -216,8 +216,20 @@ int plp_engine_sync(plp_engine *eng);
 /* Largest placement count of any decoder (row stride for fetch). */
 int plp_engine_max_items(plp_engine *eng, int32_t *max_items);
 /* Copy results: rows[n_dec]; placements (may be NULL) gets max_items * 7
- * ints per decoder, the selected packing in JSON row order. */
+ * ints per decoder, the selected packing in JSON row order (rows past a
+ * decoder's n_boxes are left as they were).  Replaces the slot-slab read of
+ * CudaDeviceBackend::fetch (gpu/plp_gpu.cu:1353-1379).  The decoders write
+ * their results into the engine's pinned host memory while they run, so this
+ * is a host copy after the stream synchronises, with no transfer of its own. */
 int plp_engine_fetch(plp_engine *eng, plp_row_result *rows, int32_t *placements);
+/* The same results without a copy: *rows points at n_dec records and
+ * *placements (may be NULL) at n_dec * row_stride ints (decoder d's packing
+ * at d * row_stride, JSON row order), both in the engine's pinned host
+ * memory, valid until the next load, reset, run or destroy.  No reference
+ * counterpart: the reference copies its slot slab into host vectors
+ * (gpu/plp_gpu.cu:1353-1379). */
+int plp_engine_results(plp_engine *eng, const plp_row_result **rows, const int32_t **placements,
+                       int32_t *row_stride);
 /* Milliseconds of device time of the last plp_engine_run (HIP events on the
  * engine's stream). */
 int plp_engine_last_ms(plp_engine *eng, float *ms);

@@ -13,7 +13,9 @@
 
 /* shape_precedence, plp_block.cpp:38-45: most copies first, then tallest
  * stack, then widest base, ties by orientation index. */
-static const orc_shape *g_cat;
+/* per thread: the comparators below read them, and callers may run
+ * decoders on several threads at once */
+static _Thread_local const orc_shape *g_cat;
 
 static int cmp_precedence(const void *pa, const void *pb) {
     const orc_shape *x = &g_cat[*(const int *)pa], *y = &g_cat[*(const int *)pb];
@@ -38,7 +40,7 @@ static int cmp_place(const void *pa, const void *pb) {
     return 0;
 }
 
-static const orc_instance *g_inst;
+static _Thread_local const orc_instance *g_inst;
 
 /* Type order of the plan, plp_block.cpp:134-138: item volume desc, index asc. */
 static int cmp_type_volume(const void *pa, const void *pb) {

@@ -443,14 +443,12 @@ out:
 
 /* --------------------------------------------------------------------- FSR */
 
-static const int *g_fsr_placed;
-
-/* (z desc, index asc) == Python sorted(range(n), key=-z), stable. */
-static int cmp_fsr_order(const void *a, const void *b) {
-    int i = *(const int *)a, j = *(const int *)b;
-    int zi = g_fsr_placed[7 * i + 2], zj = g_fsr_placed[7 * j + 2];
-    if (zi != zj) return zi > zj ? -1 : 1;
-    return i < j ? -1 : (i > j);
+/* (z desc, index asc) == Python sorted(range(n), key=-z), stable: keys
+ * (INT_MAX - z) << 32 | index sort ascending into that order (no shared
+ * state, so concurrent calls are safe). */
+static int cmp_u64(const void *a, const void *b) {
+    unsigned long long x = *(const unsigned long long *)a, y = *(const unsigned long long *)b;
+    return x < y ? -1 : (x > y);
 }
 
 /* plp_common.hpp:356-390 / solver.py:109-141. */
@@ -467,8 +465,12 @@ void orc_compute_fsr(const int *placed7, int n, double mu, double *fsr) {
         load[i] = m[i];
         idx[i] = i;
     }
-    g_fsr_placed = placed7;
-    qsort(idx, (size_t)n, sizeof(int), cmp_fsr_order);
+    unsigned long long *key = (unsigned long long *)malloc(sizeof(unsigned long long) * (size_t)n);
+    for (int i = 0; i < n; i++)
+        key[i] = ((unsigned long long)(0x7fffffffu - (unsigned)placed7[7 * i + 2]) << 32) | (unsigned)i;
+    qsort(key, (size_t)n, sizeof *key, cmp_u64);
+    for (int i = 0; i < n; i++) idx[i] = (int)(key[i] & 0xffffffffu);
+    free(key);
     for (int t = 0; t < n; t++) {
         int k = idx[t];
         const int *bk = placed7 + 7 * k;

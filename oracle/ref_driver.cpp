@@ -17,7 +17,8 @@
 //          plp_common.hpp:356) and the brute-force work count of every pass.
 //   rows   K decoders seeded seed..seed+K-1 (plp_gpu.cu:350), one JSON line
 //          each, as the reference's --batch/--device envelopes do.
-//   bench  the same rows timed over T std::threads (CPU baseline).
+//   bench  the same rows timed over T std::threads (CPU baseline); several
+//          instance files may be given, each decoded by K seeds.
 //
 // Build: see oracle/Makefile (g++ -O3 -std=c++17 -ffp-contract=off, the flags
 // of gpu/Makefile:18).
@@ -55,6 +56,7 @@ struct CountingBackend : plp::FeasibilityBackend {
 
 struct Opts {
     std::string inst_path, order = "volume";
+    std::vector<std::string> more_paths;   // bench: further instances
     long long seed = 0;
     int passes = 1, K = 1, threads = 1;
     double eps = plp::EPS_QUALITY, theta = plp::THETA;
@@ -161,7 +163,8 @@ void print_row(const Row& row, const Opts& o, long long seed) {
 
 int usage() {
     std::fprintf(stderr,
-                 "usage: ref_driver run|rows|bench <instance> [--seed N] [--passes P] "
+                 "usage: ref_driver run|rows|bench <instance> [<instance> ... (bench)] "
+                 "[--seed N] [--passes P] "
                  "[--K K] [--threads T] [--eps F] [--theta F|--no-theta] "
                  "[--order volume|input] [--first-fit] [--mu F] [--fsr] [--work]\n");
     return 2;
@@ -174,7 +177,10 @@ int main(int argc, char** argv) {
     std::string mode = argv[1];
     Opts o;
     o.inst_path = argv[2];
-    for (int i = 3; i < argc; i++) {
+    int first_opt = 3;
+    while (first_opt < argc && argv[first_opt][0] != '-') o.more_paths.push_back(argv[first_opt++]);
+    if (!o.more_paths.empty() && mode != "bench") return usage();
+    for (int i = first_opt; i < argc; i++) {
         std::string a = argv[i];
         auto next = [&]() -> std::string {
             if (i + 1 >= argc) { std::exit(usage()); }
@@ -195,13 +201,15 @@ int main(int argc, char** argv) {
         else return usage();
     }
     if (o.passes < 1 || o.K < 1 || o.threads < 1) return usage();
-    plp::Instance inst;
+    std::vector<plp::Instance> insts;
     try {
-        inst = plp::load_instance(o.inst_path);
+        insts.push_back(plp::load_instance(o.inst_path));
+        for (const auto& pth : o.more_paths) insts.push_back(plp::load_instance(pth));
     } catch (const plp::InstanceFormatError& e) {
         std::fprintf(stderr, "instance error: %s\n", e.what());
         return 1;
     }
+    const plp::Instance& inst = insts[0];
     if (mode == "run") {
         print_row(run_fixed(inst, o.seed, o), o, o.seed);
         return 0;
@@ -218,15 +226,16 @@ int main(int argc, char** argv) {
         for (int t = 0; t < o.threads; t++)
             pool.emplace_back([&]() {
                 for (;;) {
+                    // decoder k: instance k / K with seed o.seed + k % K
                     int k = next_row.fetch_add(1);
-                    if (k >= o.K) break;
-                    Row r = run_fixed(inst, o.seed + k, o);
+                    if (k >= o.K * (int)insts.size()) break;
+                    Row r = run_fixed(insts[(size_t)(k / o.K)], o.seed + k % o.K, o);
                     checksum.fetch_add(r.res.volume);
                 }
             });
         for (auto& th : pool) th.join();
         double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        double packings = (double)o.K * o.passes;
+        double packings = (double)o.K * (double)insts.size() * o.passes;
         std::printf("{\"packings\": %.0f, \"seconds\": %.6f, \"packings_per_s\": %.3f, "
                     "\"threads\": %d, \"checksum_volume\": %lld}\n",
                     packings, el, packings / el, o.threads, checksum.load());

@@ -100,15 +100,19 @@ __device__ void bitonic_u32(uint32_t* key, int n2) {
 
 }  // namespace
 
-template <int NW>
+// BIG: 64-bit extreme-point keys and support sums (StateMode kBig), else
+// 32-bit ones.
+template <int NW, bool BIG>
 __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
+    using Key = typename ModeTypes<BIG ? kBig : kNarrow>::Key;
+    using Sum = typename ModeTypes<BIG ? kBig : kNarrow>::Sum;
     const DecodeArgs& a = ba.d;
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ BlockShared sh;
     __shared__ DecState sst;
-    const BlockLds lay = block_lds_layout(a.ep_cap, a.box_cap, ba.max_types, a.use_rng, NW);
-    uint32_t* const ep_a = (uint32_t*)(lds + lay.off_ep);
-    uint32_t* const ep_b = (uint32_t*)(lds + lay.off_ep2);
+    const BlockLds lay = block_lds_layout(a.ep_cap, a.box_cap, ba.max_types, a.use_rng, NW, BIG);
+    Key* const ep_a = (Key*)(lds + lay.off_ep);
+    Key* const ep_b = (Key*)(lds + lay.off_ep2);
     uint16_t* const oidx = (uint16_t*)(lds + lay.off_oidx);
     uint64_t* const mask = (uint64_t*)(lds + lay.off_mask);
     int* const rem = (int*)(lds + lay.off_rem);
@@ -116,7 +120,7 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
     uint32_t* const plan = (uint32_t*)(lds + lay.off_plan);
     double* const load = (double*)(lds + lay.off_load);
     uint64_t* const skey = (uint64_t*)(lds + lay.off_key);
-    uint32_t* const stot = (uint32_t*)(lds + lay.off_tot);
+    Sum* const stot = (Sum*)(lds + lay.off_tot);
     Box* const items = (Box*)(lds + lay.off_items);
     Box* const box = (Box*)(lds + lay.off_box);
     uint32_t* const mt = (uint32_t*)(lds + lay.off_mt);
@@ -168,8 +172,7 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
         }
         block_sync();
         const bool skip_decoder = uni(sh.quit) != 0;
-        int32_t* out_all = a.best_all + (size_t)d * a.max_items * 7;
-        int32_t* out_acc = a.best_acc + (size_t)d * a.max_items * 7;
+        int32_t* out_sel = a.out_rows + (size_t)d * a.max_items * 7;
 
         for (int pass = 0; pass < a.n_passes && !skip_decoder; pass++) {
             // ---- pass start (wave 0): all copies left, origin only, no blocks
@@ -179,7 +182,7 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
             if (wid == 0) {
                 for (int t = lane; t < nt; t += kWave) rem[t] = tr[t].qty;
                 if (lane == 0) {
-                    ep_a[0] = 0u;   // origin; L, W, H > 0
+                    ep_a[0] = Key(0);   // origin; L, W, H > 0
                     oidx[0] = kNewPoint;
                 }
                 for (int i = lane; i < a.box_cap + 8; i += kWave) box[i] = sentinel_box();
@@ -261,7 +264,7 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
                     const int bw = uni(S.bw), bd = uni(S.bd), bh = uni(S.bh);
                     if (uni(rem[ti]) < cnt_s) continue;   // planned before, stale
                     if (s == fail_s) continue;            // same test, same (empty) answer
-                    const uint32_t* act = cur ? ep_b : ep_a;
+                    const Key* act = cur ? ep_b : ep_a;
                     const int n_chunks = (na_w + kWave - 1) / kWave;
                     int cnt_w = 0;
                     for (int c = wid; c < n_chunks; c += NW) {
@@ -278,7 +281,7 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
                             const int yhi = wave_max(c2 ? y + bd : -1);
                             const int zlo = wave_min(c2 ? z : 0x7fffffff);
                             const int zhi = wave_max(c2 ? z + bh : -1);
-                            uint32_t sup = 0;
+                            Sum sup = 0;
                             const bool c12 = pair_scan(x, y, z, bw, bd, bh, c2, box, nb_w, xlo, xhi,
                                                        ylo, yhi, zlo, zhi, cbox, sup);
                             m = ballot(pair_verdict(c12, z, bw, bd, sup));
@@ -374,7 +377,7 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
                         n_box++;
                         wave_sync();
                         if (uni(rem[ti]) <= 0) remaining_min(rem, tr, nt, wm, dm, hm);
-                        const int need = merge_points(act, cur ? ep_a : ep_b, oidx, nullptr, n_act,
+                        const int need = merge_points(act, cur ? ep_a : ep_b, oidx, (Key*)nullptr, n_act,
                                                       n_dead, box, n_box, x, y, z, X2, Y2, Z2, L, W,
                                                       H, xb, yb, a.ep_cap, false, wm, dm, hm);
                         ep_need = max(ep_need, (uint32_t)need);
@@ -437,14 +440,15 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
                     const double score = better_all || better_acc
                                              ? fsr_score<4>(items, n, a.mu, load, skey, stot, nullptr)
                                              : 0.0;
+                    const bool accept = better_acc && (!a.use_theta || score >= a.theta);
                     if (better_all) {
                         st.n_all = n; st.v_all = vol; st.s_all = score;
-                        write_rows(out_all, items, n);
                     }
-                    if (better_acc && (!a.use_theta || score >= a.theta)) {
+                    if (accept) {
                         st.n_acc = n; st.v_acc = vol; st.s_acc = score;
-                        write_rows(out_acc, items, n);
                     }
+                    // the selection only moves to the packing just decoded
+                    if (accept || (st.n_acc < 0 && better_all)) write_rows(out_sel, items, n);
                     st.passes++;
                 }
                 wave_sync();
@@ -460,6 +464,7 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
             }
             if (lane == 0) a.state[d] = sst;
         }
+        if (wid == 0 && lane == 0) a.out_rec[d] = row_record(sst, a.dec_seed[d]);
         // work_counter[1] counts the decoders that end with an error, so the
         // host reads one word, not every state, when none did
         if (wid == 0 && lane == 0 && sst.err != 0) atomicAdd(a.work_counter + 1, 1u);
@@ -467,54 +472,72 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
     }
 }
 
-hipError_t launch_block(const BlockArgs& b, int waves, int grid, int lds_bytes, hipStream_t st) {
-    hipError_t e = launch_fill_u32(b.d.work_counter, 0u, 2, st);
-    if (e != hipSuccess) return e;
+template <bool BIG>
+static hipError_t launch_block_w(const BlockArgs& b, int waves, int grid, int lds_bytes,
+                                 hipStream_t st) {
     switch (waves) {
-        case 1: hipLaunchKernelGGL(block_kernel<1>, dim3(grid), dim3(64), (size_t)lds_bytes, st, b); break;
-        case 2: hipLaunchKernelGGL(block_kernel<2>, dim3(grid), dim3(128), (size_t)lds_bytes, st, b); break;
-        case 4: hipLaunchKernelGGL(block_kernel<4>, dim3(grid), dim3(256), (size_t)lds_bytes, st, b); break;
-        case 8: hipLaunchKernelGGL(block_kernel<8>, dim3(grid), dim3(512), (size_t)lds_bytes, st, b); break;
+        case 1: hipLaunchKernelGGL((block_kernel<1, BIG>), dim3(grid), dim3(64), (size_t)lds_bytes, st, b); break;
+        case 2: hipLaunchKernelGGL((block_kernel<2, BIG>), dim3(grid), dim3(128), (size_t)lds_bytes, st, b); break;
+        case 4: hipLaunchKernelGGL((block_kernel<4, BIG>), dim3(grid), dim3(256), (size_t)lds_bytes, st, b); break;
+        case 8: hipLaunchKernelGGL((block_kernel<8, BIG>), dim3(grid), dim3(512), (size_t)lds_bytes, st, b); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <int NW>
-static hipError_t block_occupancy_of(int lds_bytes, int* blocks_per_cu) {
-    hipError_t e = hipFuncSetAttribute((const void*)block_kernel<NW>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+hipError_t launch_block(const BlockArgs& b, int waves, int grid, int lds_bytes, hipStream_t st) {
+    hipError_t e = launch_fill_u32(b.d.work_counter, 0u, 2, st);
     if (e != hipSuccess) return e;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, block_kernel<NW>, 64 * NW,
-                                                        (size_t)lds_bytes);
+    return b.d.mode == kBig ? launch_block_w<true>(b, waves, grid, lds_bytes, st)
+                            : launch_block_w<false>(b, waves, grid, lds_bytes, st);
 }
 
-template <int NW>
+template <int NW, bool BIG>
+static hipError_t block_occupancy_of(int lds_bytes, int* blocks_per_cu) {
+    hipError_t e = hipFuncSetAttribute((const void*)block_kernel<NW, BIG>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, block_kernel<NW, BIG>,
+                                                        64 * NW, (size_t)lds_bytes);
+}
+
+template <int NW, bool BIG>
 static hipError_t block_static_of(int* bytes) {
     hipFuncAttributes at;
-    hipError_t e = hipFuncGetAttributes(&at, (const void*)block_kernel<NW>);
+    hipError_t e = hipFuncGetAttributes(&at, (const void*)block_kernel<NW, BIG>);
     if (e == hipSuccess) *bytes = (int)at.sharedSizeBytes;
     return e;
 }
 
-hipError_t block_occupancy(int waves, int lds_bytes, int* blocks_per_cu) {
+template <bool BIG>
+static hipError_t block_occupancy_w(int waves, int lds_bytes, int* blocks_per_cu) {
     switch (waves) {
-        case 1: return block_occupancy_of<1>(lds_bytes, blocks_per_cu);
-        case 2: return block_occupancy_of<2>(lds_bytes, blocks_per_cu);
-        case 4: return block_occupancy_of<4>(lds_bytes, blocks_per_cu);
-        case 8: return block_occupancy_of<8>(lds_bytes, blocks_per_cu);
+        case 1: return block_occupancy_of<1, BIG>(lds_bytes, blocks_per_cu);
+        case 2: return block_occupancy_of<2, BIG>(lds_bytes, blocks_per_cu);
+        case 4: return block_occupancy_of<4, BIG>(lds_bytes, blocks_per_cu);
+        case 8: return block_occupancy_of<8, BIG>(lds_bytes, blocks_per_cu);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t block_static_lds(int waves, int* bytes) {
+hipError_t block_occupancy(int waves, int mode, int lds_bytes, int* blocks_per_cu) {
+    return mode == kBig ? block_occupancy_w<true>(waves, lds_bytes, blocks_per_cu)
+                        : block_occupancy_w<false>(waves, lds_bytes, blocks_per_cu);
+}
+
+template <bool BIG>
+static hipError_t block_static_w(int waves, int* bytes) {
     switch (waves) {
-        case 1: return block_static_of<1>(bytes);
-        case 2: return block_static_of<2>(bytes);
-        case 4: return block_static_of<4>(bytes);
-        case 8: return block_static_of<8>(bytes);
+        case 1: return block_static_of<1, BIG>(bytes);
+        case 2: return block_static_of<2, BIG>(bytes);
+        case 4: return block_static_of<4, BIG>(bytes);
+        case 8: return block_static_of<8, BIG>(bytes);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t block_static_lds(int waves, int mode, int* bytes) {
+    return mode == kBig ? block_static_w<true>(waves, bytes) : block_static_w<false>(waves, bytes);
 }
 
 }  // namespace plpk

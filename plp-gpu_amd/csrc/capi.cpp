@@ -66,9 +66,15 @@ int HostBuf::reserve(size_t n, const char* what) {
         cap = 0;
     }
     size_t want = n < 4096 ? 4096 : n + n / 4;
-    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    hipError_t e = hipHostMalloc(&p, want, flags);
     if (e != hipSuccess) {
         p = nullptr;
+        return hip_fail(e, what);
+    }
+    e = hipHostGetDevicePointer(&dev, p, 0);
+    if (e != hipSuccess) {
+        (void)hipHostFree(p);
+        p = dev = nullptr;
         return hip_fail(e, what);
     }
     cap = want;
@@ -77,7 +83,7 @@ int HostBuf::reserve(size_t n, const char* what) {
 
 void HostBuf::release() {
     if (p) (void)hipHostFree(p);
-    p = nullptr;
+    p = dev = nullptr;
     cap = 0;
 }
 
@@ -91,17 +97,18 @@ int bitlen(int v) {
 }
 
 // Validates an instance descriptor and returns the host model.
-int to_instance(const plp_instance_desc* d, plp::Instance* out) {
+int to_instance(const plp_instance_desc* d, plp::Instance* out, int32_t max_side) {
     if (!d) return fail(PLP_E_ARG, "instance descriptor is NULL");
     if (d->L <= 0 || d->W <= 0 || d->H <= 0)
         return fail(PLP_E_ARG, "container dimensions must be positive");
     if (d->n_types < 0 || (d->n_types > 0 && !d->types))
         return fail(PLP_E_ARG, "bad item type table");
-    if (d->L >= (1 << 21) || d->W >= (1 << 21) || d->H >= (1 << 21) ||
-        bitlen(d->L) + bitlen(d->W) + bitlen(d->H) > 31)
+    if (d->L > max_side || d->W > max_side || d->H > max_side)
         return fail(PLP_E_RANGE, "container " + std::to_string(d->L) + "x" + std::to_string(d->W) +
-                                     "x" + std::to_string(d->H) +
-                                     " exceeds the packed extreme-point key range");
+                                     "x" + std::to_string(d->H) + " has a side above " +
+                                     std::to_string(max_side) +
+                                     (max_side == kMaxSide ? " (21-bit packed extreme-point keys)"
+                                                           : ""));
     out->L = d->L;
     out->W = d->W;
     out->H = d->H;
@@ -250,8 +257,8 @@ extern "C" int plp_feas_check_batch(plp_ctx* c, int32_t K, const int32_t* const*
     if (!n_cand || !n_aabb || !cand_xyz || !aabb6 || !bits_out || (n_ors > 0 && !ors_wdh))
         return fail(PLP_E_ARG, "NULL argument");
     if (L <= 0 || W <= 0 || H <= 0) return fail(PLP_E_ARG, "container dimensions must be positive");
-    if (L >= (1 << 21) || W >= (1 << 21) || H >= (1 << 21) || (int64_t)L * W >= (1ll << 30))
-        return fail(PLP_E_RANGE, "container exceeds the 32-bit support-area range");
+    if (L > kMaxPlainSide || W > kMaxPlainSide || H > kMaxPlainSide)
+        return fail(PLP_E_RANGE, "container side above 2^30 - 1 (coordinate sums leave int32)");
     const int bs = plpk::feas_block_size();
     int64_t tot_cand = 0, tot_box = 0, tot_bits = 0, tot_blocks = 0;
     for (int k = 0; k < K; k++) {

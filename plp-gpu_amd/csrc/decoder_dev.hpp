@@ -17,14 +17,35 @@ struct Box {
     int32_t x, y, z, x2, y2, z2, type, pad;
 };
 
-__device__ __forceinline__ void key_xyz(uint32_t key, int xb, int yb, int& x, int& y, int& z) {
-    x = (int)(key & ((1u << xb) - 1u));
-    y = (int)((key >> xb) & ((1u << yb) - 1u));
+// Widths of the per-decoder state.  An extreme point is a packed key
+// z << (xb + yb) | y << xb | x (xb, yb the bit lengths of L and W), whose
+// numeric order is the (z, y, x) order of the reference's candidate list; a
+// pair's state is its support sum.  kNarrow and kWide hold keys and sums in
+// 32 bits (bitlen(L) + bitlen(W) + bitlen(H) <= 31, so L * W < 2^30) and the
+// pair-state cache in 16 or 32 bits; kBig holds all three in 64 bits, for
+// every container side up to 2^21 - 1 (the reference device form's range,
+// gpu/plp_gpu.cu:1263-1268).
+template <int M>
+struct ModeTypes {
+    using Key = uint32_t;
+    using Sum = uint32_t;
+};
+template <>
+struct ModeTypes<kBig> {
+    using Key = uint64_t;
+    using Sum = uint64_t;
+};
+
+template <class Key>
+__device__ __forceinline__ void key_xyz(Key key, int xb, int yb, int& x, int& y, int& z) {
+    x = (int)(key & ((Key(1) << xb) - 1));
+    y = (int)((key >> xb) & ((Key(1) << yb) - 1));
     z = (int)(key >> (xb + yb));
 }
 
-__device__ __forceinline__ uint32_t xyz_key(int x, int y, int z, int xb, int yb) {
-    return ((uint32_t)z << (xb + yb)) | ((uint32_t)y << xb) | (uint32_t)x;
+template <class Key>
+__device__ __forceinline__ Key xyz_key(int x, int y, int z, int xb, int yb) {
+    return ((Key)(uint32_t)z << (xb + yb)) | ((Key)(uint32_t)y << xb) | (Key)(uint32_t)x;
 }
 
 // check_feasible_one (plp_common.hpp:297-321) for this lane's pair:
@@ -44,8 +65,8 @@ __device__ __forceinline__ uint32_t xyz_key(int x, int y, int z, int xb, int yb)
 // pads it with sentinels to a multiple of kBoxGroup and scans it in groups
 // whose LDS reads issue together, branch-free per box (overlap and support
 // both come from the three overlap extents).  The wave leaves as soon as
-// every lane has failed.  Areas fit 32 bits: L*W < 2^30 is a load-time
-// precondition, and ox, oy < 2^21 so the 24-bit multiply is exact.
+// every lane has failed.  The support sum is a Sum: 32 bits when L*W < 2^30
+// (ox, oy < 2^21, so the 24-bit multiply is exact), else 64 bits.
 constexpr int kBoxGroup = 4;
 
 // a * b + c for a, b < 2^24 in one VALU op (the compiler otherwise pairs two
@@ -54,6 +75,13 @@ __device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) 
     uint32_t r;
     asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
     return r;
+}
+// The same for 64-bit sums (v_mad_u64_u32).
+__device__ __forceinline__ uint32_t mad_sum(uint32_t a, uint32_t b, uint32_t c) {
+    return mad_u24(a, b, c);
+}
+__device__ __forceinline__ uint64_t mad_sum(uint32_t a, uint32_t b, uint64_t c) {
+    return (uint64_t)a * b + c;
 }
 constexpr int kCullWindow = kWave + kBoxGroup;   // cbox[] entries per wave
 
@@ -65,13 +93,14 @@ __device__ __forceinline__ Box sentinel_box() {
     return b;
 }
 
+template <class Sum>
 __device__ __forceinline__ bool pair_scan(int x, int y, int z, int w, int d, int h, bool alive,
                                           const Box* __restrict__ boxes, int n_box, int xlo,
                                           int xhi, int ylo, int yhi, int zlo, int zhi,
-                                          Box* __restrict__ cbox, uint32_t& s_out) {
+                                          Box* __restrict__ cbox, Sum& s_out) {
     const int lane = lane_id();
     const int x2 = x + w, y2 = y + d, z2 = z + h;
-    uint32_t s = 0;
+    Sum s = 0;
     uint64_t am = ballot(alive);   // lanes still passing C1, as a scalar mask
     for (int top = n_box - 1; top >= 0; top -= kWave) {
         if (am == 0) break;
@@ -102,7 +131,7 @@ __device__ __forceinline__ bool pair_scan(int x, int y, int z, int w, int d, int
                 const int oxy = min(ox, oy);
                 am &= ~ballot(min(oxy, oz) > 0);
                 const bool sup = (B[u].z2 == z) & (oxy > 0);
-                s = mad_u24((uint32_t)(sup ? ox : 0), (uint32_t)oy, s);
+                s = mad_sum((uint32_t)(sup ? ox : 0), (uint32_t)oy, s);
             }
             if (am == 0) break;
         }
@@ -129,43 +158,50 @@ __device__ __forceinline__ int div_orients(int p, int no, uint32_t recip) {
 // equals 2 * s < A: when 2s < A the exact quotient is at most 0.5 - 1/(2A),
 // more than half an ulp (2^-55) below 0.5, so the correctly rounded quotient
 // stays below 0.5; when 2s >= A it is at least 0.5 and rounds to at least
-// 0.5.  Every box that passes C2 has A <= L * W < 2^31 (the container range
-// the engine accepts), so the integer form gives the reference's verdict bit
-// for bit.  It runs in 32 bits: s <= A (boxes whose top is at z do not
-// overlap one another, so their overlaps with the footprint are disjoint
-// parts of it; a clamped cache sum is smaller still), hence 2s < 2^32.
-// Branch-free; when C1 or C2 fails neither the product (24-bit operands,
-// exact below 2^32) nor the sum matters.
+// 0.5.  Every box that passes C2 has A <= L * W < 2^42 (sides below 2^21),
+// so the integer form gives the reference's verdict bit for bit.  s <= A
+// (boxes whose top is at z do not overlap one another, so their overlaps
+// with the footprint are disjoint parts of it; a clamped cache sum is
+// smaller still).  With 32-bit sums L * W < 2^30, so 2s and the product of
+// the 24-bit operands fit 32 bits; with 64-bit sums both fit 64 bits.
+// Branch-free; when C1 or C2 fails neither the product nor the sum matters.
 __device__ __forceinline__ bool pair_verdict(bool c12, int z, int w, int d, uint32_t s) {
     return c12 & ((z == 0) | (2u * s >= (uint32_t)__umul24((uint32_t)w, (uint32_t)d)));
 }
+__device__ __forceinline__ bool pair_verdict(bool c12, int z, int w, int d, uint64_t s) {
+    return c12 & ((z == 0) | (2ull * s >= (uint64_t)(uint32_t)w * (uint32_t)d));
+}
 
 // Pair-state cache entry: the support sum of a pair that passes C1 and C2,
-// or kBlocked.  Both only grow as boxes are added, so the state after one
-// more box follows from the old state and that box alone.
-constexpr uint32_t kBlocked = 0xffffffffu;
+// or blocked (all ones).  Both only grow as boxes are added, so the state
+// after one more box follows from the old state and that box alone.
+template <class Sum>
+constexpr Sum kBlockedOf = ~Sum(0);
+constexpr uint32_t kBlocked = kBlockedOf<uint32_t>;
 constexpr uint16_t kNewPoint = 0xffffu;   // list entry with no cached state
 constexpr uint16_t kCorner = 0xfff0u;     // kCorner + j: corner j of the last box (state in cstate)
 
-// Entries are 32-bit, or 16-bit with the support sum clamped to 65534 and
-// 0xffff for kBlocked (exact for C3 when every w*d <= kNarrowMaxArea).
-// One 32-bit read for either width (a 16-bit entry comes with its aligned
-// neighbour, which lies inside the cache or the region after it).
-template <bool WIDE>
-__device__ __forceinline__ uint32_t cache_get(const void* c, int i) {
-    if constexpr (WIDE) {
-        return ((const uint32_t*)c)[i];
-    } else {
+// Entries are the Sum itself (kWide: 32-bit, kBig: 64-bit), or 16-bit
+// (kNarrow) with the support sum clamped to 65534 and 0xffff for blocked
+// (exact for C3 when every w*d <= kNarrowMaxArea).
+template <int M>
+__device__ __forceinline__ typename ModeTypes<M>::Sum cache_get(const void* c, int i) {
+    using Sum = typename ModeTypes<M>::Sum;
+    if constexpr (M == kNarrow) {
         const uint32_t v = ((const uint16_t*)c)[i];
         return v == 0xffffu ? kBlocked : v;
+    } else {
+        return ((const Sum*)c)[i];
     }
 }
 
-template <bool WIDE>
-__device__ __forceinline__ void cache_put(void* c, int i, uint32_t v) {
-    if constexpr (WIDE) ((uint32_t*)c)[i] = v;
-    else ((uint16_t*)c)[i] = (uint16_t)(v == kBlocked ? 0xffffu : min(v, 0xfffeu));
+template <int M>
+__device__ __forceinline__ void cache_put(void* c, int i, typename ModeTypes<M>::Sum v) {
+    using Sum = typename ModeTypes<M>::Sum;
+    if constexpr (M == kNarrow) ((uint16_t*)c)[i] = (uint16_t)(v == kBlocked ? 0xffffu : min(v, 0xfffeu));
+    else ((Sum*)c)[i] = v;
 }
+
 
 // ---------------------------------------------------------------- FSR ----
 __device__ __forceinline__ double read_lane_f64(double v, int src) {
@@ -188,11 +224,12 @@ __device__ __forceinline__ Box box_xyz(const Box& b) {
 
 // Contact area of box i's bottom face with the top face of box j (j's top at
 // i's bottom z > 0 and the footprints overlapping), else 0.
-__device__ __forceinline__ uint32_t contact_area(const Box& i, const Box& j) {
+template <class Sum>
+__device__ __forceinline__ Sum contact_area(const Box& i, const Box& j) {
     const int ox = min(i.x2, j.x2) - max(i.x, j.x);
     const int oy = min(i.y2, j.y2) - max(i.y, j.y);
     const bool c = (j.z2 == i.z) & (i.z > 0) & (ox > 0) & (oy > 0);
-    return c ? (uint32_t)ox * (uint32_t)oy : 0u;
+    return c ? (Sum)(uint32_t)ox * (uint32_t)oy : Sum(0);
 }
 
 // Box j precedes box i in the reference's visiting order (z desc, index asc).
@@ -224,9 +261,9 @@ __device__ __forceinline__ double read_lane_of(const double (&v)[R], int q, int 
     return read_lane_f64(v[R - 1], src);
 }
 
-template <int R>
+template <int R, class Sum>
 static __device__ double fsr_score_reg(const Box* __restrict__ box, int n, double mu, int* ord,
-                                       uint32_t* tot, double* fsr_out) {
+                                       Sum* tot, double* fsr_out) {
     const int lane = lane_id();
     // m = RHO * V / 1e6 (plp_common.hpp:360-362); load starts at m
     auto mass = [](const Box& b) {
@@ -236,7 +273,7 @@ static __device__ double fsr_score_reg(const Box* __restrict__ box, int n, doubl
     bool v[R];
     Box b[R];
     double l[R];
-    uint32_t t[R];
+    Sum t[R];
     int r[R];
 #pragma unroll
     for (int q = 0; q < R; q++) {
@@ -247,7 +284,8 @@ static __device__ double fsr_score_reg(const Box* __restrict__ box, int n, doubl
         r[q] = 0;
     }
     // contact totals (the boxes whose tops touch a box's bottom do not
-    // overlap one another, so a total is at most w * d < 2^30) and ranks
+    // overlap one another, so a total is at most w * d, which fits a Sum) and
+    // ranks
     constexpr int kG = 2;   // boxes whose reads issue together
     for (int j = 0; j < n; j += kG) {
         Box B[kG];
@@ -257,7 +295,7 @@ static __device__ double fsr_score_reg(const Box* __restrict__ box, int n, doubl
         for (int u = 0; u < kG; u++) {
 #pragma unroll
             for (int q = 0; q < R; q++) {
-                t[q] += contact_area(b[q], B[u]);
+                t[q] += contact_area<Sum>(b[q], B[u]);
                 r[q] += visits_before(B[u], j + u, b[q], lane + q * kWave);
             }
         }
@@ -277,12 +315,12 @@ static __device__ double fsr_score_reg(const Box* __restrict__ box, int n, doubl
     // and total come from LDS one source ahead; its load from its lane.
     int kn = __builtin_amdgcn_readlane(o[0], 0);
     Box Bn = box_xyz(box[kn]);
-    uint32_t totn = tot[kn];
+    Sum totn = tot[kn];
     for (int tt = 0; tt < n; tt++) {
         const int k = kn;
         const int bkz = uni(Bn.z), bkx = uni(Bn.x), bky = uni(Bn.y), bkx2 = uni(Bn.x2),
                   bky2 = uni(Bn.y2);
-        const uint32_t total = uni(totn);
+        const Sum total = uni(totn);
         if (tt + 1 < n) {
             kn = read_lane_of(o, (tt + 1) >> 6, (tt + 1) & (kWave - 1));
             Bn = box_xyz(box[kn]);
@@ -322,34 +360,35 @@ static __device__ double fsr_score_reg(const Box* __restrict__ box, int n, doubl
 // registers a kernel can spare at its pass end) the state stays in registers
 // (fsr_score_reg); beyond, loads and totals sit in LDS and the order comes
 // from a bitonic sort of 64-bit keys ((2^31-1-z) << 32 | index).  Returns
-// the score; per-box FSR goes to fsr_out when non-null.
-template <int kRegBoxes = 2>
+// the score; per-box FSR goes to fsr_out when non-null.  Contact totals are
+// Sums (32-bit when every footprint is below 2^30, else 64-bit).
+template <int kRegBoxes = 2, class Sum = uint32_t>
 static __device__ double fsr_score(const Box* __restrict__ box, int n, double mu, double* load,
-                            uint64_t* skey, uint32_t* tot, double* fsr_out) {
+                            uint64_t* skey, Sum* tot, double* fsr_out) {
     const int lane = lane_id();
     if (n <= 0) return 0.0;
-    if (n <= 2 * kWave) return fsr_score_reg<2>(box, n, mu, (int*)skey, tot, fsr_out);
+    if (n <= 2 * kWave) return fsr_score_reg<2, Sum>(box, n, mu, (int*)skey, tot, fsr_out);
     if (kRegBoxes >= 4 && n <= 4 * kWave)
-        return fsr_score_reg<4>(box, n, mu, (int*)skey, tot, fsr_out);
+        return fsr_score_reg<4, Sum>(box, n, mu, (int*)skey, tot, fsr_out);
     const int n2 = next_pow2(n);
     for (int i = lane; i < n2; i += kWave)
         skey[i] = i < n ? (((uint64_t)(0x7fffffffu - (uint32_t)box[i].z) << 32) | (uint32_t)i)
                         : ~0ull;
     // load = m = RHO * V / 1e6; tot[k] = contact area of box k with the boxes
     // whose top is at its bottom (lane per k, every j in turn).  Those boxes
-    // do not overlap one another, so tot[k] <= w * d < 2^30.
+    // do not overlap one another, so tot[k] <= w * d fits a Sum.
     for (int k = lane; k < n; k += kWave) {
         const Box Bk = box[k];
         const int64_t v = (int64_t)(Bk.x2 - Bk.x) * (Bk.y2 - Bk.y) * (Bk.z2 - Bk.z);
         load[k] = 50.0 * (double)v / 1.0e6;
-        uint32_t t = 0;
+        Sum t = 0;
         if (Bk.z > 0) {
             for (int j = 0; j < n; j++) {
                 const Box Bj = box[j];
                 const int ox = min(Bk.x2, Bj.x2) - max(Bk.x, Bj.x);
                 const int oy = min(Bk.y2, Bj.y2) - max(Bk.y, Bj.y);
                 if (j != k && Bj.z2 == Bk.z && ox > 0 && oy > 0)
-                    t += (uint32_t)ox * (uint32_t)oy;
+                    t += (Sum)(uint32_t)ox * (uint32_t)oy;
             }
         }
         tot[k] = t;
@@ -378,7 +417,7 @@ static __device__ double fsr_score(const Box* __restrict__ box, int n, double mu
         const int bkz = uni(Bk.z);
         if (bkz == 0) break;   // the rest are floor boxes: nothing supports them
         const int bkx = uni(Bk.x), bky = uni(Bk.y), bkx2 = uni(Bk.x2), bky2 = uni(Bk.y2);
-        const uint32_t total = uni(tot[k]);
+        const Sum total = uni(tot[k]);
         if (total > 0) {
             const double lk = load[k];
             for (int j = lane; j < n; j += kWave) {
@@ -434,11 +473,12 @@ __device__ __forceinline__ uint32_t corner_kill(const Box* __restrict__ box, int
 }
 
 // Corners whose key is in src[0..n): bit j of the result (wave-uniform).
-__device__ __forceinline__ uint32_t corner_present(const uint32_t* src, int n, const uint32_t ck[8]) {
+template <class Key>
+__device__ __forceinline__ uint32_t corner_present(const Key* src, int n, const Key ck[8]) {
     uint32_t present = 0;
     for (int i0 = 0; i0 < n; i0 += kWave) {
         const int i = i0 + lane_id();
-        const uint32_t v = src[i < n ? i : i0];
+        const Key v = src[i < n ? i : i0];
 #pragma unroll
         for (int j = 0; j < 8; j++)
             if (ballot((i < n) & (v == ck[j]))) present |= 1u << j;
@@ -465,7 +505,8 @@ __device__ __forceinline__ uint32_t corner_present(const uint32_t* src, int n, c
 // live list without being strictly inside and the corners that are neither
 // useful nor already listed.  n_act <= cap on entry keeps every write within
 // the cap + 8 slots of a list.  One wave; returns the larger new list size.
-static __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t* oidx, uint32_t* dead,
+template <class Key>
+static __device__ int merge_points(const Key* src, Key* dst, uint16_t* oidx, Key* dead,
                             int& n_act, int& n_dead, const Box* box, int n_box, int x, int y, int z,
                             int X2, int Y2, int Z2, int L, int W, int H, int xb, int yb, int cap,
                             bool count_work, int wm, int dm, int hm) {
@@ -473,13 +514,13 @@ static __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t*
     // Corner j: bit 0 -> x side, bit 1 -> y, bit 2 -> z; with the key's field
     // order (z, then y, then x) the keys increase with j.  Lane j < 8 holds
     // corner j's key in myk.
-    uint32_t ck[8];
+    Key ck[8];
     uint32_t cdead = 0;
-    uint32_t myk = 0;
+    Key myk = 0;
 #pragma unroll
     for (int j = 0; j < 8; j++) {
         const int jx = (j & 1) ? X2 : x, jy = (j & 2) ? Y2 : y, jz = (j & 4) ? Z2 : z;
-        ck[j] = xyz_key(jx, jy, jz, xb, yb);
+        ck[j] = xyz_key<Key>(jx, jy, jz, xb, yb);
         if (lane == j) myk = ck[j];
         if ((jx + wm > L) | (jy + dm > W) | (jz + hm > H)) cdead |= 1u << j;
     }
@@ -501,7 +542,7 @@ static __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t*
         for (int i0 = 0; i0 < n_dead; i0 += kWave) {
             const int i = i0 + lane;
             const bool ok = i < n_dead;
-            const uint32_t v = ok ? dead[i] : 0u;
+            const Key v = ok ? dead[i] : Key(0);
             int px, py, pz;
             key_xyz(v, xb, yb, px, py, pz);
             const bool strict = (x < px) & (px < X2) & (y < py) & (py < Y2) & (z < pz) & (pz < Z2);
@@ -523,7 +564,7 @@ static __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t*
     for (int i0 = 0; i0 < n_act; i0 += kWave) {
         const int i = i0 + lane;
         const bool ok = i < n_act;
-        const uint32_t v = ok ? src[i] : 0u;
+        const Key v = ok ? src[i] : Key(0);
         int px, py, pz;
         key_xyz(v, xb, yb, px, py, pz);
         const bool gone = ok & ((px + wm > L) | (py + dm > W) | (pz + hm > H) |
@@ -581,7 +622,8 @@ static __device__ int merge_points(const uint32_t* src, uint32_t* dst, uint16_t*
 // The pair states (cache entries) of the 8 corners of the box nb just
 // placed, for the orientations in ors (no of them), against all n_box boxes:
 // lane j * no + o.  One wave.
-static __device__ inline void corner_states(const int* nb, uint32_t* cstate, const int* ors,
+template <class Sum>
+static __device__ inline void corner_states(const int* nb, Sum* cstate, const int* ors,
                                      int no, const Box* box, int n_box, int L, int W, int H,
                                      Box* cbox) {
     const int lane = lane_id();
@@ -595,7 +637,7 @@ static __device__ inline void corner_states(const int* nb, uint32_t* cstate, con
     const int w = ors[3 * o], d = ors[3 * o + 1], h = ors[3 * o + 2];
     const bool c2 = (lane < n) & (x + w <= L) & (y + d <= W) & (z + h <= H);
     bool c12 = false;
-    uint32_t s = 0;
+    Sum s = 0;
     if (ballot(c2) != 0) {
         const int xlo = wave_min(c2 ? x : 0x7fffffff);
         const int xhi = wave_max(c2 ? x + w : -1);
@@ -605,7 +647,7 @@ static __device__ inline void corner_states(const int* nb, uint32_t* cstate, con
         const int zhi = wave_max(c2 ? z + h : -1);
         c12 = pair_scan(x, y, z, w, d, h, c2, box, n_box, xlo, xhi, ylo, yhi, zlo, zhi, cbox, s);
     }
-    if (lane < n) cstate[lane] = c12 ? s : kBlocked;
+    if (lane < n) cstate[lane] = c12 ? s : kBlockedOf<Sum>;
 }
 
 __device__ __forceinline__ void write_rows(int32_t* dst, const Box* box, int n) {
@@ -624,6 +666,25 @@ __device__ __forceinline__ void write_rows(int32_t* dst, const Box* box, int n) 
         }
         dst[t] = v;
     }
+}
+
+// The record of a decoder's selected packing (D4', plp_common.hpp:493-511:
+// the best accepted packing, else the best of all with fallback set).
+__device__ __forceinline__ RowRecord row_record(const DecState& s, uint64_t seed) {
+    const bool have_acc = s.n_acc >= 0;
+    RowRecord r;
+    r.n_boxes = have_acc ? s.n_acc : max(0, s.n_all);
+    r.fallback = have_acc ? 0 : 1;
+    r.n_passes = s.passes;
+    r.err = s.err;
+    r.volume = have_acc ? s.v_acc : s.v_all;
+    r.score = have_acc ? s.s_acc : s.s_all;
+    r.seed = seed;
+    r.work_pair_box = s.work_pair_box;
+    r.work_live = s.work_live;
+    r.max_ep = (int32_t)s.max_ep;
+    r.ep_need = (int32_t)s.ep_need;
+    return r;
 }
 
 }  // namespace plpk

@@ -5,14 +5,18 @@
 // HBM, all allocated once per load:
 //   instance headers (64 B each), step-type table, orientation table (20 ints
 //   per instance type), decoder -> instance map, decoder seeds, work order,
-//   per decoder: MT19937 state (625 words), DecState (64 B), two packings of
-//   max_items rows x 7 ints (best overall / best accepted).
-// A run is one fill kernel (queue head) + one persistent decode launch; it
-// can be captured once in a hipGraph and replayed.
+//   per decoder: MT19937 state (625 words) and DecState (80 B).
+// Results go to pinned host memory that the kernels write directly: per
+// decoder one 64-byte record (the plp_row_result layout) at the end of a run
+// and the selected packing (max_items rows x 7 ints) whenever the selection
+// moves to the packing just decoded, so a fetch copies nothing over PCIe
+// after the kernel.  A run is one fill kernel (queue head) + one persistent
+// decode launch; it can be captured once in a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <array>
+#include <cstddef>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -30,7 +34,7 @@ struct plp_engine {
     bool loaded = false;
     int n_inst = 0, n_dec = 0, max_items = 1, box_cap = 1;
     int ep_cap = 0, lds = 0, grid = 0, occ = 0, wpd = 1;
-    int wide = 0;   // 32-bit pair-state cache
+    int mode = plpk::kNarrow;   // widths of keys, support sums and cache entries (StateMode)
     bool blocks = false;   // block-building decoder (plp_engine_load_blocks)
     plp_block_params bp{};
     int max_types = 1;
@@ -41,7 +45,8 @@ struct plp_engine {
     bool need_reset = true;
     bool states_fresh = false;
     DevBuf d_inst, d_step, d_step_min, d_ors, d_dec_inst, d_dec_seed, d_order, d_retry, d_mt, d_state,
-        d_best_all, d_best_acc, d_counter;
+        d_counter;
+    HostBuf h_rows, h_rec;   // results, written by the kernels (fine-grained pinned memory)
     std::vector<int32_t> h_order;
     std::vector<int32_t> h_dec_inst;
     std::vector<uint64_t> h_seed;
@@ -53,6 +58,15 @@ struct plp_engine {
     hipGraphExec_t gexec = nullptr;
     int g_passes = -1, g_reset = -1, g_grid = -1, g_cap = -1, g_wpd = -1;
 };
+
+// The kernels write plp_row_result records as plpk::RowRecord.
+static_assert(sizeof(plp_row_result) == sizeof(plpk::RowRecord), "record size");
+static_assert(offsetof(plp_row_result, volume) == offsetof(plpk::RowRecord, volume), "record");
+static_assert(offsetof(plp_row_result, score) == offsetof(plpk::RowRecord, score), "record");
+static_assert(offsetof(plp_row_result, seed) == offsetof(plpk::RowRecord, seed), "record");
+static_assert(offsetof(plp_row_result, work_live_pair_box) == offsetof(plpk::RowRecord, work_live),
+              "record");
+static_assert(offsetof(plp_row_result, ep_need) == offsetof(plpk::RowRecord, ep_need), "record");
 
 namespace {
 
@@ -74,16 +88,18 @@ void drop_graph(plp_engine* g) {
 // decoder kind.
 int dyn_lds(plp_engine* g, int ep_cap) {
     if (g->blocks)
-        return plpk::block_lds_layout(ep_cap, g->box_cap, g->max_types, g->use_rng, g->wpd).total;
-    return plpk::lds_layout(ep_cap, g->box_cap, g->use_rng, g->wpd, g->cfg.count_work, g->wide)
+        return plpk::block_lds_layout(ep_cap, g->box_cap, g->max_types, g->use_rng, g->wpd,
+                                      g->mode == plpk::kBig)
+            .total;
+    return plpk::lds_layout(ep_cap, g->box_cap, g->use_rng, g->wpd, g->cfg.count_work, g->mode)
         .total;
 }
 
 // The decoder kernel's own static LDS bytes.
 int static_lds(plp_engine* g) {
     int st = 0;
-    hipError_t e = g->blocks ? plpk::block_static_lds(g->wpd, &st)
-                             : plpk::decode_static_lds(g->wpd, g->wide != 0, &st);
+    hipError_t e = g->blocks ? plpk::block_static_lds(g->wpd, g->mode, &st)
+                             : plpk::decode_static_lds(g->wpd, g->mode, &st);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         st = 4096;
@@ -104,8 +120,8 @@ int shape(plp_engine* g, int ep_cap, int blocks_per_cu_req, int n_work) {
                                      std::to_string(ep_cap) +
                                      " extreme points) exceeds the block LDS limit");
     int occ = 0;
-    hipError_t e = g->blocks ? plpk::block_occupancy(g->wpd, lds, &occ)
-                             : plpk::decode_occupancy(g->wpd, g->wide != 0, lds, &occ);
+    hipError_t e = g->blocks ? plpk::block_occupancy(g->wpd, g->mode, lds, &occ)
+                             : plpk::decode_occupancy(g->wpd, g->mode, lds, &occ);
     if (e != hipSuccess) return hip_fail(e, "decoder occupancy");
     if (occ < 1) return fail(PLP_E_EPCAP, "the decoder cannot be resident with this LDS size");
     int bpc = blocks_per_cu_req > 0 ? std::min(blocks_per_cu_req, occ) : occ;
@@ -142,8 +158,8 @@ plpk::DecodeArgs make_args(plp_engine* g, int n_passes, int reset, const int32_t
     a.n_passes = n_passes;
     a.mt = (uint32_t*)g->d_mt.p;
     a.state = (DecState*)g->d_state.p;
-    a.best_all = (int32_t*)g->d_best_all.p;
-    a.best_acc = (int32_t*)g->d_best_acc.p;
+    a.out_rows = (int32_t*)g->h_rows.dev;
+    a.out_rec = (plpk::RowRecord*)g->h_rec.dev;
     a.max_items = g->max_items;
     a.ep_cap = g->ep_cap;
     a.box_cap = g->box_cap;
@@ -151,7 +167,7 @@ plpk::DecodeArgs make_args(plp_engine* g, int n_passes, int reset, const int32_t
     a.use_theta = g->params.use_theta;
     a.reset = reset;
     a.count_work = g->cfg.count_work ? 1 : 0;
-    a.wide_cache = g->wide;
+    a.mode = g->mode;
     a.eps = g->params.eps;
     a.theta = g->params.theta;
     a.mu = g->params.mu;
@@ -233,6 +249,8 @@ extern "C" int plp_engine_create(plp_ctx* ctx, const plp_engine_config* cfg, plp
         return fail(PLP_E_ARG, "config: waves_per_decoder must be 0, 1, 2, 4 or 8; caps >= 0");
     }
     g->wpd = wq > 0 ? wq : 4;
+    // fine-grained: the kernels' stores reach host memory while they run
+    g->h_rows.flags = g->h_rec.flags = hipHostMallocMapped | hipHostMallocCoherent;
     if ((e = hipEventCreate(&g->ev0)) != hipSuccess || (e = hipEventCreate(&g->ev1)) != hipSuccess) {
         delete g;
         return hip_fail(e, "hipEventCreate");
@@ -259,9 +277,10 @@ extern "C" void plp_engine_destroy(plp_engine* g) {
     drop_graph(g);
     for (DevBuf* b : {&g->d_inst, &g->d_step, &g->d_step_min, &g->d_shape, &g->d_tinfo, &g->d_byt,
                       &g->d_vorder, &g->d_ors, &g->d_dec_inst, &g->d_dec_seed,
-                      &g->d_order, &g->d_retry, &g->d_mt, &g->d_state, &g->d_best_all,
-                      &g->d_best_acc, &g->d_counter})
+                      &g->d_order, &g->d_retry, &g->d_mt, &g->d_state, &g->d_counter})
         b->release();
+    g->h_rows.release();
+    g->h_rec.release();
     if (g->ev0) (void)hipEventDestroy(g->ev0);
     if (g->ev1) (void)hipEventDestroy(g->ev1);
     if (g->h_failed) (void)hipHostFree(g->h_failed);
@@ -411,7 +430,7 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
     int max_types = 1;
     std::vector<int64_t> cost((size_t)n_inst, 0);
     int max_items = 1;
-    int wide = 0;
+    int wide = 0, big = 0;
     for (int i = 0; i < n_inst; i++) {
         plp::Instance in;
         int rc = to_instance(&insts[i], &in);
@@ -430,6 +449,9 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
             return fail(PLP_E_RANGE, "instance " + std::to_string(i) + ": more than 2^24 - 1 types");
         h.xbits = bitlen(in.L);
         h.ybits = bitlen(in.W);
+        // 32-bit keys and sums need bitlen(L) + bitlen(W) + bitlen(H) <= 31
+        // (then L * W < 2^30); beyond, the batch runs with 64-bit ones
+        if (h.xbits + h.ybits + bitlen(in.H) > 31) big = 1;
         max_types = std::max(max_types, h.n_types);
         if (bp) {
             BlockTables bt;
@@ -484,7 +506,7 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
     g->n_dec = n_dec;
     g->max_items = max_items;
     g->box_cap = (max_items + 7) / 8 * 8;   // sentinel-padded groups of 8
-    g->wide = wide;
+    g->mode = big ? plpk::kBig : wide ? plpk::kWide : plpk::kNarrow;
     g->params = *params;
     g->blocks = bp != nullptr;
     if (bp) g->bp = *bp;
@@ -520,8 +542,8 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
         (rc = g->d_retry.reserve(4 * (size_t)n_dec, "retry order")) ||
         (rc = g->d_mt.reserve(4 * plpk::kMtWords * (size_t)n_dec, "RNG states")) ||
         (rc = g->d_state.reserve(sizeof(DecState) * (size_t)n_dec, "decoder states")) ||
-        (rc = g->d_best_all.reserve(rows_bytes, "best packings")) ||
-        (rc = g->d_best_acc.reserve(rows_bytes, "accepted packings")))
+        (rc = g->h_rows.reserve(rows_bytes, "pinned result packings")) ||
+        (rc = g->h_rec.reserve(sizeof(plpk::RowRecord) * (size_t)n_dec, "pinned result records")))
         return rc;
     struct Up {
         void* dst;
@@ -672,45 +694,33 @@ extern "C" int plp_engine_max_items(plp_engine* g, int32_t* max_items) {
 
 extern "C" int plp_engine_fetch(plp_engine* g, plp_row_result* rows, int32_t* placements) {
     if (!g || !g->loaded || !rows) return fail(PLP_E_ARG, "engine not loaded");
+    if (g->passes_done == 0) return fail(PLP_E_ARG, "nothing decoded since the last load or reset");
     hipError_t e = hipSetDevice(g->ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     if ((e = hipStreamSynchronize(g->ctx->stream)) != hipSuccess) return hip_fail(e, "fetch sync");
-    int rc;
-    if (!g->states_fresh && (rc = read_states(g)) != PLP_OK) return rc;
-    const size_t stride = (size_t)g->max_items * 7;
-    std::vector<int32_t> all, acc;
+    // The kernels wrote the records and packings into pinned host memory.
+    const plpk::RowRecord* rec = (const plpk::RowRecord*)g->h_rec.p;
+    std::memcpy(rows, rec, sizeof(plp_row_result) * (size_t)g->n_dec);
     if (placements) {
-        all.resize(stride * (size_t)g->n_dec);
-        acc.resize(stride * (size_t)g->n_dec);
-        e = hipMemcpyAsync(all.data(), g->d_best_all.p, 4 * all.size(), hipMemcpyDeviceToHost,
-                           g->ctx->stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(acc.data(), g->d_best_acc.p, 4 * acc.size(), hipMemcpyDeviceToHost,
-                               g->ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(g->ctx->stream);
-        if (e != hipSuccess) return hip_fail(e, "D2H packings");
+        const size_t stride = (size_t)g->max_items * 7;
+        const int32_t* src = (const int32_t*)g->h_rows.p;
+        for (int d = 0; d < g->n_dec; d++)
+            std::memcpy(placements + stride * (size_t)d, src + stride * (size_t)d,
+                        4 * 7 * (size_t)std::max(0, rec[d].n_boxes));
     }
-    for (int d = 0; d < g->n_dec; d++) {
-        const DecState& s = g->h_state[(size_t)d];
-        plp_row_result& r = rows[d];
-        std::memset(&r, 0, sizeof r);
-        const bool have_acc = s.n_acc >= 0;
-        r.n_boxes = have_acc ? s.n_acc : std::max(0, s.n_all);
-        r.fallback = have_acc ? 0 : 1;
-        r.n_passes = s.passes;
-        r.err = s.err;
-        r.volume = have_acc ? s.v_acc : s.v_all;
-        r.score = have_acc ? s.s_acc : s.s_all;
-        r.seed = g->h_seed[(size_t)d];
-        r.work_pair_box = s.work_pair_box;
-        r.work_live_pair_box = s.work_live;
-        r.max_ep = (int32_t)s.max_ep;
-        r.ep_need = (int32_t)s.ep_need;
-        if (placements) {
-            const int32_t* src = (have_acc ? acc.data() : all.data()) + stride * (size_t)d;
-            std::memcpy(placements + stride * (size_t)d, src, 4 * 7 * (size_t)r.n_boxes);
-        }
-    }
+    return PLP_OK;
+}
+
+extern "C" int plp_engine_results(plp_engine* g, const plp_row_result** rows,
+                                  const int32_t** placements, int32_t* row_stride) {
+    if (!g || !g->loaded || !rows) return fail(PLP_E_ARG, "engine not loaded");
+    if (g->passes_done == 0) return fail(PLP_E_ARG, "nothing decoded since the last load or reset");
+    hipError_t e = hipSetDevice(g->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    if ((e = hipStreamSynchronize(g->ctx->stream)) != hipSuccess) return hip_fail(e, "results sync");
+    *rows = (const plp_row_result*)g->h_rec.p;
+    if (placements) *placements = (const int32_t*)g->h_rows.p;
+    if (row_stride) *row_stride = g->max_items * 7;
     return PLP_OK;
 }
 

@@ -21,6 +21,17 @@ struct InstHdr {
 };
 static_assert(sizeof(InstHdr) == 64, "InstHdr layout");
 
+// Widths of a decoder's extreme-point keys, support sums and pair-state
+// cache entries (the kernels' StateMode template parameter): kNarrow 32-bit
+// keys and sums with 16-bit cache entries, kWide the same with 32-bit
+// entries, kBig 64 bits for all three.
+enum StateMode { kNarrow = 0, kWide = 1, kBig = 2 };
+
+__host__ __device__ constexpr int cache_entry_bytes(int mode) {
+    return mode == kNarrow ? 2 : mode == kWide ? 4 : 8;
+}
+__host__ __device__ constexpr int key_bytes(int mode) { return mode == kBig ? 8 : 4; }
+
 // Orientation table row: n, then up to 6 (w, d, h) triples, padded to 20 ints.
 constexpr int kOrsRow = 20;
 
@@ -44,6 +55,20 @@ static_assert(sizeof(DecState) == 80, "DecState layout");
 
 constexpr int kMtWords = 625;      // 624 state words + index
 
+// One decoder's result in the layout of plp_row_result (include/plp_hip.h):
+// the selected packing's size, volume and score (best accepted, or best of
+// all when no packing reached theta), written by the decoder kernels into
+// pinned host memory at the end of every run.
+struct RowRecord {
+    int32_t n_boxes, fallback, n_passes, err;
+    int64_t volume;
+    double score;
+    uint64_t seed;
+    uint64_t work_pair_box, work_live;
+    int32_t max_ep, ep_need;
+};
+static_assert(sizeof(RowRecord) == 64, "RowRecord layout");
+
 
 struct DecodeArgs {
     const InstHdr* inst;
@@ -57,8 +82,9 @@ struct DecodeArgs {
     int32_t n_passes;
     uint32_t* mt;            // n_dec * kMtWords
     DecState* state;         // n_dec
-    int32_t* best_all;       // n_dec * max_items * 7 (JSON row order)
-    int32_t* best_acc;
+    int32_t* out_rows;       // pinned host: n_dec * max_items * 7, each decoder's selected
+                             // packing (JSON row order), rewritten whenever the selection changes
+    RowRecord* out_rec;      // pinned host: n_dec result records, written at the end of a run
     int32_t max_items;
     int32_t ep_cap;
     int32_t box_cap;
@@ -66,7 +92,7 @@ struct DecodeArgs {
     int32_t use_theta;
     int32_t reset;           // 1: seed RNG and clear selection state first
     int32_t count_work;      // 1: also keep the dead points, to count the reference's work
-    int32_t wide_cache;      // 1: 32-bit pair-state cache (some w*d > kNarrowMaxArea)
+    int32_t mode;            // StateMode of the loaded batch
     double eps, theta, mu;
     uint32_t* work_counter;  // [0] persistent-grid work queue head, [1] decoders ending in error
 };
@@ -98,15 +124,15 @@ constexpr int64_t kNarrowMaxArea = 2 * 65534;
 
 __host__ __device__ inline LdsLayout lds_layout(int32_t ep_cap, int32_t box_cap, int32_t use_rng,
                                                 int32_t n_waves = 1, int32_t count_work = 0,
-                                                int32_t wide_cache = 0) {
+                                                int32_t mode = kNarrow) {
     LdsLayout l;
-    const int32_t list = (ep_cap + 8) * 4;
+    const int32_t list = (ep_cap + 8) * key_bytes(mode);
     l.off_ep = 0;
     l.off_ep2 = list;
     l.off_oidx = 2 * list;
     l.off_cache = (l.off_oidx + (ep_cap + 8) * 2 + 15) & ~15;
     l.n_cache = ep_cap * 6;
-    l.off_dead = (l.off_cache + 2 * l.n_cache * (wide_cache ? 4 : 2) + 15) & ~15;
+    l.off_dead = (l.off_cache + 2 * l.n_cache * cache_entry_bytes(mode) + 15) & ~15;
     l.off_mask = (l.off_dead + (count_work ? list : 0) + 7) & ~7;
     l.n_mask = (ep_cap * 6 + 63) / 64;
     l.off_rel = (l.off_mask + l.n_mask * 8 + 15) & ~15;
@@ -115,7 +141,7 @@ __host__ __device__ inline LdsLayout lds_layout(int32_t ep_cap, int32_t box_cap,
     l.off_load = 0;
     l.off_key = box_cap * 8;
     l.off_tot = l.off_key + l.n_key * 8;
-    int32_t fsr_end = l.off_tot + box_cap * 4;
+    int32_t fsr_end = l.off_tot + box_cap * (mode == kBig ? 8 : 4);   // contact totals
     int32_t region = pass_end > fsr_end ? pass_end : fsr_end;
     l.off_box = (region + 15) & ~15;
     l.off_mt = l.off_box + (box_cap + 8) * 32;
@@ -165,9 +191,9 @@ struct BlockLds {
 
 __host__ __device__ inline BlockLds block_lds_layout(int32_t ep_cap, int32_t box_cap,
                                                      int32_t max_types, int32_t use_rng,
-                                                     int32_t n_waves) {
+                                                     int32_t n_waves, int32_t big = 0) {
     BlockLds l;
-    const int32_t list = (ep_cap + 8) * 4;
+    const int32_t list = (ep_cap + 8) * (big ? 8 : 4);
     l.off_ep = 0;
     l.off_ep2 = list;
     l.off_oidx = 2 * list;
@@ -182,7 +208,7 @@ __host__ __device__ inline BlockLds block_lds_layout(int32_t ep_cap, int32_t box
     l.off_load = 0;
     l.off_key = box_cap * 8;
     l.off_tot = l.off_key + l.n_key * 8;
-    l.off_items = (l.off_tot + box_cap * 4 + 15) & ~15;
+    l.off_items = (l.off_tot + box_cap * (big ? 8 : 4) + 15) & ~15;
     const int32_t fsr_end = l.off_items + box_cap * 32;
     const int32_t region = pass_end > fsr_end ? pass_end : fsr_end;
     l.off_box = (region + 15) & ~15;

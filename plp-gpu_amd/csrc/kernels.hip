@@ -36,6 +36,7 @@ struct alignas(16) StepDesc {
 // Block-wide scalars.  Wave 0 writes step before the barrier that opens the
 // step (the one ending the previous step); every wave reads it before the
 // step's verdict barrier, and the next write comes after that barrier.
+template <class Sum>
 struct StepShared {
     alignas(16) int cnt[2][8];   // feasible pairs found by each wave, by evaluated-step
                                  // parity: a wave may start step k+1 while others
@@ -44,7 +45,7 @@ struct StepShared {
     int item;       // queue item of the block
     int quit;       // the decoder stopped (capacity error): leave the pass loop
     int nb[6];      // the box just placed: x, y, z, x2, y2, z2
-    uint32_t cstate[48];   // pair states of its 8 corners x this type's orientations
+    Sum cstate[48];        // pair states of its 8 corners x this type's orientations
 };
 
 // Persistent decoder: one block of NW waves decodes one (instance, seed)
@@ -68,26 +69,28 @@ struct StepShared {
 // extreme-point merge), FSR and the selection; the other waves follow the
 // published steps and wait at the block barriers.
 // The bound asks for at most 128 VGPRs (four waves per SIMD, i.e. four
-// 4-wave decoders per CU).  WIDE selects 32-bit pair-state cache entries
-// (a.wide_cache), else 16-bit ones.
-template <int NW, bool WIDE>
+// 4-wave decoders per CU).  M (StateMode, a.mode) sets the widths of the
+// extreme-point keys, support sums and pair-state cache entries.
+template <int NW, int M>
 __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
+    using Key = typename ModeTypes<M>::Key;
+    using Sum = typename ModeTypes<M>::Sum;
+    constexpr Sum kBlk = kBlockedOf<Sum>;
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int s_ors[18 * NW];
-    __shared__ StepShared sh;
+    __shared__ StepShared<Sum> sh;
     __shared__ DecState sst;
-    const LdsLayout lay = lds_layout(a.ep_cap, a.box_cap, a.use_rng, NW, a.count_work,
-                                     a.wide_cache);
-    uint32_t* const ep_a = (uint32_t*)(lds + lay.off_ep);
-    uint32_t* const ep_b = (uint32_t*)(lds + lay.off_ep2);
-    uint32_t* const dead = (uint32_t*)(lds + lay.off_dead);
+    const LdsLayout lay = lds_layout(a.ep_cap, a.box_cap, a.use_rng, NW, a.count_work, M);
+    Key* const ep_a = (Key*)(lds + lay.off_ep);
+    Key* const ep_b = (Key*)(lds + lay.off_ep2);
+    Key* const dead = (Key*)(lds + lay.off_dead);
     uint16_t* const oidx = (uint16_t*)(lds + lay.off_oidx);
     unsigned char* const cache = lds + lay.off_cache;
-    const int cache_bytes = lay.n_cache * (WIDE ? 4 : 2);
+    const int cache_bytes = lay.n_cache * cache_entry_bytes(M);
     uint64_t* mask = (uint64_t*)(lds + lay.off_mask);
     double* load = (double*)(lds + lay.off_load);
     uint64_t* skey = (uint64_t*)(lds + lay.off_key);
-    uint32_t* stot = (uint32_t*)(lds + lay.off_tot);
+    Sum* stot = (Sum*)(lds + lay.off_tot);
     Box* box = (Box*)(lds + lay.off_box);
     uint32_t* mt = (uint32_t*)(lds + lay.off_mt);
     int32_t* s_step = (int32_t*)(lds + lay.off_steps);
@@ -143,8 +146,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
         }
         block_sync();
         const bool skip_decoder = uni(sh.quit) != 0;
-        int32_t* out_all = a.best_all + (size_t)d * a.max_items * 7;
-        int32_t* out_acc = a.best_acc + (size_t)d * a.max_items * 7;
+        int32_t* out_sel = a.out_rows + (size_t)d * a.max_items * 7;
 
         for (int pass = 0; pass < a.n_passes && !skip_decoder; pass++) {
             // ---- pass start (wave 0): origin only, no boxes
@@ -154,7 +156,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
             uint32_t n_steps_eval = 0, max_ep = 1, ep_need = 1;
             if (wid == 0) {
                 if (lane == 0) {
-                    ep_a[0] = 0u;   // origin; L, W, H > 0 so it is live
+                    ep_a[0] = Key(0);   // origin; L, W, H > 0 so it is live
                     oidx[0] = kNewPoint;
                 }
                 for (int i = lane; i < a.box_cap + 8; i += kWave) box[i] = sentinel_box();
@@ -241,7 +243,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     cur_ti = ti;
                 }
                 wave_sync();
-                const uint32_t* act = cur ? ep_b : ep_a;
+                const Key* act = cur ? ep_b : ep_a;
 
                 // ---- pair verdicts (every wave).  After a placement of
                 // the same type, a pair of a point that was in the previous
@@ -276,7 +278,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     const bool c12 = valid & (x + w <= L) & (y + dd <= W) & (z + h <= H);
                     // settled: the pair's state st (support sum or kBlocked) is
                     // known without a box scan
-                    uint32_t st = kBlocked;
+                    Sum st = kBlk;
                     bool settled = false;
                     if (inc) {
                         // a corner of the new box takes its precomputed state; a
@@ -284,23 +286,23 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                         // the new box; both read at in-bounds indices, selected
                         const bool corner = (oi >= kCorner) & (oi != kNewPoint);
                         const bool cached = oi < kCorner;
-                        const uint32_t cs = sh.cstate[corner ? (oi - kCorner) * no + o : 0];
-                        const uint32_t old = cache_get<WIDE>(c_rd, cached ? oi * no + o : 0);
+                        const Sum cs = sh.cstate[corner ? (oi - kCorner) * no + o : 0];
+                        const Sum old = cache_get<M>(c_rd, cached ? oi * no + o : 0);
                         const int ox = min(x + w, nx2) - max(x, nx);
                         const int oy = min(y + dd, ny2) - max(y, ny);
                         const int oz = min(z + h, nz2) - max(z, nz);
                         const int oxy = min(ox, oy);
                         const bool ov = min(oxy, oz) > 0;
-                        const uint32_t add = __umul24((uint32_t)(((nz2 == z) & (oxy > 0)) ? ox : 0),
-                                                      (uint32_t)oy);
-                        st = corner ? cs : (((old == kBlocked) | ov) ? kBlocked : old + add);
+                        const Sum add = mad_sum((uint32_t)(((nz2 == z) & (oxy > 0)) ? ox : 0),
+                                                (uint32_t)oy, Sum(0));
+                        st = corner ? cs : (((old == kBlk) | ov) ? kBlk : old + add);
                         settled = c12 & (corner | cached);
                     }
                     const bool scan = c12 & !settled;
                     // Settled lanes first, so that nothing of theirs stays
                     // live across the scan; a lane failing C2 is blocked.
-                    uint64_t m = ballot(settled & pair_verdict(st != kBlocked, z, w, dd, st));
-                    if (valid && !scan) cache_put<WIDE>(c_wr, p, settled ? st : kBlocked);
+                    uint64_t m = ballot(settled & pair_verdict(st != kBlk, z, w, dd, st));
+                    if (valid && !scan) cache_put<M>(c_wr, p, settled ? st : kBlk);
                     if (ballot(scan) != 0) {
                         const int xlo = wave_min(scan ? x : 0x7fffffff);
                         const int xhi = wave_max(scan ? x + w : -1);
@@ -308,10 +310,10 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                         const int yhi = wave_max(scan ? y + dd : -1);
                         const int zlo = wave_min(scan ? z : 0x7fffffff);
                         const int zhi = wave_max(scan ? z + h : -1);
-                        uint32_t s2 = 0;
+                        Sum s2 = 0;
                         const bool a2 = pair_scan(x, y, z, w, dd, h, scan, box, nb_w, xlo, xhi, ylo,
                                                   yhi, zlo, zhi, cbox, s2);
-                        if (scan) cache_put<WIDE>(c_wr, p, a2 ? s2 : kBlocked);
+                        if (scan) cache_put<M>(c_wr, p, a2 ? s2 : kBlk);
                         m |= ballot(scan & pair_verdict(a2, z, w, dd, s2));
                     }
                     mask[c] = m;   // every lane writes the same (uniform) mask
@@ -434,14 +436,17 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 if (a.count_work) st.max_ep = max(st.max_ep, max_ep);
                 st.ep_need = max(st.ep_need, ep_need);
                 if (st.err == 0) {
+                    const bool accept = better_acc && (!a.use_theta || score >= a.theta);
                     if (better_all) {
                         st.n_all = n; st.v_all = vol; st.s_all = score;
-                        write_rows(out_all, box, n);
                     }
-                    if (better_acc && (!a.use_theta || score >= a.theta)) {
+                    if (accept) {
                         st.n_acc = n; st.v_acc = vol; st.s_acc = score;
-                        write_rows(out_acc, box, n);
                     }
+                    // The selection only ever moves to the packing just decoded
+                    // (now the best accepted, or the best of all while nothing is
+                    // accepted), so the host copy is rewritten from LDS then.
+                    if (accept || (st.n_acc < 0 && better_all)) write_rows(out_sel, box, n);
                     st.passes++;
                 }
                 wave_sync();
@@ -457,6 +462,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
             }
             if (lane == 0) a.state[d] = sst;
         }
+        if (wid == 0 && lane == 0) a.out_rec[d] = row_record(sst, a.dec_seed[d]);
         // work_counter[1] counts the decoders that end with an error, so the
         // host reads one word, not every state, when none did
         if (wid == 0 && lane == 0 && sst.err != 0) atomicAdd(a.work_counter + 1, 1u);
@@ -493,7 +499,7 @@ __global__ void __launch_bounds__(kFeasBlock) feas_kernel(FeasRowArgs r) {
     }
     const int x2 = x + w, y2 = y + d, z2 = z + h;
     bool alive = valid && x2 <= r.L && y2 <= r.W && z2 <= r.H;
-    uint32_t s = 0;
+    uint64_t s = 0;   // support area: up to w * d, any container
     for (int t0 = 0; t0 < n_box; t0 += kFeasTile) {
         const int nt = min(kFeasTile, n_box - t0);
         __syncthreads();
@@ -513,7 +519,7 @@ __global__ void __launch_bounds__(kFeasBlock) feas_kernel(FeasRowArgs r) {
             const int ox = min(x2, B.x2) - max(x, B.x);
             const int oy = min(y2, B.y2) - max(y, B.y);
             const bool sup = (B.z2 == z) & (ox > 0) & (oy > 0);
-            s += sup ? __umul24((uint32_t)ox, (uint32_t)oy) : 0u;
+            s += sup ? (uint64_t)(uint32_t)ox * (uint32_t)oy : 0ull;
         }
     }
     if (alive && z > 0) alive = !((double)s / (double)((int64_t)w * d) < 0.5);
@@ -525,10 +531,10 @@ __global__ void __launch_bounds__(64) fsr_kernel(FsrArgs f) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int pk = blockIdx.x;
     const int r0 = f.pack_off[pk], n = f.pack_off[pk + 1] - r0;
-    const LdsLayout lay = lds_layout(0, f.box_cap, 0);
+    const LdsLayout lay = lds_layout(0, f.box_cap, 0, 1, 0, kBig);   // 64-bit contact totals
     double* load = (double*)(lds + lay.off_load);
     uint64_t* skey = (uint64_t*)(lds + lay.off_key);
-    uint32_t* stot = (uint32_t*)(lds + lay.off_tot);
+    uint64_t* stot = (uint64_t*)(lds + lay.off_tot);
     Box* box = (Box*)(lds + lay.off_box);
     for (int i = lane_id(); i < n; i += kWave) {
         const int32_t* q = f.rows7 + (size_t)(r0 + i) * 7;
@@ -571,14 +577,14 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int n) {
 
 // ------------------------------------------------------------ launchers --
 
-template <bool WIDE>
+template <int M>
 static hipError_t launch_decode_w(const DecodeArgs& a, int waves, int grid, int lds_bytes,
                                   hipStream_t st) {
     switch (waves) {
-        case 1: hipLaunchKernelGGL((decode_kernel<1, WIDE>), dim3(grid), dim3(64), (size_t)lds_bytes, st, a); break;
-        case 2: hipLaunchKernelGGL((decode_kernel<2, WIDE>), dim3(grid), dim3(128), (size_t)lds_bytes, st, a); break;
-        case 4: hipLaunchKernelGGL((decode_kernel<4, WIDE>), dim3(grid), dim3(256), (size_t)lds_bytes, st, a); break;
-        case 8: hipLaunchKernelGGL((decode_kernel<8, WIDE>), dim3(grid), dim3(512), (size_t)lds_bytes, st, a); break;
+        case 1: hipLaunchKernelGGL((decode_kernel<1, M>), dim3(grid), dim3(64), (size_t)lds_bytes, st, a); break;
+        case 2: hipLaunchKernelGGL((decode_kernel<2, M>), dim3(grid), dim3(128), (size_t)lds_bytes, st, a); break;
+        case 4: hipLaunchKernelGGL((decode_kernel<4, M>), dim3(grid), dim3(256), (size_t)lds_bytes, st, a); break;
+        case 8: hipLaunchKernelGGL((decode_kernel<8, M>), dim3(grid), dim3(512), (size_t)lds_bytes, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -587,8 +593,12 @@ static hipError_t launch_decode_w(const DecodeArgs& a, int waves, int grid, int 
 hipError_t launch_decode(const DecodeArgs& a, int waves, int grid, int lds_bytes,
                          hipStream_t st) {
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, a.work_counter, 0u, 2);
-    return a.wide_cache ? launch_decode_w<true>(a, waves, grid, lds_bytes, st)
-                        : launch_decode_w<false>(a, waves, grid, lds_bytes, st);
+    switch (a.mode) {
+        case kNarrow: return launch_decode_w<kNarrow>(a, waves, grid, lds_bytes, st);
+        case kWide: return launch_decode_w<kWide>(a, waves, grid, lds_bytes, st);
+        case kBig: return launch_decode_w<kBig>(a, waves, grid, lds_bytes, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st) {
@@ -611,52 +621,61 @@ hipError_t launch_rng_script(const RngArgs& g, int n_seeds, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int NW, bool WIDE>
+template <int NW, int M>
 static hipError_t occupancy_of(int lds_bytes, int* blocks_per_cu) {
-    hipError_t e = hipFuncSetAttribute((const void*)decode_kernel<NW, WIDE>,
+    hipError_t e = hipFuncSetAttribute((const void*)decode_kernel<NW, M>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
     if (e != hipSuccess) return e;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, decode_kernel<NW, WIDE>,
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, decode_kernel<NW, M>,
                                                         64 * NW, (size_t)lds_bytes);
 }
 
-template <int NW, bool WIDE>
+template <int NW, int M>
 static hipError_t static_lds_of(int* bytes) {
     hipFuncAttributes at;
-    hipError_t e = hipFuncGetAttributes(&at, (const void*)decode_kernel<NW, WIDE>);
+    hipError_t e = hipFuncGetAttributes(&at, (const void*)decode_kernel<NW, M>);
     if (e == hipSuccess) *bytes = (int)at.sharedSizeBytes;
     return e;
 }
 
-template <bool WIDE>
+template <int M>
 static hipError_t decode_static_lds_w(int waves, int* bytes) {
     switch (waves) {
-        case 1: return static_lds_of<1, WIDE>(bytes);
-        case 2: return static_lds_of<2, WIDE>(bytes);
-        case 4: return static_lds_of<4, WIDE>(bytes);
-        case 8: return static_lds_of<8, WIDE>(bytes);
+        case 1: return static_lds_of<1, M>(bytes);
+        case 2: return static_lds_of<2, M>(bytes);
+        case 4: return static_lds_of<4, M>(bytes);
+        case 8: return static_lds_of<8, M>(bytes);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t decode_static_lds(int waves, bool wide, int* bytes) {
-    return wide ? decode_static_lds_w<true>(waves, bytes) : decode_static_lds_w<false>(waves, bytes);
+hipError_t decode_static_lds(int waves, int mode, int* bytes) {
+    switch (mode) {
+        case kNarrow: return decode_static_lds_w<kNarrow>(waves, bytes);
+        case kWide: return decode_static_lds_w<kWide>(waves, bytes);
+        case kBig: return decode_static_lds_w<kBig>(waves, bytes);
+        default: return hipErrorInvalidValue;
+    }
 }
 
-template <bool WIDE>
+template <int M>
 static hipError_t decode_occupancy_w(int waves, int lds_bytes, int* blocks_per_cu) {
     switch (waves) {
-        case 1: return occupancy_of<1, WIDE>(lds_bytes, blocks_per_cu);
-        case 2: return occupancy_of<2, WIDE>(lds_bytes, blocks_per_cu);
-        case 4: return occupancy_of<4, WIDE>(lds_bytes, blocks_per_cu);
-        case 8: return occupancy_of<8, WIDE>(lds_bytes, blocks_per_cu);
+        case 1: return occupancy_of<1, M>(lds_bytes, blocks_per_cu);
+        case 2: return occupancy_of<2, M>(lds_bytes, blocks_per_cu);
+        case 4: return occupancy_of<4, M>(lds_bytes, blocks_per_cu);
+        case 8: return occupancy_of<8, M>(lds_bytes, blocks_per_cu);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t decode_occupancy(int waves, bool wide, int lds_bytes, int* blocks_per_cu) {
-    return wide ? decode_occupancy_w<true>(waves, lds_bytes, blocks_per_cu)
-                : decode_occupancy_w<false>(waves, lds_bytes, blocks_per_cu);
+hipError_t decode_occupancy(int waves, int mode, int lds_bytes, int* blocks_per_cu) {
+    switch (mode) {
+        case kNarrow: return decode_occupancy_w<kNarrow>(waves, lds_bytes, blocks_per_cu);
+        case kWide: return decode_occupancy_w<kWide>(waves, lds_bytes, blocks_per_cu);
+        case kBig: return decode_occupancy_w<kBig>(waves, lds_bytes, blocks_per_cu);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t fsr_prepare(int lds_bytes) {

@@ -153,7 +153,7 @@ extern "C" int plp_plain_run(plp_ctx* ctx, const plp_instance_desc* desc, uint64
     if (n_passes < 1) return fail(PLP_E_ARG, "n_passes must be >= 1");
     if (p->order != 0 && p->order != 1) return fail(PLP_E_ARG, "order must be 0 or 1");
     plp::Instance in;
-    int rc = to_instance(desc, &in);
+    int rc = to_instance(desc, &in, kMaxPlainSide);
     if (rc != PLP_OK) return rc;
     PlainRunner runner(ctx, in, seed, 1, *p);
     for (int i = 0; i < n_passes; i++)

@@ -133,6 +133,9 @@ plp::RunResult to_result(const plp_row_result& r, const int32_t* rows, double ti
 
 }  // namespace
 
+// Largest container side of the device-resident engine (2^21 - 1).
+constexpr int plpi_max_side = (1 << 21) - 1;
+
 int main(int argc, char** argv) {
     std::string instance_path, order = "volume";
     long long seed = 0;
@@ -238,7 +241,18 @@ int main(int argc, char** argv) {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     };
     std::vector<plp::RunResult> results;
-    if (batch > 0 || plain) {
+    // The device-resident engine packs extreme points into keys of 21 bits
+    // per side.  The reference's --device exits 2 beyond that
+    // (gpu/plp_gpu.cu:1263-1268); its default host run has no such limit
+    // (gpu/plp_gpu.cpp:80-143), so a single run of a larger container takes
+    // the plain form, whose host decode loop keeps the points unpacked.
+    const bool beyond_keys = inst.L > plpi_max_side || inst.W > plpi_max_side || inst.H > plpi_max_side;
+    if (device > 0 && beyond_keys) {
+        std::fprintf(stderr, "--device: container dims exceed the 21-bit pack (L=%d W=%d H=%d)\n",
+                     inst.L, inst.W, inst.H);
+        return 2;
+    }
+    if (batch > 0 || plain || beyond_keys) {
         plpi::PlainRunner runner(ctx, inst, (uint64_t)seed, K, p);
         for (int done = 0;;) {
             rc = runner.run_pass();

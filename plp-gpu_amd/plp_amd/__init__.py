@@ -29,7 +29,8 @@ EXPORTS = [
     "plp_ctx_create", "plp_ctx_destroy", "plp_feas_check_all", "plp_feas_check_batch",
     "plp_fsr_eval", "plp_rng_script", "plp_plain_run", "plp_engine_create",
     "plp_engine_destroy", "plp_engine_load", "plp_engine_load_blocks", "plp_engine_reset", "plp_engine_run", "plp_engine_sync",
-    "plp_engine_max_items", "plp_engine_fetch", "plp_engine_last_ms", "plp_engine_describe",
+    "plp_engine_max_items", "plp_engine_fetch", "plp_engine_results", "plp_engine_last_ms",
+    "plp_engine_describe",
     "plp_engine_autotune", "plp_server_create", "plp_server_destroy", "plp_server_submit",
     "plp_server_poll", "plp_server_inflight", "plp_server_query",
 ]
@@ -106,6 +107,8 @@ _sig = {
     "plp_engine_sync": (C.c_int, [_P]),
     "plp_engine_max_items": (C.c_int, [_P, _I32P]),
     "plp_engine_fetch": (C.c_int, [_P, C.POINTER(RowResult), _P]),
+    "plp_engine_results": (C.c_int, [_P, C.POINTER(C.POINTER(RowResult)), C.POINTER(_I32P),
+                                     _I32P]),
     "plp_engine_last_ms": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "plp_engine_describe": (C.c_int, [_P, _I32P, _I32P, _I32P, _I32P]),
     "plp_engine_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(EngineConfig),
@@ -320,13 +323,33 @@ class Engine:
         return {"waves_per_decoder": cfg.waves_per_decoder, "ep_cap": cfg.ep_cap,
                 "blocks_per_cu": cfg.blocks_per_cu}, ms.value
 
-    def fetch(self, placements=True):
-        rows = (RowResult * self.n_dec)()
+    def max_items(self) -> int:
         mi = C.c_int32()
         _check(_lib.plp_engine_max_items(self.h, C.byref(mi)))
-        pl = np.zeros((self.n_dec, mi.value, 7), np.int32) if placements else None
+        return mi.value
+
+    def fetch(self, placements=True, out=None):
+        """(rows, placements): plp_engine_fetch into new arrays, or into
+        out = (RowResult array of n_dec, int32 array (n_dec, max_items, 7))."""
+        if out is None:
+            rows = (RowResult * self.n_dec)()
+            pl = np.zeros((self.n_dec, self.max_items(), 7), np.int32) if placements else None
+        else:
+            rows, pl = out
         _check(_lib.plp_engine_fetch(self.h, rows, _ptr(pl) if placements else None))
-        return list(rows), pl
+        return (list(rows) if out is None else rows), pl
+
+    def results(self):
+        """Zero-copy view of the results in the engine's pinned host memory
+        (plp_engine_results): (record array of n_dec RowResult, int32 array
+        (n_dec, max_items, 7)); valid until the next load, reset, run or close."""
+        rp = C.POINTER(RowResult)()
+        pp = _I32P()
+        stride = C.c_int32()
+        _check(_lib.plp_engine_results(self.h, C.byref(rp), C.byref(pp), C.byref(stride)))
+        rows = C.cast(rp, C.POINTER(RowResult * self.n_dec)).contents
+        pl = np.ctypeslib.as_array(pp, shape=(self.n_dec, stride.value // 7, 7))
+        return rows, pl
 
 
 class Server:
@@ -382,6 +405,26 @@ class Server:
         n = C.c_int32()
         _check(_lib.plp_server_inflight(self.h, C.byref(n)))
         return n.value
+
+
+def decoder_digest(n_boxes, volume, score, fallback, n_passes, seed, placements) -> bytes:
+    """SHA-256 of one decoder's result: the reference JSON fields (score as its
+    IEEE-754 bits) and the selected packing's rows (int32 little-endian)."""
+    import hashlib
+    import struct
+    head = struct.pack("<iqdiiQ", int(n_boxes), int(volume), float(score), int(bool(fallback)),
+                       int(n_passes), int(seed))
+    rows = np.ascontiguousarray(np.asarray(placements, "<i4").reshape(-1, 7)[: int(n_boxes)])
+    return hashlib.sha256(head + rows.tobytes()).digest()
+
+
+def results_digest(rows, placements) -> tuple[str, list[str]]:
+    """(digest of all decoders, per-decoder digests as 16 hex digits) of
+    engine rows and their placements."""
+    import hashlib
+    per = [decoder_digest(r.n_boxes, r.volume, r.score, r.fallback, r.n_passes, r.seed,
+                          placements[k]) for k, r in enumerate(rows)]
+    return hashlib.sha256(b"".join(per)).hexdigest(), [d.hex()[:16] for d in per]
 
 
 def to_json_dict(row: RowResult, placements) -> dict:

@@ -85,12 +85,33 @@ def test_cli_mu_env_and_flag_agree():
     assert a["stability_score"] == want["stability_score"]
 
 
-def test_cli_range_error_exit_code(tmp_path):
+@pytest.mark.parametrize("name", ["cube2000", "br1_mm", "hc40_mm", "side_max", "side_over"])
+def test_cli_big_containers_match_reference_cli(tmp_path, name):
+    """The default single run decodes every container the reference's host
+    build does (gpu/plp_gpu.cpp:80-143): sides up to 2^21 - 1 on the device
+    engine with 64-bit keys, beyond that through the plain form."""
+    with open(golden_path("big_cases.json")) as f:
+        case = {c["name"]: c for c in json.load(f)["cases"]}[name]
+    path = tmp_path / "inst.txt"
+    path.write_text(case["text"])
+    with open(golden_path("big_cases.json")) as f:
+        args = json.load(f)["args"]
+    for key, want in case["runs"].items():
+        got = json.loads(run([str(path)] + args[key]))
+        assert strip(got) == strip(want), (name, key)
+
+
+def test_cli_device_range_exit_code(tmp_path):
+    """--device keeps the reference's 21-bit limit and exit code 2
+    (gpu/plp_gpu.cu:1263-1268)."""
+    with open(golden_path("big_cases.json")) as f:
+        case = {c["name"]: c for c in json.load(f)["cases"]}["side_over"]
     big = tmp_path / "big.txt"
-    big.write_text("1\n1 0\n2000 2000 2000\n1\n1 10 1 10 1 10 1 1\n")
-    p = subprocess.run([CLI, str(big), "--single-pass", "--json"], capture_output=True, text=True,
-                       timeout=120)
+    big.write_text(case["text"])
+    p = subprocess.run([CLI, str(big), "--device", "2", "--single-pass", "--json"],
+                       capture_output=True, text=True, timeout=120)
     assert p.returncode == 2
+    assert "exceed the 21-bit pack" in p.stderr
 
 
 def test_cli_serve_json_lines_match_device_rows():

@@ -1,29 +1,46 @@
-"""The device engines at the benchmark's full sizes, checked through properties
-that hold at any size, plus bit-exact oracle spot checks.
+"""The device engines at the benchmark's full sizes, decoder for decoder.
 
-Every returned packing (best accepted, or best of all on fallback) must:
-  * lie in the container and never overlap (C2 and pairwise C1);
-  * give every box above the floor at least half its footprint in support
+Every decoder of every workload is compared with the host: its result digest
+(plp_amd.decoder_digest: the reference JSON fields with the score's bits, and
+the packing's rows) must equal the one tests/golden/fullsize_digests.json
+pins (written by make_fullsize_digests.py, where the C oracle and the
+reference's g++ build agree on every decoder), and, except for the
+880x349x330 workload whose host decode takes minutes, the C oracle decodes
+every decoder again here on a thread pool and must match field for field.
+Workloads (plp_amd.synth.workload): the 20-type instance with 1024 decoders
+x 4 passes, 256 uneven instances x 16 seeds in one launch, the 100-type
+instance (600 items, about 100 placed) with 2048 decoders, the same 600 items
+in 880x349x330 (all placed, reference lists past 3000 points) with 2048
+decoders, and the block-building engine on the 20-type instance with 1024
+decoders x 4 passes.
+
+Every returned packing is also checked for properties that hold at any size:
+  * it lies in the container and no two boxes overlap (C2 and pairwise C1);
+  * every box above the floor has at least half its footprint in support
     (C3, the integer form 2 * support >= w * d; for the block engine C3 holds
-    per block, so this check is left to the oracle comparison there);
-  * use only orientations its type allows and at most qty copies per type;
-  * report the volume and box count of its rows;
-  * carry the stability score the FSR kernel computes for its rows, bit for
-    bit, with fallback set exactly when that score is below theta.
-A few decoders of each workload are also decoded by the oracle and must match
-field for field.  Workloads: the 20-type instance with 1024 decoders x 4
-passes, 256 uneven instances x 16 seeds in one launch, the 100-type instance
-(600 items, about 100 placed) with 2048 decoders, and the block-building engine on the
-20-type instance with 1024 decoders x 4 passes.
+    per block, so there the oracle comparison covers it);
+  * only allowed orientations, at most qty copies per type;
+  * the volume and box count of its rows;
+  * the stability score the FSR kernel computes for its rows, bit for bit,
+    with fallback set exactly when that score is below theta.
 """
+import json
+import os
+import random
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import pytest
 
+from conftest import golden_path
 from oracle_lib import Oracle
 
 pytestmark = pytest.mark.gpu
 
 EPS, THETA = 0.01, 0.9
+THREADS = min(16, os.cpu_count() or 1)   # the host share of one GPU
+with open(golden_path("fullsize_digests.json")) as _f:
+    PINNED = json.load(_f)["workloads"]
 FIELDS = ("n_boxes", "volume_cm3", "stability_score", "fallback", "n_passes", "seed",
           "placements")
 
@@ -95,8 +112,8 @@ def _check_all(plp, ctx, insts, dec_inst, rows, pl, theta, per_box_support=True)
         _check_packing(insts[dec_inst[k]], r, pl[k], theta, per_box_support)
 
 
-def _run(plp, ctx, texts, dec_inst, seeds, passes, blocks=None):
-    eng = plp.Engine(ctx, use_graph=True)
+def _run(plp, ctx, texts, dec_inst, seeds, passes, blocks=None, count_work=False):
+    eng = plp.Engine(ctx, use_graph=not count_work, count_work=count_work)
     eng.load([plp.Instance.from_text(tx) for tx in texts], dec_inst, seeds,
              plp.params(eps=EPS, theta=THETA), blocks=blocks)
     eng.run(passes)
@@ -106,42 +123,83 @@ def _run(plp, ctx, texts, dec_inst, seeds, passes, blocks=None):
     return rows, pl
 
 
-def test_twenty_types_1024_decoders_4_passes(plp, ctx, orc):
+def _oracle_all(orc, insts, dec_inst, seeds, passes, which=None, block=False):
+    """Oracle results of the decoders `which` (default: all), on a thread pool."""
+    which = range(len(seeds)) if which is None else which
+
+    def one(k):
+        f = orc.block_run if block else orc.run
+        return k, f(insts[dec_inst[k]], seed=seeds[k], passes=passes, eps=EPS, theta=THETA)
+
+    with ThreadPoolExecutor(THREADS) as ex:
+        return dict(ex.map(one, which))
+
+
+def _compare(plp, rows, pl, want):
+    bad = [k for k, w in want.items()
+           if plp.to_json_dict(rows[k], pl[k]) != {f: w[f] for f in FIELDS}]
+    assert not bad, f"{len(bad)} decoders differ from the oracle, first {bad[:8]}"
+
+
+def _compare_pinned(plp, name, rows, pl):
+    digest, per = plp.results_digest(rows, pl)
+    pin = PINNED[name]
+    bad = [k for k in range(len(rows)) if per[k] != pin["per_decoder"][k]]
+    assert not bad, f"{len(bad)} decoders differ from the pinned digests, first {bad[:8]}"
+    assert digest == pin["digest"]
+
+
+def _workload_test(plp, ctx, orc, name):
     from plp_amd import synth
-    text = synth.multi_instance()
-    inst = orc.parse(text)[0]
-    K, P = 1024, 4
-    rows, pl = _run(plp, ctx, [text], [0] * K, list(range(1, K + 1)), P)
-    _check_all(plp, ctx, [inst], [0] * K, rows, pl, THETA)
-    for k in (0, 511, 1023):
-        want = orc.run(inst, seed=k + 1, passes=P, eps=EPS, theta=THETA)
-        assert plp.to_json_dict(rows[k], pl[k]) == {f: want[f] for f in FIELDS}, k
+    texts, dec_inst, seeds, passes = synth.workload(name)
+    insts = [orc.parse(tx)[0] for tx in texts]
+    rows, pl = _run(plp, ctx, texts, dec_inst, seeds, passes)
+    _check_all(plp, ctx, insts, dec_inst, rows, pl, THETA)
+    _compare_pinned(plp, name, rows, pl)
+    return texts, insts, dec_inst, seeds, passes, rows, pl
+
+
+def test_twenty_types_1024_decoders_4_passes(plp, ctx, orc):
+    _, insts, dec_inst, seeds, passes, rows, pl = _workload_test(plp, ctx, orc, "twenty_1024x4")
+    _compare(plp, rows, pl, _oracle_all(orc, insts, dec_inst, seeds, passes))
 
 
 def test_256_instances_16_seeds(plp, ctx, orc):
-    from plp_amd import synth
-    texts = synth.mixed_batch(256)
-    insts = [orc.parse(tx)[0] for tx in texts]
-    dec_inst = [i for i in range(256) for _ in range(16)]
-    seeds = [1 + s for _ in range(256) for s in range(16)]
-    rows, pl = _run(plp, ctx, texts, dec_inst, seeds, 1)
-    _check_all(plp, ctx, insts, dec_inst, rows, pl, THETA)
-    for k in (0, 17, 2000, 4095):
-        want = orc.run(insts[dec_inst[k]], seed=seeds[k], passes=1, eps=EPS, theta=THETA)
-        assert plp.to_json_dict(rows[k], pl[k]) == {f: want[f] for f in FIELDS}, k
+    _, insts, dec_inst, seeds, passes, rows, pl = _workload_test(plp, ctx, orc, "mixed_256x16")
+    _compare(plp, rows, pl, _oracle_all(orc, insts, dec_inst, seeds, passes))
 
 
 def test_hundred_types_2048_decoders(plp, ctx, orc):
-    from plp_amd import synth
-    text = synth.large_instance()
-    inst = orc.parse(text)[0]
-    K = 2048
-    rows, pl = _run(plp, ctx, [text], [0] * K, list(range(1, K + 1)), 1)
+    _, insts, dec_inst, seeds, passes, rows, pl = _workload_test(plp, ctx, orc, "hundred_2048")
     assert max(r.n_boxes for r in rows) > 64
-    _check_all(plp, ctx, [inst], [0] * K, rows, pl, THETA)
-    for k in (0, 2047):
-        want = orc.run(inst, seed=k + 1, passes=1, eps=EPS, theta=THETA)
-        assert plp.to_json_dict(rows[k], pl[k]) == {f: want[f] for f in FIELDS}, k
+    _compare(plp, rows, pl, _oracle_all(orc, insts, dec_inst, seeds, passes))
+
+
+def test_hundred_types_fit_2048_decoders(plp, ctx, orc):
+    """All 600 boxes placed: lists, boxes and FSR far past the reference
+    device kernel's caps (4096 points at most, plp_gpu.cu:533; 256 FSR boxes,
+    plp_gpu.cu:1121-1146).  Every decoder against the pinned digests; a
+    sample of 64 against the oracle here; 256 decoders again from a 128-point
+    capacity, so that every one of them outgrows it and is rerun at the
+    capacity the engine escalates to."""
+    texts, insts, dec_inst, seeds, passes, rows, pl = _workload_test(plp, ctx, orc,
+                                                                       "hundred_fit_2048")
+    assert min(r.n_boxes for r in rows) > 500
+    assert max(r.ep_need for r in rows) > 544
+    sample = sorted(random.Random(7).sample(range(len(seeds)), 64))
+    want = _oracle_all(orc, insts, dec_inst, seeds, passes, which=sample)
+    _compare(plp, rows, pl, want)
+    assert max(w["max_cand"] for w in want.values()) > 2000   # the reference's list
+    eng = plp.Engine(ctx, ep_cap=128)
+    eng.load([plp.Instance.from_text(texts[0])], dec_inst[:256], seeds[:256],
+             plp.params(eps=EPS, theta=THETA))
+    eng.run(passes)
+    eng.sync()
+    assert eng.describe()["ep_cap"] > 544
+    erows, epl = eng.fetch()
+    eng.close()
+    _, per = plp.results_digest(erows, epl)
+    assert per == PINNED["hundred_fit_2048"]["per_decoder"][:256]
 
 
 def test_block_engine_1024_decoders_4_passes(plp, ctx, orc):
@@ -149,11 +207,9 @@ def test_block_engine_1024_decoders_4_passes(plp, ctx, orc):
     text = synth.multi_instance()
     inst = orc.parse(text)[0]
     K, P = 1024, 4
-    rows, pl = _run(plp, ctx, [text], [0] * K, list(range(1, K + 1)), P,
-                    blocks=plp.block_params())
+    seeds = list(range(1, K + 1))
+    rows, pl = _run(plp, ctx, [text], [0] * K, seeds, P, blocks=plp.block_params())
     # C3 holds for each block's footprint (blockproof/plp_block.cpp tests the
     # block shape), not for every item of a block's bottom layer
     _check_all(plp, ctx, [inst], [0] * K, rows, pl, THETA, per_box_support=False)
-    for k in (0, 1023):
-        want = orc.block_run(inst, seed=k + 1, passes=P, eps=EPS, theta=THETA)
-        assert plp.to_json_dict(rows[k], pl[k]) == {f: want[f] for f in FIELDS}, k
+    _compare(plp, rows, pl, _oracle_all(orc, [inst], [0] * K, seeds, P, block=True))

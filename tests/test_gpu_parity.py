@@ -54,9 +54,12 @@ def row_fields(plp, row, pl):
     return d
 
 
+FIELDS = ("n_boxes", "volume_cm3", "stability_score", "fallback", "n_passes", "seed",
+          "placements")
+
+
 def ref_fields(res):
-    return {k: res[k] for k in ("n_boxes", "volume_cm3", "stability_score", "fallback",
-                                "n_passes", "seed", "placements")}
+    return {k: res[k] for k in FIELDS}
 
 
 # ------------------------------------------------------------- RNG --------
@@ -380,12 +383,131 @@ def test_engine_autotune_keeps_results(plp, ctx, ref_runs):
         [ref_fields(c["result"]) for c in cases]
 
 
-def test_engine_rejects_out_of_range_container(plp, ctx):
+# ------------------------------------- containers past 32-bit packed keys ---
+# Sides up to 2^21 - 1, the reference device form's range
+# (gpu/plp_gpu.cu:1263-1268), through 64-bit keys and support sums.
+
+def _big_cases():
+    with open(golden_path("big_cases.json")) as f:
+        return {c["name"]: c for c in json.load(f)["cases"]}
+
+
+@pytest.mark.parametrize("name", ["cube2000", "br1_mm", "hc40_mm", "side_max"])
+def test_engine_big_containers_match_reference_and_oracle(plp, ctx, orc, name):
+    """The reference CLI's eps 0 and eps 0.3 single runs (its g++ host
+    build), and 16 seeds x 2 passes against the oracle, at 1 and 4 waves."""
+    case = _big_cases()[name]
+    inst = plp.Instance.from_text(case["text"])
+    o_inst = orc.parse(case["text"])[0]
+    for wpd in (1, 4):
+        eng = plp.Engine(ctx, waves_per_decoder=wpd)
+        for key, prm, seed in (("eps0", plp.params(eps=0.0, use_theta=False), 0),
+                               ("eps03_seed5", plp.params(eps=0.3, theta=0.9), 5)):
+            eng.load([inst], [0], [seed], prm)
+            eng.run(1)
+            eng.sync()
+            rows, pl = eng.fetch()
+            assert plp.to_json_dict(rows[0], pl[0]) == ref_fields(case["runs"][key]), (key, wpd)
+        seeds = list(range(100, 116))
+        eng.load([inst], [0] * 16, seeds, plp.params(eps=0.3, theta=0.9))
+        eng.run(2)
+        eng.sync()
+        rows, pl = eng.fetch()
+        eng.close()
+        for k, sd in enumerate(seeds):
+            want = orc.run(o_inst, seed=sd, passes=2, eps=0.3, theta=0.9)
+            assert plp.to_json_dict(rows[k], pl[k]) == {f: want[f] for f in FIELDS}, (k, wpd)
+
+
+def test_engine_millimetre_instance_is_the_centimetre_one_scaled(plp, ctx):
+    """Every decision of the decoder is invariant under scaling all lengths
+    by 10 (C2 and C1 compare lengths, C3 compares a ratio, the (z, y, x)
+    order is kept, the RNG sees the same feasible counts): the BR1 packings
+    in mm are the cm packings times 10, seed for seed."""
+    cm = plp.Instance.from_text(load_text("data/instances/BR1_050.txt"))
+    mm = plp.Instance.from_text(_big_cases()["br1_mm"]["text"])
+    seeds = list(range(1, 65))
+    out = []
+    for inst in (cm, mm):
+        eng = plp.Engine(ctx)
+        eng.load([inst], [0] * 64, seeds, plp.params(eps=0.3, use_theta=False))
+        eng.run(1)
+        eng.sync()
+        out.append(eng.fetch())
+        eng.close()
+    (rc, pc), (rm, pm) = out
+    for k in range(64):
+        n = rc[k].n_boxes
+        assert rm[k].n_boxes == n and rm[k].volume == 1000 * rc[k].volume
+        want = np.asarray(pc[k][:n]).copy()
+        want[:, 1:] *= 10
+        assert (np.asarray(pm[k][:n]) == want).all(), k
+
+
+def test_engine_huge_footprints_64bit_sums(plp, ctx, orc):
+    """Footprints past 2^31 (support sums past 2^32): the oracle keeps
+    Python's exact integers (plp/solver.py), as this build does; the
+    reference's g++ build would overflow its int product w * d there."""
+    types = [(60000, 1, 50000, 1, 300, 1, 2), (40000, 1, 45000, 1, 200, 1, 3),
+             (20000, 1, 70000, 1, 250, 1, 2), (90000, 0, 30000, 1, 150, 1, 2)]
+    inst = plp.Instance(100000, 100000, 1000, types)
+    o_inst = (100000, 100000, 1000, types)
+    seeds = list(range(1, 33))
     eng = plp.Engine(ctx)
-    big = plp.Instance(2000, 2000, 2000, [(10, 1, 10, 1, 10, 1, 1)])
+    eng.load([inst], [0] * 32, seeds, plp.params(eps=0.3, theta=0.5))
+    eng.run(2)
+    eng.sync()
+    rows, pl = eng.fetch()
+    eng.close()
+    assert max(r.n_boxes for r in rows) >= 5
+    for k, sd in enumerate(seeds):
+        want = orc.run(o_inst, seed=sd, passes=2, eps=0.3, theta=0.5)
+        assert plp.to_json_dict(rows[k], pl[k]) == {f: want[f] for f in FIELDS}, k
+    row, rpl = ctx.plain_run(inst, 7, 2, plp.params(eps=0.3, theta=0.5))
+    want = orc.run(o_inst, seed=7, passes=2, eps=0.3, theta=0.5)
+    assert plp.to_json_dict(row, rpl) == {f: want[f] for f in FIELDS}
+
+
+def test_feas_check_big_areas_vs_oracle(plp, ctx, orc):
+    rng = np.random.default_rng(11)
+    L, W, H = 100000, 100000, 1000
+    boxes = []
+    for _ in range(40):
+        x, y = (int(v) for v in rng.integers(0, 80000, 2))
+        w, d = (int(v) for v in rng.integers(5000, 20000, 2))
+        z = int(rng.choice([0, 200, 400]))
+        boxes.append((x, y, z, x + w, y + d, z + 200))
+    cand = [(int(rng.integers(0, 90000)), int(rng.integers(0, 90000)), int(rng.choice([0, 200, 400])))
+            for _ in range(300)]
+    ors = [(60000, 50000, 100), (20000, 20000, 100), (90000, 40000, 50)]
+    got = ctx.feas_check_all(cand, ors, boxes, L, W, H)
+    want = orc.check_all(cand, ors, boxes, L, W, H)
+    assert (got == want).all() and got.any()
+
+
+def test_engine_rejects_side_past_21_bits(plp, ctx):
+    eng = plp.Engine(ctx)
+    big = plp.Instance.from_text(_big_cases()["side_over"]["text"])
     with pytest.raises(plp.PlpError) as ei:
         eng.load([big], [0], [0], plp.params())
     assert ei.value.code == plp.PLP_E_RANGE
+    eng.close()
+
+
+def test_block_engine_big_container_vs_oracle(plp, ctx, orc):
+    case = _big_cases()["cube2000"]
+    inst = plp.Instance.from_text(case["text"])
+    o_inst = orc.parse(case["text"])[0]
+    seeds = list(range(1, 17))
+    eng = plp.Engine(ctx)
+    eng.load([inst], [0] * 16, seeds, plp.params(eps=0.3, theta=0.9), blocks=plp.block_params())
+    eng.run(2)
+    eng.sync()
+    rows, pl = eng.fetch()
+    eng.close()
+    for k, sd in enumerate(seeds):
+        want = orc.block_run(o_inst, seed=sd, passes=2, eps=0.3, theta=0.9)
+        assert plp.to_json_dict(rows[k], pl[k]) == {f: want[f] for f in FIELDS}, k
 
 
 # ----------------------------------------------------------- plain form ---
