@@ -6,6 +6,7 @@ missing -- there is no fallback path.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import weakref
@@ -186,11 +187,23 @@ def params(eps=0.3, theta=0.9, mu=0.2, use_theta=True, first_fit=False, order="v
                   0 if order == "volume" else 1, 0)
 
 
+# Contexts and servers still open when the interpreter exits are closed
+# before the HIP runtime's own exit handlers run.
+_OPEN = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    for obj in list(_OPEN):
+        obj.close()
+
+
 class Ctx:
     def __init__(self, device: int = 0):
         self.h = C.c_void_p()
         self._children = weakref.WeakSet()
         _check(_lib.plp_ctx_create(device, C.byref(self.h)))
+        _OPEN.add(self)
 
     def close(self):
         # Engines borrow the context's stream: destroy them first.
@@ -359,6 +372,7 @@ class Server:
         self.h = C.c_void_p()
         _check(_lib.plp_server_create(device, max_inflight, mem_budget, C.byref(self.h)))
         self._req = {}
+        _OPEN.add(self)
 
     def close(self):
         if self.h:

@@ -104,7 +104,7 @@ def test_admission_by_slots_and_memory(plp):
 def test_bad_request_is_rejected_and_server_keeps_serving(plp):
     reqs = _requests(plp)
     srv = plp.Server(0, max_inflight=2)
-    big = plp.Instance(2000, 2000, 2000, [(10, 1, 10, 1, 10, 1, 1)])
+    big = plp.Instance(1 << 21, 2000, 2000, [(10, 1, 10, 1, 10, 1, 1)])   # a side past 21 bits
     with pytest.raises(plp.PlpError) as ei:
         srv.submit(big, 8, 0, 1, plp.params())
     assert ei.value.code == plp.PLP_E_RANGE
