@@ -71,6 +71,22 @@ int plp_feas_check_all(plp_ctx *ctx, const int32_t *cand_xyz, int32_t n_cand,
                        const int32_t *aabb6, int32_t n_aabb,
                        int32_t L, int32_t W, int32_t H, uint8_t *bits_out);
 
+/* plp_feas_check_all without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and errors as plp_feas_check_all, but inputs,
+ * row tables and output bits are staged in a 16 MiB device array that each
+ * device holds for the life of the process, and copied straight from and
+ * to the caller's buffers: the call makes no hipMalloc, hipHostMalloc or
+ * other allocation.  A step needing more than 16 MiB (12 B per candidate,
+ * 24 B per box, 1 B per pair, plus about 1 KB) returns PLP_E_ARG.  Calls on
+ * one device share the array, so calls that overlap in time on one device
+ * race; the array keeps the last call's contents. */
+int plp_feas_check_all_noalloc(plp_ctx *ctx, const int32_t *cand_xyz, int32_t n_cand,
+                               const int32_t *ors_wdh, int32_t n_ors,
+                               const int32_t *aabb6, int32_t n_aabb,
+                               int32_t L, int32_t W, int32_t H, uint8_t *bits_out);
+
 /* ------------------------------------------ seam 2: batched per-step form */
 /* Replaces CudaBatchBackend::check_all_batch (gpu/plp_gpu.cu:174-255): K
  * rows that share one orientation table, each with its own candidates and

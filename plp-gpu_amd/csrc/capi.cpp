@@ -211,6 +211,7 @@ extern "C" int plp_ctx_create(int32_t device, plp_ctx** out) {
     if (device < 0 || device >= n) return fail(PLP_E_ARG, "device index out of range");
     e = hipSetDevice(device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    noalloc_setup(device);
     plp_ctx* c = new plp_ctx();
     c->device = device;
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

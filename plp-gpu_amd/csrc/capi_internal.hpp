@@ -19,6 +19,10 @@ constexpr int32_t kMaxPlainSide = (1 << 30) - 1;
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 int bitlen(int v);
+// Takes this device's address of the allocation-free entry point's scratch
+// array (noalloc.hip); called with the device current.
+void noalloc_setup(int device);
+
 // Validates an instance descriptor (sides up to max_side) and returns the
 // host model.
 int to_instance(const plp_instance_desc* d, plp::Instance* out, int32_t max_side = kMaxSide);

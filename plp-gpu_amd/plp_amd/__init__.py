@@ -27,7 +27,8 @@ PLP_OK, PLP_E_ARG, PLP_E_HIP, PLP_E_NOMEM, PLP_E_EPCAP, PLP_E_RANGE, PLP_E_FORMA
 
 EXPORTS = [
     "plp_status_string", "plp_last_error", "plp_device_count", "plp_parse_instance",
-    "plp_ctx_create", "plp_ctx_destroy", "plp_feas_check_all", "plp_feas_check_batch",
+    "plp_ctx_create", "plp_ctx_destroy", "plp_feas_check_all", "plp_feas_check_all_noalloc",
+    "plp_feas_check_batch",
     "plp_fsr_eval", "plp_rng_script", "plp_plain_run", "plp_engine_create",
     "plp_engine_destroy", "plp_engine_load", "plp_engine_load_blocks", "plp_engine_reset", "plp_engine_run", "plp_engine_sync",
     "plp_engine_max_items", "plp_engine_fetch", "plp_engine_results", "plp_engine_last_ms",
@@ -91,6 +92,8 @@ _sig = {
     "plp_ctx_destroy": (None, [_P]),
     "plp_feas_check_all": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32, _P, C.c_int32,
                                      C.c_int32, C.c_int32, C.c_int32, _P]),
+    "plp_feas_check_all_noalloc": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32, _P, C.c_int32,
+                                             C.c_int32, C.c_int32, C.c_int32, _P]),
     "plp_feas_check_batch": (C.c_int, [_P, C.c_int32, _P, _P, _P, C.c_int32, _P, _P,
                                        C.c_int32, C.c_int32, C.c_int32, _P]),
     "plp_fsr_eval": (C.c_int, [_P, _P, _P, C.c_int32, C.c_double, _P, _P]),
@@ -223,6 +226,16 @@ class Ctx:
         bits = np.zeros(len(cand) * len(ors), np.uint8)
         _check(_lib.plp_feas_check_all(self.h, _ptr(cand), len(cand), _ptr(ors), len(ors),
                                        _ptr(aabb), len(aabb), L, W, H, _ptr(bits)))
+        return bits
+
+    def feas_check_all_noalloc(self, cand, ors, aabb, L, W, H, out=None) -> np.ndarray:
+        """plp_feas_check_all_noalloc; out (uint8, n_cand * n_ors) may be given."""
+        cand = np.ascontiguousarray(np.asarray(cand, np.int32).reshape(-1, 3))
+        ors = np.ascontiguousarray(np.asarray(ors, np.int32).reshape(-1, 3))
+        aabb = np.ascontiguousarray(np.asarray(aabb, np.int32).reshape(-1, 6))
+        bits = np.zeros(len(cand) * len(ors), np.uint8) if out is None else out
+        _check(_lib.plp_feas_check_all_noalloc(self.h, _ptr(cand), len(cand), _ptr(ors), len(ors),
+                                               _ptr(aabb), len(aabb), L, W, H, _ptr(bits)))
         return bits
 
     def feas_check_batch(self, cands, ors, aabbs, L, W, H):
