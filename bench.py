@@ -163,15 +163,17 @@ def serve(plp_amd, synth, seconds: float, rate: float, passes: int, max_inflight
     insts = [plp_amd.Instance.from_text(synth.mixed_instance(s)) for (_, s, _) in arrivals]
     prm = plp_amd.params(eps=EPS, theta=THETA)
     srv = plp_amd.Server(0, max_inflight=max_inflight)
-    # warm every slot's buffers with the largest request shape
+    # warm every slot's buffers with the largest request shape (pass counts
+    # differ, so that no two of these share a launch and each takes a slot)
     big = max(range(len(arrivals)), key=lambda i: arrivals[i][2] * insts[i].n_items)
     warm = []
-    for _ in range(max_inflight):
-        rid = srv.submit(insts[big], arrivals[big][2], 1, passes, prm)
+    for k in range(max_inflight):
+        rid = srv.submit(insts[big], arrivals[big][2], 1, passes + 1 + k, prm)
         if rid is not None:
             warm.append(rid)
     while warm:
         warm = [r for r in warm if srv.poll(r) is None]
+    st0 = srv.stats()
     pending, inflight, lat, done_dec = [], {}, [], 0
     nxt = 0
     t0 = time.perf_counter()
@@ -196,6 +198,7 @@ def serve(plp_amd, synth, seconds: float, rate: float, passes: int, max_inflight
         if not inflight and not pending and nxt < len(arrivals):
             time.sleep(max(0.0, min(0.002, arrivals[nxt][0] - (time.perf_counter() - t0))))
     wall = time.perf_counter() - t0
+    st = srv.stats()
     srv.close()
     lat.sort()
     q = lambda f: lat[min(len(lat) - 1, int(f * len(lat)))]  # noqa: E731
@@ -203,7 +206,9 @@ def serve(plp_amd, synth, seconds: float, rate: float, passes: int, max_inflight
             "passes_per_decoder": passes, "max_inflight": max_inflight,
             "packings_per_s": round(done_dec * passes / wall, 1),
             "p50_ms": round(1e3 * q(0.50), 2), "p99_ms": round(1e3 * q(0.99), 2),
-            "max_ms": round(1e3 * lat[-1], 2)}
+            "max_ms": round(1e3 * lat[-1], 2),
+            "launches": st["launches"] - st0["launches"],
+            "requests_per_launch": round(len(lat) / max(1, st["launches"] - st0["launches"]), 3)}
 
 
 def load_pinned() -> dict:

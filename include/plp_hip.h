@@ -261,23 +261,33 @@ int plp_engine_autotune(plp_engine *eng, int32_t n_passes, plp_engine_config *be
                         float *best_ms);
 
 /* ------------------------------------------------------- serving loop */
-/* Continuous batching: requests (one instance, n_decoders seeds) are
- * admitted while device memory and slots allow, run concurrently on their
- * own streams, and complete independently.  No reference counterpart (the
- * reference runs one instance per process). */
+/* Continuous batching: a request is one instance decoded by n_decoders
+ * seeds.  Admitted requests queue; each poll or query collects finished
+ * launches and then, while one of the max_inflight slots is free, launches
+ * the queued requests that share the head's pass count and parameters
+ * together as one multi-instance engine run (up to 65536 decoders) on that
+ * slot's own stream: the requests submitted since the last poll share a
+ * launch, and launches on different slots overlap on the device.  A request's
+ * results depend only on its own instance, seeds and parameters.  No
+ * reference counterpart (the reference runs one instance per process). */
 typedef struct plp_server plp_server;
 
 int plp_server_create(int32_t device, int32_t max_inflight, int64_t mem_budget_bytes,
                       plp_server **out);
 void plp_server_destroy(plp_server *srv);
-/* Admit a request; PLP_E_BUSY when no slot/memory is free right now.
- * *req_id identifies it.  Seeds are seed0 .. seed0 + n_decoders - 1. */
+/* Admit a request; PLP_E_BUSY when the memory budget (the footprints of
+ * the requests admitted and not yet returned), the device's free memory or
+ * the queue bound (4096 waiting requests) does not allow it now.  *req_id
+ * identifies it.  Seeds are seed0 .. seed0 + n_decoders - 1. */
 int plp_server_submit(plp_server *srv, const plp_instance_desc *inst, int32_t n_decoders,
                       uint64_t seed0, int32_t n_passes, const plp_params *params,
                       int64_t *req_id);
 /* Poll: *done = 1 when the request finished; then rows (n_decoders) and
  * placements (may be NULL, max_items * 7 ints per row) are filled and the
- * request's memory is released.  *max_items receives the row stride. */
+ * request is released.  *max_items receives the row stride: the request's
+ * item count (sum of qty, at least 1).  Polling (and plp_server_query) also
+ * collects finished launches and starts queued requests: a submitted
+ * request starts at the next poll or query of any request. */
 int plp_server_poll(plp_server *srv, int64_t req_id, int32_t *done, plp_row_result *rows,
                     int32_t *placements, int32_t placements_cap_rows, int32_t *max_items);
 /* Requests currently admitted. */
@@ -285,6 +295,9 @@ int plp_server_inflight(plp_server *srv, int32_t *n);
 /* *done = 1 when the request's decode has finished (its results wait for
  * plp_server_poll); does not release anything. */
 int plp_server_query(plp_server *srv, int64_t req_id, int32_t *done);
+/* Launches so far, and the requests and decoders they carried (requests
+ * per launch > 1 means arrivals were coalesced). */
+int plp_server_stats(plp_server *srv, int64_t *launches, int64_t *requests, int64_t *decoders);
 
 #ifdef __cplusplus
 }

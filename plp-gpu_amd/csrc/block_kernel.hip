@@ -494,19 +494,12 @@ hipError_t launch_block(const BlockArgs& b, int waves, int grid, int lds_bytes, 
 
 template <int NW, bool BIG>
 static hipError_t block_occupancy_of(int lds_bytes, int* blocks_per_cu) {
-    hipError_t e = hipFuncSetAttribute((const void*)block_kernel<NW, BIG>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
-    if (e != hipSuccess) return e;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, block_kernel<NW, BIG>,
-                                                        64 * NW, (size_t)lds_bytes);
+    return cached_occupancy((const void*)block_kernel<NW, BIG>, 64 * NW, lds_bytes, blocks_per_cu);
 }
 
 template <int NW, bool BIG>
 static hipError_t block_static_of(int* bytes) {
-    hipFuncAttributes at;
-    hipError_t e = hipFuncGetAttributes(&at, (const void*)block_kernel<NW, BIG>);
-    if (e == hipSuccess) *bytes = (int)at.sharedSizeBytes;
-    return e;
+    return cached_static_lds((const void*)block_kernel<NW, BIG>, bytes);
 }
 
 template <bool BIG>

@@ -43,14 +43,19 @@ struct plp_engine {
     bool blocks = false;   // block-building decoder (plp_engine_load_blocks)
     plp_block_params bp{};
     int max_types = 1;
-    DevBuf d_shape, d_tinfo, d_byt, d_vorder;
+    // Read-only tables of a load, in one device buffer uploaded from one
+    // pinned staging buffer (one copy per load).
+    enum Tab { kInst, kStep, kStepMin, kShape, kTinfo, kByType, kVorder, kOrs, kDecInst, kDecSeed,
+               kOrder, kNumTabs };
+    DevBuf d_tab;
+    HostBuf h_tab;
+    size_t tab_off[kNumTabs] = {};
     plp_params params{};
     int use_rng = 0;
     int passes_done = 0;
     bool need_reset = true;
     bool states_fresh = false;
-    DevBuf d_inst, d_step, d_step_min, d_ors, d_dec_inst, d_dec_seed, d_order, d_retry, d_mt, d_state,
-        d_counter;
+    DevBuf d_retry, d_mt, d_state, d_counter;
     HostBuf h_rows, h_rec;   // results, written by the kernels (fine-grained pinned memory)
     std::vector<int32_t> h_order;
     std::vector<int32_t> h_dec_inst;
@@ -146,6 +151,8 @@ constexpr int32_t kNoBox = 1 << 30;
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
+const void* table(const plp_engine* g, int t) { return (const char*)g->d_tab.p + g->tab_off[t]; }
+
 void drop_graph(plp_engine* g) {
     if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
@@ -217,12 +224,12 @@ int largest_cap(plp_engine* g) {
 plpk::DecodeArgs make_args(plp_engine* g, int n_passes, int reset, const int32_t* order,
                            int n_work) {
     plpk::DecodeArgs a;
-    a.inst = (const InstHdr*)g->d_inst.p;
-    a.step_type = (const int32_t*)g->d_step.p;
-    a.step_min = (const int32_t*)g->d_step_min.p;
-    a.ors_tab = (const int32_t*)g->d_ors.p;
-    a.dec_inst = (const int32_t*)g->d_dec_inst.p;
-    a.dec_seed = (const uint64_t*)g->d_dec_seed.p;
+    a.inst = (const InstHdr*)table(g, plp_engine::kInst);
+    a.step_type = (const int32_t*)table(g, plp_engine::kStep);
+    a.step_min = (const int32_t*)table(g, plp_engine::kStepMin);
+    a.ors_tab = (const int32_t*)table(g, plp_engine::kOrs);
+    a.dec_inst = (const int32_t*)table(g, plp_engine::kDecInst);
+    a.dec_seed = (const uint64_t*)table(g, plp_engine::kDecSeed);
     a.work_order = order;
     a.n_work = n_work;
     a.n_passes = n_passes;
@@ -250,10 +257,10 @@ hipError_t launch(plp_engine* g, const plpk::DecodeArgs& a, int grid, hipStream_
     if (!g->blocks) return plpk::launch_decode(a, g->wpd, grid, g->lds, st);
     plpk::BlockArgs b;
     b.d = a;
-    b.shape = (const plpk::BlockShapeRec*)g->d_shape.p;
-    b.tinfo = (const plpk::BlockTypeRec*)g->d_tinfo.p;
-    b.by_type = (const int32_t*)g->d_byt.p;
-    b.vorder = (const int32_t*)g->d_vorder.p;
+    b.shape = (const plpk::BlockShapeRec*)table(g, plp_engine::kShape);
+    b.tinfo = (const plpk::BlockTypeRec*)table(g, plp_engine::kTinfo);
+    b.by_type = (const int32_t*)table(g, plp_engine::kByType);
+    b.vorder = (const int32_t*)table(g, plp_engine::kVorder);
     b.topk = g->bp.topk;
     b.pick = g->bp.pick;
     b.max_types = g->max_types;
@@ -262,7 +269,8 @@ hipError_t launch(plp_engine* g, const plpk::DecodeArgs& a, int grid, hipStream_
 
 int launch_run(plp_engine* g, int n_passes, int reset) {
     plp_ctx* c = g->ctx;
-    plpk::DecodeArgs a = make_args(g, n_passes, reset, (const int32_t*)g->d_order.p, g->n_dec);
+    plpk::DecodeArgs a = make_args(g, n_passes, reset,
+                                   (const int32_t*)table(g, plp_engine::kOrder), g->n_dec);
     hipError_t e;
     if (g->cfg.use_graph) {
         if (!g->gexec || g->g_passes != n_passes || g->g_reset != reset || g->g_grid != g->grid ||
@@ -345,10 +353,8 @@ extern "C" void plp_engine_destroy(plp_engine* g) {
     (void)hipSetDevice(g->ctx->device);
     (void)hipStreamSynchronize(g->ctx->stream);
     drop_graph(g);
-    for (DevBuf* b : {&g->d_inst, &g->d_step, &g->d_step_min, &g->d_shape, &g->d_tinfo, &g->d_byt,
-                      &g->d_vorder, &g->d_ors, &g->d_dec_inst, &g->d_dec_seed,
-                      &g->d_order, &g->d_retry, &g->d_mt, &g->d_state, &g->d_counter})
-        b->release();
+    for (DevBuf* b : {&g->d_tab, &g->d_retry, &g->d_mt, &g->d_state, &g->d_counter}) b->release();
+    g->h_tab.release();
     g->h_rows.release();
     g->h_rec.release();
     if (g->ev0) (void)hipEventDestroy(g->ev0);
@@ -597,46 +603,42 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
         return cost[(size_t)dec_inst[a]] > cost[(size_t)dec_inst[b]];
     });
     const size_t rows_bytes = (size_t)n_dec * (size_t)max_items * 7 * sizeof(int32_t);
+    struct Up {
+        const void* src;
+        size_t n;
+    } ups[plp_engine::kNumTabs] = {
+        {hdr.data(), sizeof(InstHdr) * hdr.size()},
+        {steps.data(), 4 * steps.size()},
+        {step_min.data(), 4 * step_min.size()},
+        {bt_all.shape.data(), sizeof(plpk::BlockShapeRec) * bt_all.shape.size()},
+        {bt_all.tinfo.data(), sizeof(plpk::BlockTypeRec) * bt_all.tinfo.size()},
+        {bt_all.by_type.data(), 4 * bt_all.by_type.size()},
+        {bt_all.vorder.data(), 4 * bt_all.vorder.size()},
+        {ors.data(), 4 * ors.size()},
+        {dec_inst, 4 * (size_t)n_dec},
+        {dec_seed, 8 * (size_t)n_dec},
+        {g->h_order.data(), 4 * (size_t)n_dec},
+    };
+    size_t tab_bytes = 0;
+    for (int t = 0; t < plp_engine::kNumTabs; t++) {
+        g->tab_off[t] = tab_bytes;
+        tab_bytes += (ups[t].n + 255) & ~(size_t)255;
+    }
     int rc;
-    if ((rc = g->d_inst.reserve(sizeof(InstHdr) * hdr.size(), "instance headers")) ||
-        (rc = g->d_step.reserve(4 * steps.size(), "step table")) ||
-        (rc = g->d_step_min.reserve(4 * step_min.size(), "step minimum dimensions")) ||
-        (rc = g->d_shape.reserve(sizeof(plpk::BlockShapeRec) * bt_all.shape.size(), "block shapes")) ||
-        (rc = g->d_tinfo.reserve(sizeof(plpk::BlockTypeRec) * bt_all.tinfo.size(), "block types")) ||
-        (rc = g->d_byt.reserve(4 * bt_all.by_type.size(), "block shape lists")) ||
-        (rc = g->d_vorder.reserve(4 * bt_all.vorder.size(), "block type order")) ||
-        (rc = g->d_ors.reserve(4 * ors.size(), "orientation table")) ||
-        (rc = g->d_dec_inst.reserve(4 * (size_t)n_dec, "decoder map")) ||
-        (rc = g->d_dec_seed.reserve(8 * (size_t)n_dec, "decoder seeds")) ||
-        (rc = g->d_order.reserve(4 * (size_t)n_dec, "work order")) ||
+    if ((rc = g->d_tab.reserve(tab_bytes, "engine tables")) ||
+        (rc = g->h_tab.reserve(tab_bytes, "pinned table staging")) ||
         (rc = g->d_retry.reserve(4 * (size_t)n_dec, "retry order")) ||
         (rc = g->d_mt.reserve(4 * plpk::kMtWords * (size_t)n_dec, "RNG states")) ||
         (rc = g->d_state.reserve(sizeof(DecState) * (size_t)n_dec, "decoder states")) ||
         (rc = g->h_rows.reserve(rows_bytes, "pinned result packings")) ||
         (rc = g->h_rec.reserve(sizeof(plpk::RowRecord) * (size_t)n_dec, "pinned result records")))
         return rc;
-    struct Up {
-        void* dst;
-        const void* src;
-        size_t n;
-    } ups[] = {
-        {g->d_inst.p, hdr.data(), sizeof(InstHdr) * hdr.size()},
-        {g->d_step.p, steps.data(), 4 * steps.size()},
-        {g->d_step_min.p, step_min.data(), 4 * step_min.size()},
-        {g->d_shape.p, bt_all.shape.data(), sizeof(plpk::BlockShapeRec) * bt_all.shape.size()},
-        {g->d_tinfo.p, bt_all.tinfo.data(), sizeof(plpk::BlockTypeRec) * bt_all.tinfo.size()},
-        {g->d_byt.p, bt_all.by_type.data(), 4 * bt_all.by_type.size()},
-        {g->d_vorder.p, bt_all.vorder.data(), 4 * bt_all.vorder.size()},
-        {g->d_ors.p, ors.data(), 4 * ors.size()},
-        {g->d_dec_inst.p, dec_inst, 4 * (size_t)n_dec},
-        {g->d_dec_seed.p, dec_seed, 8 * (size_t)n_dec},
-        {g->d_order.p, g->h_order.data(), 4 * (size_t)n_dec},
-    };
-    for (const Up& u : ups) {
-        e = hipMemcpyAsync(u.dst, u.src, u.n, hipMemcpyHostToDevice, g->ctx->stream);
-        if (e != hipSuccess) return hip_fail(e, "H2D engine tables");
-    }
-    if ((e = hipStreamSynchronize(g->ctx->stream)) != hipSuccess) return hip_fail(e, "table upload sync");
+    // One copy: the stream has drained (above), so the staging buffer is free;
+    // the decode launches that read the tables follow the copy on the stream.
+    for (int t = 0; t < plp_engine::kNumTabs; t++)
+        if (ups[t].n) std::memcpy((char*)g->h_tab.p + g->tab_off[t], ups[t].src, ups[t].n);
+    e = hipMemcpyAsync(g->d_tab.p, g->h_tab.p, tab_bytes, hipMemcpyHostToDevice, g->ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "H2D engine tables");
     int cap = g->cfg.ep_cap;
     if (cap <= 0) {
         // Automatic capacity: the largest (up to 4 points per item) that

@@ -17,6 +17,12 @@
 //   rng_script_kernel  device MT19937 probe (plp_gpu.cu:642-661)
 #include "kernels.hpp"
 
+#include <map>
+#include <mutex>
+#include <set>
+#include <tuple>
+#include <utility>
+
 #include "decoder_dev.hpp"
 #include "engine_types.hpp"
 #include "mt19937.hpp"
@@ -577,6 +583,60 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int n) {
 
 // ------------------------------------------------------------ launchers --
 
+// A kernel's own __shared__ bytes, remembered per kernel.
+hipError_t cached_static_lds(const void* kernel, int* bytes) {
+    static std::mutex mu;
+    static std::map<const void*, int> memo;
+    std::lock_guard<std::mutex> lk(mu);
+    const auto it = memo.find(kernel);
+    if (it != memo.end()) {
+        *bytes = it->second;
+        return hipSuccess;
+    }
+    hipFuncAttributes at;
+    const hipError_t e = hipFuncGetAttributes(&at, kernel);
+    if (e == hipSuccess) *bytes = memo[kernel] = (int)at.sharedSizeBytes;
+    return e;
+}
+
+// Resident blocks per CU of a kernel for a dynamic LDS size, remembered per
+// (kernel, size): engine loads ask for many sizes, a serving loop for the
+// same ones again and again.  The first query of a kernel also allows it the
+// device's whole per-block LDS, so that any size the occupancy admits can be
+// launched.
+hipError_t cached_occupancy(const void* kernel, int block_threads, int lds_bytes,
+                            int* blocks_per_cu) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, const void*, int>, int> memo;   // (device, kernel, lds)
+    static std::set<std::pair<int, const void*>> opened;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(mu);
+    const auto key = std::make_tuple(dev, kernel, lds_bytes);
+    const auto it = memo.find(key);
+    if (it != memo.end()) {
+        *blocks_per_cu = it->second;
+        return hipSuccess;
+    }
+    if (!opened.count({dev, kernel})) {
+        int max_lds = 0;
+        hipFuncAttributes at;
+        if ((e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)) !=
+            hipSuccess)
+            return e;
+        if ((e = hipFuncGetAttributes(&at, kernel)) != hipSuccess) return e;
+        e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                max_lds - (int)at.sharedSizeBytes);
+        if (e != hipSuccess) return e;
+        opened.insert({dev, kernel});
+    }
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, kernel, block_threads,
+                                                     (size_t)lds_bytes);
+    if (e == hipSuccess) memo[key] = *blocks_per_cu;
+    return e;
+}
+
 template <int M>
 static hipError_t launch_decode_w(const DecodeArgs& a, int waves, int grid, int lds_bytes,
                                   hipStream_t st) {
@@ -623,19 +683,12 @@ hipError_t launch_rng_script(const RngArgs& g, int n_seeds, hipStream_t st) {
 
 template <int NW, int M>
 static hipError_t occupancy_of(int lds_bytes, int* blocks_per_cu) {
-    hipError_t e = hipFuncSetAttribute((const void*)decode_kernel<NW, M>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
-    if (e != hipSuccess) return e;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, decode_kernel<NW, M>,
-                                                        64 * NW, (size_t)lds_bytes);
+    return cached_occupancy((const void*)decode_kernel<NW, M>, 64 * NW, lds_bytes, blocks_per_cu);
 }
 
 template <int NW, int M>
 static hipError_t static_lds_of(int* bytes) {
-    hipFuncAttributes at;
-    hipError_t e = hipFuncGetAttributes(&at, (const void*)decode_kernel<NW, M>);
-    if (e == hipSuccess) *bytes = (int)at.sharedSizeBytes;
-    return e;
+    return cached_static_lds((const void*)decode_kernel<NW, M>, bytes);
 }
 
 template <int M>

@@ -55,6 +55,10 @@ hipError_t decode_static_lds(int waves, int mode, int* bytes);   // the kernel's
 hipError_t block_occupancy(int waves, int mode, int lds_bytes, int* blocks_per_cu);
 hipError_t block_static_lds(int waves, int mode, int* bytes);
 hipError_t fsr_prepare(int lds_bytes);
+// Occupancy and static LDS of a kernel, memoised (kernels.hip).
+hipError_t cached_occupancy(const void* kernel, int block_threads, int lds_bytes,
+                            int* blocks_per_cu);
+hipError_t cached_static_lds(const void* kernel, int* bytes);
 int feas_block_size();
 
 }  // namespace plpk

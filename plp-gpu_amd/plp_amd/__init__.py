@@ -34,7 +34,7 @@ EXPORTS = [
     "plp_engine_max_items", "plp_engine_fetch", "plp_engine_results", "plp_engine_last_ms",
     "plp_engine_describe",
     "plp_engine_autotune", "plp_server_create", "plp_server_destroy", "plp_server_submit",
-    "plp_server_poll", "plp_server_inflight", "plp_server_query",
+    "plp_server_poll", "plp_server_inflight", "plp_server_query", "plp_server_stats",
 ]
 
 
@@ -125,6 +125,8 @@ _sig = {
                                   _I32P]),
     "plp_server_inflight": (C.c_int, [_P, _I32P]),
     "plp_server_query": (C.c_int, [_P, C.c_int64, _I32P]),
+    "plp_server_stats": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int64)]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(_lib, _name)
@@ -432,6 +434,11 @@ class Server:
         n = C.c_int32()
         _check(_lib.plp_server_inflight(self.h, C.byref(n)))
         return n.value
+
+    def stats(self) -> dict:
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(_lib.plp_server_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"launches": a.value, "requests": b.value, "decoders": c.value}
 
 
 def decoder_digest(n_boxes, volume, score, fallback, n_passes, seed, placements) -> bytes:

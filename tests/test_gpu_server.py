@@ -79,26 +79,89 @@ def test_mixed_results_match_oracle_sample(plp):
     srv.close()
 
 
-def test_admission_by_slots_and_memory(plp):
-    reqs = _requests(plp)
-    srv = plp.Server(0, max_inflight=2)
-    (_, inst, n, s0, p, prm) = reqs[2]
-    a = srv.submit(inst, n, s0, p, prm)
-    b = srv.submit(inst, n, s0 + n, p, prm)
-    assert a is not None and b is not None
-    assert srv.submit(inst, n, s0, p, prm) is None          # both slots busy
-    _wait(srv, a, placements=False)
-    c = srv.submit(inst, n, s0, p, prm)                      # a slot freed up
-    assert c is not None
-    _wait(srv, b, placements=False)
-    _wait(srv, c, placements=False)
+def test_queued_requests_share_one_launch(plp):
+    """One slot: a long request runs alone; the ten submitted while it runs
+    wait and then go to the device together, as one multi-instance launch;
+    every result equals its request decoded alone."""
+    from plp_amd import synth
+    prm = plp.params(eps=0.01, theta=0.9)
+    big = plp.Instance.from_text(synth.multi_instance())
+    small = [(plp.Instance.from_text(synth.mixed_instance(9100 + i)), 16 + 7 * i, 50 * i + 3)
+             for i in range(10)]
+    srv = plp.Server(0, max_inflight=1)
+    first = srv.submit(big, 4096, 1, 4, prm)
+    assert srv.poll(first) is None   # the poll launches it (it runs for milliseconds)
+    ids = [srv.submit(inst, n, s0, 4, prm) for (inst, n, s0) in small]
+    assert first is not None and all(i is not None for i in ids)
+    assert srv.inflight() == 11
+    _wait(srv, first, placements=False)
+    got = [_wait(srv, rid) for rid in ids]
+    assert srv.stats() == {"launches": 2, "requests": 11,
+                           "decoders": 4096 + sum(n for (_, n, _) in small)}
     srv.close()
+    ctx = plp.Ctx(0)
+    for (rows, pl), (inst, n, s0) in zip(got, small):
+        assert [plp.to_json_dict(r, q) for r, q in zip(rows, pl)] == \
+            _alone(plp, ctx, inst, n, s0, 4, prm)
+    ctx.close()
+
+
+def test_admission_by_memory(plp):
+    reqs = _requests(plp)
+    (_, inst, n, s0, p, prm) = reqs[2]
     tiny = plp.Server(0, max_inflight=4, mem_budget=1 << 20)
     assert tiny.submit(inst, 4096, 1, 1, prm) is None        # over the memory budget
     small = tiny.submit(inst, 4, 1, 1, prm)
     assert small is not None
     _wait(tiny, small, placements=False)
+    again = tiny.submit(inst, 200, 1, 1, prm)                  # returned requests free the budget
+    assert again is not None
+    _wait(tiny, again, placements=False)
     tiny.close()
+
+
+def test_poisson_burst_each_response_equals_alone(plp):
+    """A replayed burst of Poisson arrivals (1500 requests/s, 256..4096
+    decoders log-uniform, one request of 4096) on 4 slots: requests overlap
+    on the device and queue behind busy slots, and every response equals the
+    same request decoded alone."""
+    import math
+    import random
+    from plp_amd import synth
+    rng = random.Random(77)
+    prm = plp.params(eps=0.01, theta=0.9)
+    arrivals, t = [], 0.0
+    for i in range(40):
+        t += rng.expovariate(1500.0)
+        n = 4096 if i == 5 else int(round(math.exp(rng.uniform(math.log(256), math.log(4096)))))
+        arrivals.append((t, plp.Instance.from_text(synth.mixed_instance(9500 + i)), n,
+                         1 + 10000 * i))
+    srv = plp.Server(0, max_inflight=4)
+    pending, results = {}, {}
+    t0 = time.perf_counter()
+    nxt = 0
+    while nxt < len(arrivals) or pending:
+        now = time.perf_counter() - t0
+        while nxt < len(arrivals) and arrivals[nxt][0] <= now:
+            (_, inst, n, s0) = arrivals[nxt]
+            rid = srv.submit(inst, n, s0, 1, prm)
+            assert rid is not None
+            pending[rid] = nxt
+            nxt += 1
+        for rid in list(pending):
+            res = srv.poll(rid, placements=True)
+            if res is not None:
+                results[pending.pop(rid)] = res
+        assert time.perf_counter() - t0 < 60.0
+    st = srv.stats()
+    srv.close()
+    assert st["requests"] == 40 and st["launches"] <= 40
+    ctx = plp.Ctx(0)
+    for i, (_, inst, n, s0) in enumerate(arrivals):
+        rows, pl = results[i]
+        assert [plp.to_json_dict(r, q) for r, q in zip(rows, pl)] == \
+            _alone(plp, ctx, inst, n, s0, 1, prm), i
+    ctx.close()
 
 
 def test_bad_request_is_rejected_and_server_keeps_serving(plp):
