@@ -286,6 +286,8 @@ def main() -> int:
                     help="length of the Poisson serving run in extras (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-inflight", type=int, default=8)
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the Poisson serving run; prints its JSON")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration)")
     a = ap.parse_args()
@@ -306,6 +308,10 @@ def main() -> int:
     import plp_amd
     from plp_amd import synth
 
+    if a.serve_only:
+        print(json.dumps(serve(plp_amd, synth, a.serve_seconds, a.serve_rate, 1,
+                               a.serve_inflight)), flush=True)
+        return 0
     pinned = load_pinned()
     ctx = plp_amd.Ctx(0)
     prm = plp_amd.params(eps=EPS, theta=THETA)
