@@ -226,11 +226,12 @@ def timed_workload(plp_amd, synth, ctx, name: str, steps: int, warmup: int, shap
     insts = [plp_amd.Instance.from_text(t) for t in texts]
     prm = plp_amd.params(eps=EPS, theta=THETA)
     if shape:
-        wpd, cap, bpc = (int(v) for v in shape.split(","))
+        wpd, cap, bpc, boxes = ([int(v) for v in shape.split(",")] + [0])[:4]
         eng = plp_amd.Engine(ctx, ep_cap=cap, blocks_per_cu=bpc, use_graph=True,
-                             waves_per_decoder=wpd)
+                             waves_per_decoder=wpd, box_cap=boxes)
         eng.load(insts, dec_inst, seeds, prm, blocks=blocks)
-        tuned = {"waves_per_decoder": wpd, "ep_cap": cap, "blocks_per_cu": bpc, "fixed": True}
+        tuned = {"waves_per_decoder": wpd, "ep_cap": cap, "blocks_per_cu": bpc, "box_cap": boxes,
+                 "fixed": True}
     else:
         eng = plp_amd.Engine(ctx, use_graph=True)
         eng.load(insts, dec_inst, seeds, prm, blocks=blocks)
@@ -281,7 +282,8 @@ def main() -> int:
     ap.add_argument("--workload", default=HEADLINE,
                     help="headline workload (plp_amd.synth.workload names)")
     ap.add_argument("--shape", default="",
-                    help="fixed launch shape WPD,EP_CAP,BLOCKS_PER_CU instead of autotuning")
+                    help="fixed launch shape WPD,EP_CAP,BLOCKS_PER_CU[,BOX_CAP] instead of "
+                         "autotuning")
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="length of the Poisson serving run in extras (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)

@@ -151,6 +151,9 @@ typedef struct {
     int32_t count_work;          /* 1: also keep the dead extreme points so that rows report the
                                     reference's list size (max_ep) and brute-force work
                                     (work_pair_box); costs a third list of ep_cap entries */
+    int32_t box_cap;             /* placed boxes a decoder holds in LDS, 0 = every item of the
+                                    largest instance; a decoder that would place one more is
+                                    rerun with room for all (results do not depend on it) */
 } plp_engine_config;
 
 typedef struct {

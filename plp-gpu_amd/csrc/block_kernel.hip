@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
                     st0.passes = 0; st0.err = 0; st0.n_all = -1; st0.n_acc = -1;
                     st0.v_all = 0; st0.v_acc = 0; st0.s_all = 0.0; st0.s_acc = 0.0;
                     st0.work_pair_box = 0; st0.work_live = 0; st0.work_steps = 0; st0.max_ep = 0;
-                    st0.ep_need = 0; st0.pad = 0;
+                    st0.ep_need = 0; st0.box_need = 0;
                 } else {
                     st0 = a.state[d];
                 }

@@ -382,7 +382,8 @@ extern "C" int plp_fsr_eval(plp_ctx* c, const int32_t* rows7, const int32_t* pac
     }
     const int64_t n_rows = pack_off[n_packs];
     if (n_rows > 0 && !rows7) return fail(PLP_E_ARG, "rows7 is NULL");
-    const plpk::LdsLayout lay = plpk::lds_layout(0, box_cap, 0);
+    // the layout fsr_kernel uses: 64-bit contact totals
+    const plpk::LdsLayout lay = plpk::lds_layout(0, box_cap, 0, 1, 0, plpk::kBig, 0);
     if (lay.total > c->max_lds)
         return fail(PLP_E_ARG, "packing too large for one block's LDS (" + std::to_string(box_cap) + " boxes)");
     size_t o_rows = 0;

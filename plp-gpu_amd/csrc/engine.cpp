@@ -37,7 +37,10 @@ struct plp_engine {
     plp_ctx* ctx = nullptr;
     plp_engine_config cfg{};
     bool loaded = false;
-    int n_inst = 0, n_dec = 0, max_items = 1, box_cap = 1;
+    int n_inst = 0, n_dec = 0, max_items = 1;
+    int box_cap = 8;    // placed boxes a decoder holds in LDS (multiple of 8)
+    int full_box = 8;   // every item of the largest instance, rounded up to 8
+    int step_cap = 8;   // step-table entries in LDS
     int ep_cap = 0, lds = 0, grid = 0, occ = 0, wpd = 1;
     int mode = plpk::kNarrow;   // widths of keys, support sums and cache entries (StateMode)
     bool blocks = false;   // block-building decoder (plp_engine_load_blocks)
@@ -168,7 +171,8 @@ int dyn_lds(plp_engine* g, int ep_cap) {
         return plpk::block_lds_layout(ep_cap, g->box_cap, g->max_types, g->use_rng, g->wpd,
                                       g->mode == plpk::kBig)
             .total;
-    return plpk::lds_layout(ep_cap, g->box_cap, g->use_rng, g->wpd, g->cfg.count_work, g->mode)
+    return plpk::lds_layout(ep_cap, g->box_cap, g->use_rng, g->wpd, g->cfg.count_work, g->mode,
+                            g->step_cap)
         .total;
 }
 
@@ -240,6 +244,7 @@ plpk::DecodeArgs make_args(plp_engine* g, int n_passes, int reset, const int32_t
     a.max_items = g->max_items;
     a.ep_cap = g->ep_cap;
     a.box_cap = g->box_cap;
+    a.step_cap = g->step_cap;
     a.use_rng = g->use_rng;
     a.use_theta = g->params.use_theta;
     a.reset = reset;
@@ -322,6 +327,7 @@ extern "C" int plp_engine_create(plp_ctx* ctx, const plp_engine_config* cfg, plp
     if (cfg) g->cfg = *cfg;
     const int wq = g->cfg.waves_per_decoder;
     if ((wq != 0 && wq != 1 && wq != 2 && wq != 4 && wq != 8) || g->cfg.ep_cap < 0 ||
+        g->cfg.box_cap < 0 ||
         g->cfg.blocks_per_cu < 0) {
         delete g;
         return fail(PLP_E_ARG, "config: waves_per_decoder must be 0, 1, 2, 4 or 8; caps >= 0");
@@ -581,7 +587,10 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
     g->n_inst = n_inst;
     g->n_dec = n_dec;
     g->max_items = max_items;
-    g->box_cap = (max_items + 7) / 8 * 8;   // sentinel-padded groups of 8
+    g->full_box = g->step_cap = round_up(max_items, 8);   // sentinel-padded groups of 8
+    // the block engine keeps every item (its blocks decompose into items)
+    g->box_cap = (bp || g->cfg.box_cap <= 0) ? g->full_box
+                                             : std::min(g->full_box, round_up(g->cfg.box_cap, 8));
     g->mode = big ? plpk::kBig : wide ? plpk::kWide : plpk::kNarrow;
     g->params = *params;
     g->blocks = bp != nullptr;
@@ -709,13 +718,13 @@ extern "C" int plp_engine_sync(plp_engine* g) {
     if (*g->h_failed == 0) return PLP_OK;   // no decoder failed: states stay on the device
     int rc = read_states(g);
     if (rc != PLP_OK) return rc;
-    // Capacity escalation: a decoder whose extreme-point list outgrew the
-    // LDS capacity stopped with err set; rerun exactly those decoders from
-    // their seed with the largest capacity one block can hold.  Results do
-    // not depend on the capacity, so the rerun reproduces what an unlimited
-    // run would give.  Afterwards the capacity is the largest list any
-    // decoder needed, which these decoders' later runs fit with the best
-    // residency.
+    // Capacity escalation: a decoder whose extreme-point list or placed
+    // boxes outgrew its LDS capacity stopped with err set; rerun exactly those
+    // decoders from their seed with room for every item and the largest
+    // extreme-point capacity one block can then hold.  Results do not depend
+    // on the capacities, so the rerun reproduces what an unlimited run would
+    // give.  Afterwards the capacities are the most any decoder needed, which
+    // these decoders' later runs fit with the best residency.
     bool escalated = false;
     for (;;) {
         std::vector<int32_t> failed;
@@ -726,8 +735,10 @@ extern "C" int plp_engine_sync(plp_engine* g) {
                 need = std::max(need, (int)g->h_state[(size_t)d].ep_need);
             }
         if (failed.empty()) break;
+        const bool box_grows = g->box_cap < g->full_box;
+        g->box_cap = g->full_box;
         const int cap = largest_cap(g);
-        if (cap <= g->ep_cap || cap < need || shape(g, cap, 1, g->n_dec) != PLP_OK)
+        if ((cap <= g->ep_cap && !box_grows) || cap < need || shape(g, cap, 1, g->n_dec) != PLP_OK)
             return fail(PLP_E_EPCAP, std::to_string(failed.size()) +
                                          " decoder(s) need more than " + std::to_string(g->ep_cap) +
                                          " extreme points, beyond the block LDS limit");
@@ -749,8 +760,12 @@ extern "C" int plp_engine_sync(plp_engine* g) {
         escalated = true;
     }
     if (escalated) {
-        int need_all = 64;
-        for (const DecState& st : g->h_state) need_all = std::max(need_all, (int)st.ep_need);
+        int need_all = 64, box_all = 1;
+        for (const DecState& st : g->h_state) {
+            need_all = std::max(need_all, (int)st.ep_need);
+            box_all = std::max(box_all, (int)st.box_need);
+        }
+        if (!g->blocks) g->box_cap = std::min(g->full_box, round_up(box_all, 8));
         if ((rc = shape(g, round_up(need_all, 32), g->cfg.blocks_per_cu, g->n_dec)) != PLP_OK)
             return rc;
         drop_graph(g);
@@ -828,14 +843,20 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
     if (!g || !g->loaded) return fail(PLP_E_ARG, "engine not loaded");
     if (n_passes < 1) return fail(PLP_E_ARG, "n_passes must be >= 1");
     const plp_engine_config saved = g->cfg;
-    // Probe run: learn the largest extreme-point list of this batch.
+    // Probe run: learn the largest extreme-point list and the most placed
+    // boxes of this batch; every candidate shape then holds exactly those
+    // boxes (less LDS per decoder, more decoders per CU).
     g->need_reset = true;
     int rc = plp_engine_run(g, n_passes);
     if (rc == PLP_OK) rc = plp_engine_sync(g);
     if (rc == PLP_OK && !g->states_fresh) rc = read_states(g);
     if (rc != PLP_OK) return rc;
-    int need = 1;
-    for (const DecState& s : g->h_state) need = std::max(need, (int)s.ep_need);
+    int need = 1, box_need = 1;
+    for (const DecState& s : g->h_state) {
+        need = std::max(need, (int)s.ep_need);
+        box_need = std::max(box_need, (int)s.box_need);
+    }
+    if (!g->blocks) g->box_cap = std::min(g->full_box, round_up(box_need, 8));
     std::vector<int> caps;
     for (int num : {4, 5, 6})   // 1.0x, 1.25x, 1.5x the observed peak
         caps.push_back(std::max(64, round_up(need * num / 4, 32)));
@@ -867,6 +888,7 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
                     bms = g->last_ms;
                     bcfg = g->cfg;
                     bcfg.ep_cap = g->ep_cap;   // escalation may have raised it
+                    bcfg.box_cap = g->box_cap;
                     bcfg.waves_per_decoder = wpd;
                 }
             }
@@ -875,6 +897,8 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
     g->cfg = saved;
     g->cfg.ep_cap = bcfg.ep_cap;
     g->cfg.blocks_per_cu = bcfg.blocks_per_cu;
+    g->cfg.box_cap = bcfg.box_cap;
+    g->box_cap = bcfg.box_cap > 0 ? bcfg.box_cap : g->box_cap;
     g->wpd = bcfg.waves_per_decoder > 0 ? bcfg.waves_per_decoder : saved_wpd;
     rc = shape(g, bcfg.ep_cap, bcfg.blocks_per_cu, g->n_dec);
     if (rc != PLP_OK) return rc;

@@ -49,7 +49,7 @@ struct DecState {
     uint32_t work_steps;     // steps that evaluated feasibility
     uint32_t max_ep;         // largest reference extreme-point list (count_work only)
     uint32_t ep_need;        // largest list this decoder held (the capacity it needs)
-    uint32_t pad;
+    uint32_t box_need;       // most boxes it placed in a pass (box_cap + 1: it ran out)
 };
 static_assert(sizeof(DecState) == 80, "DecState layout");
 
@@ -85,9 +85,10 @@ struct DecodeArgs {
     int32_t* out_rows;       // pinned host: n_dec * max_items * 7, each decoder's selected
                              // packing (JSON row order), rewritten whenever the selection changes
     RowRecord* out_rec;      // pinned host: n_dec result records, written at the end of a run
-    int32_t max_items;
+    int32_t max_items;       // row stride of out_rows (the largest item count)
     int32_t ep_cap;
-    int32_t box_cap;
+    int32_t box_cap;         // placed boxes a decoder holds in LDS; one more stops it (escalation)
+    int32_t step_cap;        // step-table entries in LDS (>= every instance's step count)
     int32_t use_rng;
     int32_t use_theta;
     int32_t reset;           // 1: seed RNG and clear selection state first
@@ -104,7 +105,7 @@ struct DecodeArgs {
 // copies) is reused by FSR after the pass (loads, sort keys, contact
 // totals).  Each list has ep_cap + 8 slots: a merge writes at most 8 entries
 // past the capacity before the overflow is detected.  The box array has
-// box_cap slots plus one sentinel; the step table (box_cap >= steps) follows
+// box_cap slots plus sentinels; the step table (step_cap entries) follows
 // the MT state.
 struct LdsLayout {
     int32_t off_ep, off_ep2, off_oidx, off_cache, n_cache, off_dead, off_mask, n_mask, off_rel;
@@ -123,8 +124,8 @@ __host__ __device__ inline int32_t next_pow2(int32_t v) {
 constexpr int64_t kNarrowMaxArea = 2 * 65534;
 
 __host__ __device__ inline LdsLayout lds_layout(int32_t ep_cap, int32_t box_cap, int32_t use_rng,
-                                                int32_t n_waves = 1, int32_t count_work = 0,
-                                                int32_t mode = kNarrow) {
+                                                int32_t n_waves, int32_t count_work,
+                                                int32_t mode, int32_t step_cap) {
     LdsLayout l;
     const int32_t list = (ep_cap + 8) * key_bytes(mode);
     l.off_ep = 0;
@@ -146,7 +147,7 @@ __host__ __device__ inline LdsLayout lds_layout(int32_t ep_cap, int32_t box_cap,
     l.off_box = (region + 15) & ~15;
     l.off_mt = l.off_box + (box_cap + 8) * 32;
     l.off_steps = l.off_mt + (use_rng ? 624 * 4 : 0);   // per step: type | n_ors << 24
-    l.total = l.off_steps + box_cap * 4;
+    l.total = l.off_steps + step_cap * 4;
     return l;
 }
 

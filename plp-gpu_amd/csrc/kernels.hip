@@ -86,7 +86,8 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
     __shared__ int s_ors[18 * NW];
     __shared__ StepShared<Sum> sh;
     __shared__ DecState sst;
-    const LdsLayout lay = lds_layout(a.ep_cap, a.box_cap, a.use_rng, NW, a.count_work, M);
+    const LdsLayout lay = lds_layout(a.ep_cap, a.box_cap, a.use_rng, NW, a.count_work, M,
+                                     a.step_cap);
     Key* const ep_a = (Key*)(lds + lay.off_ep);
     Key* const ep_b = (Key*)(lds + lay.off_ep2);
     Key* const dead = (Key*)(lds + lay.off_dead);
@@ -132,7 +133,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     st0.passes = 0; st0.err = 0; st0.n_all = -1; st0.n_acc = -1;
                     st0.v_all = 0; st0.v_acc = 0; st0.s_all = 0.0; st0.s_acc = 0.0;
                     st0.work_pair_box = 0; st0.work_live = 0; st0.work_steps = 0; st0.max_ep = 0;
-                    st0.ep_need = 0; st0.pad = 0;
+                    st0.ep_need = 0; st0.box_need = 0;
                 } else {
                     st0 = a.state[d];
                 }
@@ -160,6 +161,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
             int fail_ti = -1;   // type whose last copy found nothing since the last placement
             uint64_t work = 0, live = 0;
             uint32_t n_steps_eval = 0, max_ep = 1, ep_need = 1;
+            bool box_over = false;   // wave 0: the pass ran out of box capacity
             if (wid == 0) {
                 if (lane == 0) {
                     ep_a[0] = Key(0);   // origin; L, W, H > 0 so it is live
@@ -348,6 +350,17 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     continue;
                 }
                 fail_ti = -1;
+                if (nb_w >= a.box_cap) {
+                    // one box past the LDS box capacity: this decoder stops and
+                    // the host reruns it with room for every item
+                    if (wid == 0) {
+                        box_over = true;
+                        if (lane == 0) sh.quit = 1;
+                        advance(s_cur + 1, true);
+                    }
+                    block_sync();
+                    continue;
+                }
 
                 // ---- pick and place (wave 0)
                 int x = 0, y = 0, z = 0, X2 = 0, Y2 = 0, Z2 = 0;
@@ -441,6 +454,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 st.work_steps += n_steps_eval;
                 if (a.count_work) st.max_ep = max(st.max_ep, max_ep);
                 st.ep_need = max(st.ep_need, ep_need);
+                st.box_need = max(st.box_need, box_over ? (uint32_t)a.box_cap + 1u : (uint32_t)n);
                 if (st.err == 0) {
                     const bool accept = better_acc && (!a.use_theta || score >= a.theta);
                     if (better_all) {
@@ -537,7 +551,7 @@ __global__ void __launch_bounds__(64) fsr_kernel(FsrArgs f) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int pk = blockIdx.x;
     const int r0 = f.pack_off[pk], n = f.pack_off[pk + 1] - r0;
-    const LdsLayout lay = lds_layout(0, f.box_cap, 0, 1, 0, kBig);   // 64-bit contact totals
+    const LdsLayout lay = lds_layout(0, f.box_cap, 0, 1, 0, kBig, 0);   // 64-bit totals
     double* load = (double*)(lds + lay.off_load);
     uint64_t* skey = (uint64_t*)(lds + lay.off_key);
     uint64_t* stot = (uint64_t*)(lds + lay.off_tot);

@@ -232,6 +232,31 @@ def test_engine_capacity_escalation(plp, ctx, ref_runs, count_work):
             assert rows[0].max_ep >= c["result"]["max_cand"], c["case"]
 
 
+@pytest.mark.parametrize("box_cap,ep_cap", [(8, 0), (16, 64), (48, 0)])
+def test_engine_box_capacity_escalation(plp, ctx, box_cap, ep_cap):
+    """A box capacity below what decoders place stops them; the engine reruns
+    them with room for every item (and, with a small extreme-point capacity
+    too, more points), then settles on the most boxes any decoder placed.
+    Every result equals the pinned digests of the uneven 256-instance batch
+    (first 512 decoders) and of a graph replay after the settling."""
+    import json as _json
+    from plp_amd import synth
+    texts, dec_inst, seeds, passes = synth.workload("mixed_256x16")
+    with open(golden_path("fullsize_digests.json")) as f:
+        pin = _json.load(f)["workloads"]["mixed_256x16"]["per_decoder"][:512]
+    insts = [plp.Instance.from_text(t) for t in texts[:32]]
+    eng = plp.Engine(ctx, box_cap=box_cap, ep_cap=ep_cap, use_graph=True)
+    eng.load(insts, dec_inst[:512], seeds[:512], plp.params(eps=0.01, theta=0.9))
+    for _ in range(2):
+        eng.reset()
+        eng.run(passes)
+        eng.sync()
+        rows, pl = eng.fetch()
+        assert all(r.err == 0 for r in rows)
+        assert plp.results_digest(rows, pl)[1] == pin
+    eng.close()
+
+
 def test_engine_graph_replay_matches_eager(plp, ctx, ref_runs):
     cases = [c for c in ref_runs if c["case"].startswith("multi_c2_s")]
     insts = [inst_from(plp, cases[0]["instance"])]
