@@ -6,9 +6,11 @@
 // in item-volume order (eps: Fisher-Yates shuffle), each decomposed into
 // blocks (a x b base of one orientation, k high) by taking, while copies
 // remain, the first shape in shape_precedence order whose copy count fits
-// (eps: uniformly among the topk largest that fit).  The plan is sorted into
-// placement order (plp_block.cpp:156-165) by a bitonic network in LDS over
-// the shapes' placement ranks.  Placement (plp_block.cpp:168-183): every
+// (eps: uniformly among the topk largest that fit), read from per-type
+// greedy tables staged in LDS.  The plan is sorted into placement order
+// (plp_block.cpp:156-165) on the shapes' placement ranks: up to 64 blocks by
+// counting ranks in registers, beyond by a bitonic network in LDS.
+// Placement (plp_block.cpp:168-183): every
 // planned block still affordable is tested at every extreme point, in (z, y,
 // x) order, with the decoder's feasibility predicate; the pick is the first
 // feasible point, a random one (eps), or the one with the most bottom support
@@ -124,6 +126,9 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
     Box* const items = (Box*)(lds + lay.off_items);
     Box* const box = (Box*)(lds + lay.off_box);
     uint32_t* const mt = (uint32_t*)(lds + lay.off_mt);
+    BlockTypeRec* const ltr = (BlockTypeRec*)(lds + lay.off_types);
+    int32_t* const lvord = (int32_t*)(lds + lay.off_vord);
+    GreedyEntry* const lgt = (GreedyEntry*)(lds + lay.off_gtab);
     const int lane = lane_id();
     const int wid = (int)(threadIdx.x >> 6);
     Box* const cbox = (Box*)(lds + lay.off_rel) + wid * kCullWindow;
@@ -139,8 +144,15 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
         const InstHdr I = a.inst[uni(a.dec_inst[d])];
         const int L = I.L, W = I.W, H = I.H, xb = I.xbits, yb = I.ybits, nt = I.n_types;
         const BlockShapeRec* const shapes = ba.shape + I.shape_off;
-        const BlockTypeRec* const tr = ba.tinfo + I.tinfo_off;
-        const int32_t* const vord = ba.vorder + I.vorder_off;
+        // the instance's type records, type order and greedy-plan entries go
+        // to LDS once per decoder: the plan reads them in serial chains
+        for (int t = threadIdx.x; t < nt; t += 64 * NW) {
+            ltr[t] = ba.tinfo[I.tinfo_off + t];
+            lvord[t] = ba.vorder[I.vorder_off + t];
+        }
+        for (int e = threadIdx.x; e < I.n_steps; e += 64 * NW) lgt[e] = ba.gtab[I.gtab_off + e];
+        const BlockTypeRec* const tr = ltr;
+        const int32_t* const vord = lvord;
 
         // ---- decoder state: in LDS (sst) between pass ends; wave 0
         int mti = 624;
@@ -222,31 +234,47 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
                         for (int q = 0; q < n_types; q++) {
                             const int t = uni(tlist[q]);
                             int r = uni(rem[t]);
-                            const int off = tr[t].shp_off, size = tr[t].shp_cnt;
+                            const int off = uni(tr[t].shp_off), size = uni(tr[t].shp_cnt);
+                            const int gt = uni(tr[t].gt_off);
                             while (r > 0) {
-                                // shapes are cnt-descending: those that fit
-                                // form a suffix starting after every cnt > r
-                                int start = 0;
-                                for (int j0 = 0; j0 < size; j0 += kWave) {
-                                    const int j = j0 + lane;
-                                    const bool big = j < size && shapes[ba.by_type[off + j]].cnt > r;
-                                    start += __popcll(ballot(big));
-                                }
+                                // the first shape (precedence order) whose copy
+                                // count fits r, from the type's greedy entries
+                                const GreedyEntry g = lgt[gt + r - 1];
+                                const int start = uni(g.start);
                                 if (start == size) break;
                                 const int tk = min(ba.topk, size - start);
                                 int pk = 0;
                                 if (use_eps && mt_random(mt, mti) >= 1.0 - a.eps)
                                     pk = mt_randrange(mt, mti, tk);
-                                const int s = uni(ba.by_type[off + start + pk]);
+                                int s = uni(g.s0), cnt = uni(g.cnt0);
+                                if (pk != 0) {   // an eps pick among the topk largest that fit
+                                    s = uni(ba.by_type[off + start + pk]);
+                                    cnt = uni(shapes[s].cnt);
+                                }
                                 if (lane == 0) plan[n_plan] = (uint32_t)s;
                                 n_plan++;
-                                r -= uni(shapes[s].cnt);
+                                r -= cnt;
                             }
                         }
-                        const int n2 = next_pow2(max(n_plan, 1));
-                        for (int i = n_plan + lane; i < n2; i += kWave) plan[i] = 0xffffffffu;
-                        wave_sync();
-                        bitonic_u32(plan, n2);
+                        if (n_plan <= kWave) {
+                            // placement order in registers: lane i's key goes to
+                            // its rank (equal keys are the same shape, any order)
+                            wave_sync();
+                            const uint32_t key = lane < n_plan ? plan[lane] : 0xffffffffu;
+                            int rank = 0;
+                            for (int j = 0; j < n_plan; j++) {
+                                const uint32_t kj = read_lane(key, j);
+                                rank += (kj < key) | ((kj == key) & (j < lane));
+                            }
+                            wave_sync();
+                            if (lane < n_plan) plan[rank] = key;
+                            wave_sync();
+                        } else {
+                            const int n2 = next_pow2(n_plan);
+                            for (int i = n_plan + lane; i < n2; i += kWave) plan[i] = 0xffffffffu;
+                            wave_sync();
+                            bitonic_u32(plan, n2);
+                        }
                     }
                     if (lane == 0) sh.n_plan = n_plan;
                 }
@@ -257,9 +285,12 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
                 // ---- placement
                 bool any = false;
                 int fail_s = -1;   // shape whose last test found nothing, no block placed since
+                // each test's shape record is read one test ahead
+                BlockShapeRec S_next = shapes[uni((int)plan[0])];
                 for (int q = 0; q < n_plan; q++) {
                     const int s = uni((int)plan[q]);
-                    const BlockShapeRec S = shapes[s];
+                    const BlockShapeRec S = S_next;
+                    if (q + 1 < n_plan) S_next = shapes[uni((int)plan[q + 1])];
                     const int ti = uni(S.ti), cnt_s = uni(S.cnt);
                     const int bw = uni(S.bw), bd = uni(S.bd), bh = uni(S.bh);
                     if (uni(rem[ti]) < cnt_s) continue;   // planned before, stale
@@ -424,7 +455,10 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
                             it.z = uni(B.z) + t * ih;
                             it.x2 = it.x + iw; it.y2 = it.y + id; it.z2 = it.z + ih;
                             it.type = uni(B.type);
-                            it.pad = 0;
+                            // an item below the block's top layer supports the
+                            // one above it; a top-layer item may support
+                            // something only if some block rests on the block
+                            it.pad = ((t < bk - 1) | ((uni(B.pad) & kSupports) != 0)) ? kSupports : 0;
                             items[n + l] = it;
                         }
                         vol += (int64_t)cnt_b * iw * id * ih;
@@ -432,14 +466,19 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
                     }
                     wave_sync();
                     // D4' (plp_block.cpp:228-275): a packing that beats neither
-                    // best cannot be selected, so its score is never needed
+                    // best cannot be selected, so its score is never needed;
+                    // nor is the score of one that beats only the best
+                    // accepted when it provably stays below theta
                     const bool better_all =
                         st.n_all < 0 || vol > st.v_all || (vol == st.v_all && n > st.n_all);
                     const bool better_acc =
                         st.n_acc < 0 || vol > st.v_acc || (vol == st.v_acc && n > st.n_acc);
-                    const double score = better_all || better_acc
-                                             ? fsr_score<4>(items, n, a.mu, load, skey, stot, nullptr)
-                                             : 0.0;
+                    bool need = better_all || better_acc;
+                    if (need && !better_all && a.use_theta && n > 0 &&
+                        (double)fsr_count_bound(items, n, a.mu) / (double)n < a.theta)
+                        need = false;
+                    const double score =
+                        need ? fsr_score<4>(items, n, a.mu, load, skey, stot, nullptr) : 0.0;
                     const bool accept = better_acc && (!a.use_theta || score >= a.theta);
                     if (better_all) {
                         st.n_all = n; st.v_all = vol; st.s_all = score;

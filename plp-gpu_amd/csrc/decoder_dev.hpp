@@ -448,10 +448,41 @@ static __device__ double fsr_score(const Box* __restrict__ box, int n, double mu
     return (double)c / (double)n;
 }
 
+// Bit of Box::pad set on every box that another box rests on (its top at
+// that box's bottom, the footprints overlapping).  The item decoder's pad is
+// otherwise 0; the block decoder keeps a, b and k in bits 0..23.
+constexpr int kSupports = 1 << 30;
+
+// An upper bound on the number of boxes whose FSR (plp_common.hpp:356-390)
+// is >= 1.  compute_fsr adds to load[j] only when box j supports the box
+// being visited, so a box that supports none keeps load = m, and its FSR
+// ((mu / 0.3) * m) / m is computed here in fsr_score's operation order; a
+// supporting box counts as one.  One wave.
+__device__ __forceinline__ int fsr_count_bound(const Box* __restrict__ box, int n, double mu) {
+    int c = 0;
+    for (int i0 = 0; i0 < n; i0 += kWave) {
+        const int i = i0 + lane_id();
+        bool hit = false;
+        if (i < n) {
+            const Box B = box[i];
+            if (B.pad & kSupports) {
+                hit = true;
+            } else {
+                const int64_t v = (int64_t)(B.x2 - B.x) * (B.y2 - B.y) * (B.z2 - B.z);
+                const double m = 50.0 * (double)v / 1.0e6;
+                hit = (mu / 0.3) * m / m >= 1.0;
+            }
+        }
+        c += __popcll(ballot(hit));
+    }
+    return c;
+}
+
 // Corners j of the box [x,X2) x [y,Y2) x [z,Z2) (bit 0 of j -> x side, bit 1
 // -> y, bit 2 -> z) whose smallest later box [jx, jx+wm) x [jy, jy+dm) x
 // [jz, jz+hm) meets one of boxes[0..n_box): bit j of the result (wave-uniform).
-__device__ __forceinline__ uint32_t corner_kill(const Box* __restrict__ box, int n_box, int x, int y,
+// On the way, the boxes the new box rests on get their kSupports mark.
+__device__ __forceinline__ uint32_t corner_kill(Box* __restrict__ box, int n_box, int x, int y,
                                                 int z, int X2, int Y2, int Z2, int wm, int dm,
                                                 int hm) {
     // per lane: the corners its boxes kill, as the AND of per-axis masks
@@ -461,6 +492,8 @@ __device__ __forceinline__ uint32_t corner_kill(const Box* __restrict__ box, int
         const int bi = b0 + lane_id();
         const bool ok = bi < n_box;
         const Box B = box[ok ? bi : b0];
+        if (ok & (B.z2 == z) & (B.x < X2) & (x < B.x2) & (B.y < Y2) & (y < B.y2))
+            box[bi].pad = B.pad | kSupports;
         const uint32_t mx = (((x < B.x2) & (B.x < x + wm)) ? 0x55u : 0u) |
                             (((X2 < B.x2) & (B.x < X2 + wm)) ? 0xaau : 0u);
         const uint32_t my = (((y < B.y2) & (B.y < y + dm)) ? 0x33u : 0u) |
@@ -507,7 +540,7 @@ __device__ __forceinline__ uint32_t corner_present(const Key* src, int n, const 
 // the cap + 8 slots of a list.  One wave; returns the larger new list size.
 template <class Key>
 static __device__ int merge_points(const Key* src, Key* dst, uint16_t* oidx, Key* dead,
-                            int& n_act, int& n_dead, const Box* box, int n_box, int x, int y, int z,
+                            int& n_act, int& n_dead, Box* box, int n_box, int x, int y, int z,
                             int X2, int Y2, int Z2, int L, int W, int H, int xb, int yb, int cap,
                             bool count_work, int wm, int dm, int hm) {
     const int lane = lane_id();

@@ -48,8 +48,8 @@ struct plp_engine {
     int max_types = 1;
     // Read-only tables of a load, in one device buffer uploaded from one
     // pinned staging buffer (one copy per load).
-    enum Tab { kInst, kStep, kStepMin, kShape, kTinfo, kByType, kVorder, kOrs, kDecInst, kDecSeed,
-               kOrder, kNumTabs };
+    enum Tab { kInst, kStep, kStepMin, kShape, kTinfo, kByType, kVorder, kGtab, kOrs, kDecInst,
+               kDecSeed, kOrder, kNumTabs };
     DevBuf d_tab;
     HostBuf h_tab;
     size_t tab_off[kNumTabs] = {};
@@ -266,6 +266,7 @@ hipError_t launch(plp_engine* g, const plpk::DecodeArgs& a, int grid, hipStream_
     b.tinfo = (const plpk::BlockTypeRec*)table(g, plp_engine::kTinfo);
     b.by_type = (const int32_t*)table(g, plp_engine::kByType);
     b.vorder = (const int32_t*)table(g, plp_engine::kVorder);
+    b.gtab = (const plpk::GreedyEntry*)table(g, plp_engine::kGtab);
     b.topk = g->bp.topk;
     b.pick = g->bp.pick;
     b.max_types = g->max_types;
@@ -380,6 +381,7 @@ struct BlockTables {
     std::vector<plpk::BlockTypeRec> tinfo;
     std::vector<int32_t> by_type;   // instance-local shape indices
     std::vector<int32_t> vorder;
+    std::vector<plpk::GreedyEntry> gtab;   // per type, r = 1 .. qty
 };
 
 void block_tables(const plp::Instance& in, const plp_block_params& bp, int32_t by_type_base,
@@ -446,6 +448,19 @@ void block_tables(const plp::Instance& in, const plp_block_params& bp, int32_t b
         tinfo[(size_t)ti].shp_off = by_type_base + (int32_t)out->by_type.size();
         tinfo[(size_t)ti].shp_cnt = (int32_t)mine.size();
         for (int i : mine) out->by_type.push_back(rank[(size_t)i]);
+        // the precedence list is cnt-descending, so the shapes that fit r
+        // copies are the suffix after every cnt > r
+        tinfo[(size_t)ti].gt_off = (int32_t)out->gtab.size();
+        std::vector<plpk::GreedyEntry> col((size_t)std::max(0, in.items[(size_t)ti].qty));
+        for (int r = (int)col.size(); r >= 1; r--) {
+            size_t start = 0;
+            while (start < mine.size() && cat[(size_t)mine[start]].cnt > r) start++;
+            plpk::GreedyEntry& e = col[(size_t)r - 1];
+            e.start = (int32_t)start;
+            e.s0 = start < mine.size() ? rank[(size_t)mine[start]] : -1;
+            e.cnt0 = start < mine.size() ? cat[(size_t)mine[start]].cnt : 0;
+        }
+        out->gtab.insert(out->gtab.end(), col.begin(), col.end());
     }
     out->tinfo = tinfo;
     out->vorder.resize((size_t)nt);
@@ -541,10 +556,12 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
             h.shape_off = (int32_t)bt_all.shape.size();
             h.tinfo_off = (int32_t)bt_all.tinfo.size();
             h.vorder_off = (int32_t)bt_all.vorder.size();
+            h.gtab_off = (int32_t)bt_all.gtab.size();
             bt_all.shape.insert(bt_all.shape.end(), bt.shape.begin(), bt.shape.end());
             bt_all.tinfo.insert(bt_all.tinfo.end(), bt.tinfo.begin(), bt.tinfo.end());
             bt_all.by_type.insert(bt_all.by_type.end(), bt.by_type.begin(), bt.by_type.end());
             bt_all.vorder.insert(bt_all.vorder.end(), bt.vorder.begin(), bt.vorder.end());
+            bt_all.gtab.insert(bt_all.gtab.end(), bt.gtab.begin(), bt.gtab.end());
         }
         steps.insert(steps.end(), seq.begin(), seq.end());
         // Componentwise smallest orientation of every type; a point where a
@@ -602,6 +619,7 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
     if (bt_all.tinfo.empty()) bt_all.tinfo.push_back(plpk::BlockTypeRec{});
     if (bt_all.by_type.empty()) bt_all.by_type.push_back(0);
     if (bt_all.vorder.empty()) bt_all.vorder.push_back(0);
+    if (bt_all.gtab.empty()) bt_all.gtab.push_back(plpk::GreedyEntry{});
     g->h_dec_inst.assign(dec_inst, dec_inst + n_dec);
     g->h_seed.assign(dec_seed, dec_seed + n_dec);
     // Work order: most expensive instances first (longest-processing-time
@@ -623,6 +641,7 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
         {bt_all.tinfo.data(), sizeof(plpk::BlockTypeRec) * bt_all.tinfo.size()},
         {bt_all.by_type.data(), 4 * bt_all.by_type.size()},
         {bt_all.vorder.data(), 4 * bt_all.vorder.size()},
+        {bt_all.gtab.data(), sizeof(plpk::GreedyEntry) * bt_all.gtab.size()},
         {ors.data(), 4 * ors.size()},
         {dec_inst, 4 * (size_t)n_dec},
         {dec_seed, 8 * (size_t)n_dec},

@@ -17,7 +17,8 @@ struct InstHdr {
     // block-building engine only: first block shape, type record and entry of
     // the volume-ordered type list of this instance
     int32_t shape_off, tinfo_off, vorder_off;
-    int32_t pad[4];
+    int32_t gtab_off;   // block engine: first greedy-plan entry of this instance
+    int32_t pad[3];
 };
 static_assert(sizeof(InstHdr) == 64, "InstHdr layout");
 
@@ -163,9 +164,18 @@ struct BlockShapeRec {
 
 // Per (instance, type): copies, whether any orientation is allowed, the
 // type's shapes in shape_precedence order (by_type[shp_off .. +shp_cnt)),
-// and the componentwise smallest orientation.
+// the componentwise smallest orientation, and where the type's greedy-plan
+// entries start (instance-local entry index of r = 1 copies left).
 struct BlockTypeRec {
-    int32_t qty, has_ors, shp_off, shp_cnt, wmin, dmin, hmin, pad;
+    int32_t qty, has_ors, shp_off, shp_cnt, wmin, dmin, hmin, gt_off;
+};
+
+// Greedy plan entry of (type, r copies left): the first shape in
+// shape_precedence order whose copy count fits r is the start-th of the
+// type's list (start == shp_cnt: none fits); s0 is that shape (-1 if none)
+// and cnt0 its copy count.
+struct GreedyEntry {
+    int32_t start, s0, cnt0;
 };
 
 struct BlockArgs {
@@ -174,6 +184,7 @@ struct BlockArgs {
     const BlockTypeRec* tinfo;     // all instances' types (InstHdr::tinfo_off)
     const int32_t* by_type;        // shape indices per type, precedence order
     const int32_t* vorder;         // per instance: types by item volume desc, index asc
+    const GreedyEntry* gtab;       // per instance and type, r = 1 .. qty (InstHdr::gtab_off)
     int32_t topk;                  // eps-greedy choice among the topk largest fitting shapes
     int32_t pick;                  // 0 first, 1 support, 2 contact (plp_block.cpp:56)
     int32_t max_types;             // largest type count of the batch
@@ -183,11 +194,11 @@ struct BlockArgs {
 // index, verdict masks, remaining copies and the type list, the sweep plan
 // (sorted in place), per-wave culled box windows; after the pass the first
 // region holds FSR's arrays and the decomposed items.  Then the placed
-// blocks and the MT state.
+// blocks, the MT state and the instance's type tables.
 struct BlockLds {
     int32_t off_ep, off_ep2, off_oidx, off_mask, off_rem, off_tlist, off_plan, n_plan, off_rel;
     int32_t off_load, off_key, n_key, off_tot, off_items;
-    int32_t off_box, off_mt, total;
+    int32_t off_box, off_mt, off_types, off_vord, off_gtab, total;
 };
 
 __host__ __device__ inline BlockLds block_lds_layout(int32_t ep_cap, int32_t box_cap,
@@ -214,7 +225,12 @@ __host__ __device__ inline BlockLds block_lds_layout(int32_t ep_cap, int32_t box
     const int32_t region = pass_end > fsr_end ? pass_end : fsr_end;
     l.off_box = (region + 15) & ~15;
     l.off_mt = l.off_box + (box_cap + 8) * 32;
-    l.total = l.off_mt + (use_rng ? 624 * 4 : 0);
+    // the instance's type records, volume order and greedy-plan entries
+    // (one per item: box_cap >= the item count), staged per decoder
+    l.off_types = l.off_mt + (use_rng ? 624 * 4 : 0);
+    l.off_vord = l.off_types + max_types * (int32_t)sizeof(BlockTypeRec);
+    l.off_gtab = l.off_vord + max_types * 4;
+    l.total = l.off_gtab + box_cap * (int32_t)sizeof(GreedyEntry);
     return l;
 }
 

@@ -444,9 +444,16 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                                         (vol == uni(sst.v_all) && n > uni(sst.n_all));
                 const bool better_acc = uni(sst.n_acc) < 0 || vol > uni(sst.v_acc) ||
                                         (vol == uni(sst.v_acc) && n > uni(sst.n_acc));
+                // Only a packing that would enter the accepted pool can change
+                // the selection unless it beats the best of all; when the
+                // count of boxes that can reach FSR >= 1 keeps its score below
+                // theta, it cannot, and its score is never needed.
+                bool need = scored && (better_all || better_acc);
+                if (need && !better_all && a.use_theta && n > 0 &&
+                    (double)fsr_count_bound(box, n, a.mu) / (double)n < a.theta)
+                    need = false;
                 double score = 0.0;
-                if (scored && (better_all || better_acc))
-                    score = fsr_score(box, n, a.mu, load, skey, stot, nullptr);
+                if (need) score = fsr_score(box, n, a.mu, load, skey, stot, nullptr);
                 DecState st = sst;
                 if (overflow) st.err = 4;   // PLP_E_EPCAP
                 st.work_pair_box += work;
