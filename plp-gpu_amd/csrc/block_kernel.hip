@@ -103,9 +103,12 @@ __device__ void bitonic_u32(uint32_t* key, int n2) {
 }  // namespace
 
 // BIG: 64-bit extreme-point keys and support sums (StateMode kBig), else
-// 32-bit ones.
+// 32-bit ones.  The bound asks for at most 168 VGPRs (three waves per SIMD):
+// the 32-bit form then needs no scratch memory.  A block decoder's time is
+// the latency of its serial chain (one wave per decoder runs as fast as
+// four), so residency beyond that buys nothing.
 template <int NW, bool BIG>
-__global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
+__global__ void __launch_bounds__(64 * NW, 3) block_kernel(BlockArgs ba) {
     using Key = typename ModeTypes<BIG ? kBig : kNarrow>::Key;
     using Sum = typename ModeTypes<BIG ? kBig : kNarrow>::Sum;
     const DecodeArgs& a = ba.d;
@@ -431,13 +434,19 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
                 if (quit || !any) break;
             }
 
-            // ---- pass end (wave 0): decomposed items, FSR, D4'
+            // ---- pass end (wave 0): decomposed items, FSR, D4'.  The state
+            // in LDS (sst) is updated field by field (a whole-struct copy in
+            // registers would partly live in scratch memory).
             if (wid == 0) {
                 const bool overflow = uni(sh.quit) != 0;
-                DecState st = sst;
-                if (overflow) st.err = 4;   // PLP_E_EPCAP
-                st.ep_need = max(st.ep_need, ep_need);
-                if (st.err == 0) {
+                const int err = overflow ? 4 : uni(sst.err);   // 4: PLP_E_EPCAP
+                const uint32_t need_ep = max(uni(sst.ep_need), ep_need);
+                wave_sync();
+                if (lane == 0) {
+                    sst.err = err;
+                    sst.ep_need = need_ep;
+                }
+                if (err == 0) {
                     // blocks -> items in block order, i (x) outer, j (y), t (z) inner
                     int n = 0;
                     int64_t vol = 0;
@@ -469,10 +478,12 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
                     // best cannot be selected, so its score is never needed;
                     // nor is the score of one that beats only the best
                     // accepted when it provably stays below theta
+                    const int n_all = uni(sst.n_all), n_acc = uni(sst.n_acc);
+                    const int64_t v_all = uni(sst.v_all), v_acc = uni(sst.v_acc);
                     const bool better_all =
-                        st.n_all < 0 || vol > st.v_all || (vol == st.v_all && n > st.n_all);
+                        n_all < 0 || vol > v_all || (vol == v_all && n > n_all);
                     const bool better_acc =
-                        st.n_acc < 0 || vol > st.v_acc || (vol == st.v_acc && n > st.n_acc);
+                        n_acc < 0 || vol > v_acc || (vol == v_acc && n > n_acc);
                     bool need = better_all || better_acc;
                     if (need && !better_all && a.use_theta && n > 0 &&
                         (double)fsr_count_bound(items, n, a.mu) / (double)n < a.theta)
@@ -480,18 +491,20 @@ __global__ void __launch_bounds__(64 * NW, 4) block_kernel(BlockArgs ba) {
                     const double score =
                         need ? fsr_score<4>(items, n, a.mu, load, skey, stot, nullptr) : 0.0;
                     const bool accept = better_acc && (!a.use_theta || score >= a.theta);
-                    if (better_all) {
-                        st.n_all = n; st.v_all = vol; st.s_all = score;
-                    }
-                    if (accept) {
-                        st.n_acc = n; st.v_acc = vol; st.s_acc = score;
-                    }
                     // the selection only moves to the packing just decoded
-                    if (accept || (st.n_acc < 0 && better_all)) write_rows(out_sel, items, n);
-                    st.passes++;
+                    if (accept || (n_acc < 0 && better_all)) write_rows(out_sel, items, n);
+                    wave_sync();
+                    if (lane == 0) {
+                        if (better_all) {
+                            sst.n_all = n; sst.v_all = vol; sst.s_all = score;
+                        }
+                        if (accept) {
+                            sst.n_acc = n; sst.v_acc = vol; sst.s_acc = score;
+                        }
+                        sst.passes = sst.passes + 1;
+                    }
                 }
                 wave_sync();
-                if (lane == 0) sst = st;
             }
             block_sync();
             if (uni(sh.quit) != 0) break;
