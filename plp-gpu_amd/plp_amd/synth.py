@@ -101,7 +101,11 @@ def workload(name: str):
       mixed_256x16      256 instances of 3..100 types x 16 seeds (4096 decoders), 1 pass
       hundred_2048      the 100-type instance (587x233x220), 2048 decoders, 1 pass
       hundred_fit_2048  the 100-type instance in 880x349x330, 2048 decoders, 1 pass
+      blocks_twenty_1024x4  twenty_1024x4 decoded by the block-building engine
+                        (reference block options, plp_amd.block_params())
     All with eps 0.01 and theta 0.9."""
+    if name.startswith("blocks_"):
+        return workload(name[len("blocks_"):])
     if name == "twenty_1024x4":
         return [multi_instance()], [0] * 1024, list(range(1, 1025)), 4
     if name == "mixed_256x16":
@@ -114,7 +118,12 @@ def workload(name: str):
     raise KeyError(name)
 
 
-WORKLOADS = ("twenty_1024x4", "mixed_256x16", "hundred_2048", "hundred_fit_2048")
+WORKLOADS = ("twenty_1024x4", "mixed_256x16", "hundred_2048", "hundred_fit_2048",
+             "blocks_twenty_1024x4")
+
+
+def is_block_workload(name: str) -> bool:
+    return name.startswith("blocks_")
 
 
 def parse_types(text: str):

@@ -5,8 +5,9 @@ of the full-size benchmark workloads (run in the build container).
 For each workload of plp_amd.synth.workload() every decoder is decoded twice
 on the host, by the C oracle (oracle/build/liboracle.so) and by the
 reference's own g++ build (oracle/_ref/ref_driver, compiled from
-gpu/plp_common.hpp by `make -C oracle ref`), and the two must agree decoder
-for decoder.  The fixture keeps each decoder's digest (plp_amd.decoder_digest
+gpu/plp_common.hpp by `make -C oracle ref`; for the block-building workload
+the block oracle and oracle/_ref/block_driver over blockproof/plp_block.cpp),
+and the two must agree decoder for decoder.  The fixture keeps each decoder's digest (plp_amd.decoder_digest
 of the reference JSON fields and rows, first 16 hex digits) and the digest of
 the whole workload, which the GPU tests and bench.py compare the engine's
 results with.
@@ -34,15 +35,16 @@ from plp_amd import decoder_digest, synth  # noqa: E402
 from oracle_lib import Oracle  # noqa: E402
 
 REF_DRIVER = os.path.join(REPO, "oracle", "_ref", "ref_driver")
+BLOCK_DRIVER = os.path.join(REPO, "oracle", "_ref", "block_driver")
 OUT = os.path.join(HERE, "fullsize_digests.json")
 
 
-def oracle_digests(orc, texts, dec_inst, seeds, passes, threads):
+def oracle_digests(orc, texts, dec_inst, seeds, passes, threads, blocks=False):
     insts = [orc.parse(t)[0] for t in texts]
 
     def one(k):
-        r = orc.run(insts[dec_inst[k]], seed=seeds[k], passes=passes, eps=synth.EPS,
-                    theta=synth.THETA)
+        run = orc.block_run if blocks else orc.run
+        r = run(insts[dec_inst[k]], seed=seeds[k], passes=passes, eps=synth.EPS, theta=synth.THETA)
         return decoder_digest(r["n_boxes"], r["volume_cm3"], r["stability_score"], r["fallback"],
                               r["n_passes"], r["seed"], r["placements"])
 
@@ -50,7 +52,7 @@ def oracle_digests(orc, texts, dec_inst, seeds, passes, threads):
         return list(ex.map(one, range(len(seeds))))
 
 
-def reference_digests(texts, dec_inst, seeds, passes, threads):
+def reference_digests(texts, dec_inst, seeds, passes, threads, blocks=False):
     """ref_driver rows over runs of consecutive seeds of one instance."""
     jobs = []   # (instance, first seed, count, first decoder)
     k = 0
@@ -71,9 +73,10 @@ def reference_digests(texts, dec_inst, seeds, passes, threads):
 
     def run(job):
         inst, s0, n, k0 = job
-        p = subprocess.run([REF_DRIVER, "rows", paths[inst], "--seed", str(s0), "--K", str(n),
-                            "--passes", str(passes), "--eps", str(synth.EPS), "--theta",
-                            str(synth.THETA)], check=True, capture_output=True, text=True)
+        p = subprocess.run([BLOCK_DRIVER if blocks else REF_DRIVER, "rows", paths[inst], "--seed",
+                            str(s0), "--K", str(n), "--passes", str(passes), "--eps",
+                            str(synth.EPS), "--theta", str(synth.THETA)],
+                           check=True, capture_output=True, text=True)
         for i, line in enumerate(p.stdout.splitlines()):
             r = json.loads(line)
             out[k0 + i] = decoder_digest(r["n_boxes"], r["volume_cm3"], r["stability_score"],
@@ -104,10 +107,11 @@ def main() -> None:
         if a.only and name != a.only:
             continue
         texts, dec_inst, seeds, passes = synth.workload(name)
+        blocks = synth.is_block_workload(name)
         t0 = time.time()
-        mine = oracle_digests(orc, texts, dec_inst, seeds, passes, a.threads)
+        mine = oracle_digests(orc, texts, dec_inst, seeds, passes, a.threads, blocks)
         t1 = time.time()
-        ref = reference_digests(texts, dec_inst, seeds, passes, a.threads)
+        ref = reference_digests(texts, dec_inst, seeds, passes, a.threads, blocks)
         t2 = time.time()
         bad = [k for k in range(len(seeds)) if mine[k] != ref[k]]
         print(f"{name}: {len(seeds)} decoders, oracle {t1 - t0:.1f} s, reference {t2 - t1:.1f} s, "
@@ -118,7 +122,9 @@ def main() -> None:
             "passes": passes,
             "digest": hashlib.sha256(b"".join(mine)).hexdigest(),
             "per_decoder": [d.hex()[:16] for d in mine],
-            "source": "C oracle and reference g++ build (ref_driver rows), identical per decoder",
+            "source": ("C block oracle and reference g++ block engine (block_driver rows)"
+                       if blocks else "C oracle and reference g++ build (ref_driver rows)")
+                      + ", identical per decoder",
         }
     with open(OUT, "w") as f:
         json.dump(fixture, f, indent=0)

@@ -212,4 +212,5 @@ def test_block_engine_1024_decoders_4_passes(plp, ctx, orc):
     # C3 holds for each block's footprint (blockproof/plp_block.cpp tests the
     # block shape), not for every item of a block's bottom layer
     _check_all(plp, ctx, [inst], [0] * K, rows, pl, THETA, per_box_support=False)
+    _compare_pinned(plp, "blocks_twenty_1024x4", rows, pl)
     _compare(plp, rows, pl, _oracle_all(orc, [inst], [0] * K, seeds, P, block=True))
