@@ -190,8 +190,11 @@ int plp_engine_create(plp_ctx *ctx, const plp_engine_config *cfg, plp_engine **o
 void plp_engine_destroy(plp_engine *eng);
 
 /* Upload instances and decoders; resets every decoder (RNG seeded, no
- * packing yet).  Instances must satisfy L, W, H < 2^21 and
- * bitlen(L) + bitlen(W) + bitlen(H) <= 31 (PLP_E_RANGE otherwise). */
+ * packing yet).  Every side must be at most 2^21 - 1 (PLP_E_RANGE
+ * otherwise), the device form's range (gpu/plp_gpu.cu:1263-1268).  When some
+ * instance has bitlen(L) + bitlen(W) + bitlen(H) > 31 the batch runs with
+ * 64-bit extreme-point keys and support sums, else with 32-bit ones; the
+ * results are the same. */
 int plp_engine_load(plp_engine *eng, const plp_instance_desc *insts, int32_t n_inst,
                     const int32_t *dec_inst, const uint64_t *dec_seed, int32_t n_dec,
                     const plp_params *params);
