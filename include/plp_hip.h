@@ -154,6 +154,12 @@ typedef struct {
     int32_t box_cap;             /* placed boxes a decoder holds in LDS, 0 = every item of the
                                     largest instance; a decoder that would place one more is
                                     rerun with room for all (results do not depend on it) */
+    int32_t verdicts;            /* item decoder's verdicts: 1 = pair states of the last two
+                                    steps, reused when the type repeats, other steps scanning
+                                    the boxes; 2 = one orientation-free state per extreme
+                                    point, kept current as boxes are placed; 0 = automatic
+                                    (2, and plp_engine_autotune tries both).  Results do not
+                                    depend on it. */
 } plp_engine_config;
 
 typedef struct {

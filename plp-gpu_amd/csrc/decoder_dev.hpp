@@ -538,11 +538,23 @@ __device__ __forceinline__ uint32_t corner_present(const Key* src, int n, const 
 // live list without being strictly inside and the corners that are neither
 // useful nor already listed.  n_act <= cap on entry keeps every write within
 // the cap + 8 slots of a list.  One wave; returns the larger new list size.
+//
+// With point states (SlotMerge given) each list entry carries the slot of
+// its state instead of oidx: a kept point keeps its slot and an inserted
+// corner j takes the slot lane j holds in sm->cslot.  (The slots of the
+// points that leave are freed by the wave that updates the states, which
+// applies the same rule.)
+struct SlotMerge {
+    const uint16_t* src;   // slot of each src entry
+    uint16_t* dst;         // slot of each dst entry
+    uint32_t cslot;        // lane j < 8: the slot of corner j
+};
+
 template <class Key>
 static __device__ int merge_points(const Key* src, Key* dst, uint16_t* oidx, Key* dead,
                             int& n_act, int& n_dead, Box* box, int n_box, int x, int y, int z,
                             int X2, int Y2, int Z2, int L, int W, int H, int xb, int yb, int cap,
-                            bool count_work, int wm, int dm, int hm) {
+                            bool count_work, int wm, int dm, int hm, SlotMerge* sm = nullptr) {
     const int lane = lane_id();
     // Corner j: bit 0 -> x side, bit 1 -> y, bit 2 -> z; with the key's field
     // order (z, then y, then x) the keys increase with j.  Lane j < 8 holds
@@ -612,7 +624,13 @@ static __device__ int merge_points(const Key* src, Key* dst, uint16_t* oidx, Key
         }
         const bool keep = ok & !gone & (eq == 0);
         const uint64_t km = ballot(keep);
-        if (keep) {
+        if (sm) {
+            if (keep) {
+                const int pos = base + rank_below(km) + shift;
+                dst[pos] = v;
+                sm->dst[pos] = sm->src[i];
+            }
+        } else if (keep) {
             const int pos = base + rank_below(km) + shift;
             dst[pos] = v;
             oidx[pos] = (uint16_t)i;
@@ -636,7 +654,8 @@ static __device__ int merge_points(const Key* src, Key* dst, uint16_t* oidx, Key
     if (lane < 8 && ((ins >> lane) & 1u)) {
         const int pos = klt + __popc(ins & ((1u << lane) - 1u));
         dst[pos] = myk;
-        oidx[pos] = (uint16_t)(kCorner + lane);
+        if (sm) sm->dst[pos] = (uint16_t)sm->cslot;
+        else oidx[pos] = (uint16_t)(kCorner + lane);
     }
     if (count_work) {
         const uint32_t dadd = cdead & ~present & ~eq_dead;

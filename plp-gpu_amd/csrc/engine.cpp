@@ -43,6 +43,7 @@ struct plp_engine {
     int step_cap = 8;   // step-table entries in LDS
     int ep_cap = 0, lds = 0, grid = 0, occ = 0, wpd = 1;
     int mode = plpk::kNarrow;   // widths of keys, support sums and cache entries (StateMode)
+    int verd = plpk::kPointState;   // how the item decoder reaches its verdicts (VerdictKind)
     bool blocks = false;   // block-building decoder (plp_engine_load_blocks)
     plp_block_params bp{};
     int max_types = 1;
@@ -69,7 +70,7 @@ struct plp_engine {
     float last_ms = 0.0f;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int g_passes = -1, g_reset = -1, g_grid = -1, g_cap = -1, g_wpd = -1;
+    int g_passes = -1, g_reset = -1, g_grid = -1, g_cap = -1, g_wpd = -1, g_verd = -1;
 };
 
 // The kernels write plp_row_result records as plpk::RowRecord.
@@ -161,7 +162,7 @@ void drop_graph(plp_engine* g) {
     if (g->graph) (void)hipGraphDestroy(g->graph);
     g->gexec = nullptr;
     g->graph = nullptr;
-    g->g_passes = g->g_reset = g->g_grid = g->g_cap = g->g_wpd = -1;
+    g->g_passes = g->g_reset = g->g_grid = g->g_cap = g->g_wpd = g->g_verd = -1;
 }
 
 // Dynamic LDS bytes of one decoder block for a capacity, for the loaded
@@ -170,6 +171,10 @@ int dyn_lds(plp_engine* g, int ep_cap) {
     if (g->blocks)
         return plpk::block_lds_layout(ep_cap, g->box_cap, g->max_types, g->use_rng, g->wpd,
                                       g->mode == plpk::kBig)
+            .total;
+    if (g->verd == plpk::kPointState)
+        return plpk::pts_layout(ep_cap, g->box_cap, g->use_rng, g->cfg.count_work, g->mode,
+                                g->step_cap)
             .total;
     return plpk::lds_layout(ep_cap, g->box_cap, g->use_rng, g->wpd, g->cfg.count_work, g->mode,
                             g->step_cap)
@@ -180,7 +185,7 @@ int dyn_lds(plp_engine* g, int ep_cap) {
 int static_lds(plp_engine* g) {
     int st = 0;
     hipError_t e = g->blocks ? plpk::block_static_lds(g->wpd, g->mode, &st)
-                             : plpk::decode_static_lds(g->wpd, g->mode, &st);
+                             : plpk::decode_static_lds(g->wpd, g->mode, g->verd, &st);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         st = 4096;
@@ -202,7 +207,7 @@ int shape(plp_engine* g, int ep_cap, int blocks_per_cu_req, int n_work) {
                                      " extreme points) exceeds the block LDS limit");
     int occ = 0;
     hipError_t e = g->blocks ? plpk::block_occupancy(g->wpd, g->mode, lds, &occ)
-                             : plpk::decode_occupancy(g->wpd, g->mode, lds, &occ);
+                             : plpk::decode_occupancy(g->wpd, g->mode, g->verd, lds, &occ);
     if (e != hipSuccess) return hip_fail(e, "decoder occupancy");
     if (occ < 1) return fail(PLP_E_EPCAP, "the decoder cannot be resident with this LDS size");
     int bpc = blocks_per_cu_req > 0 ? std::min(blocks_per_cu_req, occ) : occ;
@@ -250,6 +255,7 @@ plpk::DecodeArgs make_args(plp_engine* g, int n_passes, int reset, const int32_t
     a.reset = reset;
     a.count_work = g->cfg.count_work ? 1 : 0;
     a.mode = g->mode;
+    a.verdicts = g->verd;
     a.eps = g->params.eps;
     a.theta = g->params.theta;
     a.mu = g->params.mu;
@@ -280,7 +286,7 @@ int launch_run(plp_engine* g, int n_passes, int reset) {
     hipError_t e;
     if (g->cfg.use_graph) {
         if (!g->gexec || g->g_passes != n_passes || g->g_reset != reset || g->g_grid != g->grid ||
-            g->g_cap != g->ep_cap || g->g_wpd != g->wpd) {
+            g->g_cap != g->ep_cap || g->g_wpd != g->wpd || g->g_verd != g->verd) {
             drop_graph(g);
             e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
             if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
@@ -295,6 +301,7 @@ int launch_run(plp_engine* g, int n_passes, int reset) {
             g->g_grid = g->grid;
             g->g_cap = g->ep_cap;
             g->g_wpd = g->wpd;
+            g->g_verd = g->verd;
         }
         e = hipGraphLaunch(g->gexec, c->stream);
         if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
@@ -328,12 +335,14 @@ extern "C" int plp_engine_create(plp_ctx* ctx, const plp_engine_config* cfg, plp
     if (cfg) g->cfg = *cfg;
     const int wq = g->cfg.waves_per_decoder;
     if ((wq != 0 && wq != 1 && wq != 2 && wq != 4 && wq != 8) || g->cfg.ep_cap < 0 ||
-        g->cfg.box_cap < 0 ||
-        g->cfg.blocks_per_cu < 0) {
+        g->cfg.box_cap < 0 || g->cfg.blocks_per_cu < 0 || g->cfg.verdicts < 0 ||
+        g->cfg.verdicts > 2) {
         delete g;
-        return fail(PLP_E_ARG, "config: waves_per_decoder must be 0, 1, 2, 4 or 8; caps >= 0");
+        return fail(PLP_E_ARG, "config: waves_per_decoder must be 0, 1, 2, 4 or 8; caps >= 0; "
+                               "verdicts 0, 1 or 2");
     }
     g->wpd = wq > 0 ? wq : 4;
+    g->verd = g->cfg.verdicts == 1 ? plpk::kPairCache : plpk::kPointState;
     // fine-grained: the kernels' stores reach host memory while they run
     g->h_rows.flags = g->h_rec.flags = hipHostMallocMapped | hipHostMallocCoherent;
     if ((e = hipEventCreate(&g->ev0)) != hipSuccess || (e = hipEventCreate(&g->ev1)) != hipSuccess) {
@@ -521,7 +530,7 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
     g->loaded = false;
     std::vector<InstHdr> hdr((size_t)n_inst);
     std::vector<int32_t> steps;
-    std::vector<int32_t> step_min;   // per step: min w, d, h over the later steps, 0
+    std::vector<int32_t> step_min;   // per step: min w, d, h, 0, max w, d, h, 0 over the later steps
     std::vector<int32_t> ors;
     BlockTables bt_all;               // block engine: every instance's tables
     int max_types = 1;
@@ -567,16 +576,30 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
         // Componentwise smallest orientation of every type; a point where a
         // box of the smallest dimensions of all later steps cannot go can
         // host none of them.
+        // The componentwise largest of the later steps bounds which placed
+        // boxes can still meet or support a later box at a point.
         std::vector<std::array<int32_t, 3>> tmin(in.items.size(), {kNoBox, kNoBox, kNoBox});
+        std::vector<std::array<int32_t, 3>> tmax(in.items.size(), {0, 0, 0});
         for (size_t t = 0; t < in.items.size(); t++)
             for (const auto& o : plp::orientations(in.items[t]))
-                for (int c = 0; c < 3; c++) tmin[t][(size_t)c] = std::min(tmin[t][(size_t)c], (int32_t)o[(size_t)c]);
+                for (int c = 0; c < 3; c++) {
+                    tmin[t][(size_t)c] = std::min(tmin[t][(size_t)c], (int32_t)o[(size_t)c]);
+                    tmax[t][(size_t)c] = std::max(tmax[t][(size_t)c], (int32_t)o[(size_t)c]);
+                    // 16-bit point-state entries hold sides up to 65535
+                    if (o[(size_t)c] > 65535) wide = 1;
+                }
         {
-            std::vector<int32_t> mins(4 * seq.size(), 0);
-            std::array<int32_t, 3> m = {kNoBox, kNoBox, kNoBox};
+            std::vector<int32_t> mins(8 * seq.size(), 0);
+            std::array<int32_t, 3> m = {kNoBox, kNoBox, kNoBox}, mx = {0, 0, 0};
             for (size_t k = seq.size(); k-- > 0;) {
-                for (int c = 0; c < 3; c++) mins[4 * k + (size_t)c] = m[(size_t)c];
-                for (int c = 0; c < 3; c++) m[(size_t)c] = std::min(m[(size_t)c], tmin[(size_t)seq[k]][(size_t)c]);
+                for (int c = 0; c < 3; c++) {
+                    mins[8 * k + (size_t)c] = m[(size_t)c];
+                    mins[8 * k + 4 + (size_t)c] = mx[(size_t)c];
+                }
+                for (int c = 0; c < 3; c++) {
+                    m[(size_t)c] = std::min(m[(size_t)c], tmin[(size_t)seq[k]][(size_t)c]);
+                    mx[(size_t)c] = std::max(mx[(size_t)c], tmax[(size_t)seq[k]][(size_t)c]);
+                }
             }
             step_min.insert(step_min.end(), mins.begin(), mins.end());
         }
@@ -597,7 +620,7 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
     }
     if (ors.empty()) ors.assign(plpk::kOrsRow, 0);
     if (steps.empty()) steps.push_back(0);
-    if (step_min.empty()) step_min.assign(4, kNoBox);
+    if (step_min.empty()) step_min = {kNoBox, kNoBox, kNoBox, 0, 0, 0, 0, 0};
     for (int d = 0; d < n_dec; d++)
         if (dec_inst[d] < 0 || dec_inst[d] >= n_inst)
             return fail(PLP_E_ARG, "dec_inst out of range");
@@ -885,9 +908,16 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
     float bms = -1.0f;
     plp_engine_config bcfg = saved;
     const int saved_wpd = g->wpd;
+    const int saved_verd = g->verd;
+    int best_verd = saved_verd;
+    // verdict kinds to try: the configured one, or both for the item decoder
+    std::vector<int> verds = {saved_verd};
+    if (!g->blocks && saved.verdicts == 0) verds = {plpk::kPointState, plpk::kPairCache};
+    for (int verd : verds)
     for (int wpd : {1, 2, 4, 8}) {
         if (saved.waves_per_decoder != 0 && wpd != saved.waves_per_decoder) continue;
         g->wpd = wpd;
+        g->verd = verd;
         for (int cap : caps) {
             g->cfg.ep_cap = cap;
             g->cfg.blocks_per_cu = 0;
@@ -909,6 +939,7 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
                     bcfg.ep_cap = g->ep_cap;   // escalation may have raised it
                     bcfg.box_cap = g->box_cap;
                     bcfg.waves_per_decoder = wpd;
+                    best_verd = verd;
                 }
             }
         }
@@ -919,6 +950,7 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
     g->cfg.box_cap = bcfg.box_cap;
     g->box_cap = bcfg.box_cap > 0 ? bcfg.box_cap : g->box_cap;
     g->wpd = bcfg.waves_per_decoder > 0 ? bcfg.waves_per_decoder : saved_wpd;
+    g->verd = best_verd;
     rc = shape(g, bcfg.ep_cap, bcfg.blocks_per_cu, g->n_dec);
     if (rc != PLP_OK) return rc;
     drop_graph(g);
@@ -928,6 +960,7 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
     if (best) {
         *best = g->cfg;
         best->waves_per_decoder = g->wpd;
+        best->verdicts = g->blocks ? 0 : (g->verd == plpk::kPairCache ? 1 : 2);
     }
     if (best_ms) *best_ms = bms;
     return PLP_OK;

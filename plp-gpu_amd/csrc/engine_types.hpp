@@ -74,7 +74,8 @@ static_assert(sizeof(RowRecord) == 64, "RowRecord layout");
 struct DecodeArgs {
     const InstHdr* inst;
     const int32_t* step_type;
-    const int32_t* step_min;     // 4 ints per step: smallest w, d, h over the later steps
+    const int32_t* step_min;     // 8 ints per step: smallest w, d, h over the later steps, 0,
+                                 // largest w, d, h over the later steps, 0
     const int32_t* ors_tab;
     const int32_t* dec_inst;
     const uint64_t* dec_seed;
@@ -95,6 +96,7 @@ struct DecodeArgs {
     int32_t reset;           // 1: seed RNG and clear selection state first
     int32_t count_work;      // 1: also keep the dead points, to count the reference's work
     int32_t mode;            // StateMode of the loaded batch
+    int32_t verdicts;        // VerdictKind of the kernel
     double eps, theta, mu;
     uint32_t* work_counter;  // [0] persistent-grid work queue head, [1] decoders ending in error
 };
@@ -148,6 +150,71 @@ __host__ __device__ inline LdsLayout lds_layout(int32_t ep_cap, int32_t box_cap,
     l.off_box = (region + 15) & ~15;
     l.off_mt = l.off_box + (box_cap + 8) * 32;
     l.off_steps = l.off_mt + (use_rng ? 624 * 4 : 0);   // per step: type | n_ors << 24
+    l.total = l.off_steps + step_cap * 4;
+    return l;
+}
+
+// ----------------------------------------------- per-point states ----
+//
+// The decoder's two ways to reach a step's pair verdicts (DecodeArgs.verdicts):
+//   kPairCache   pair states (support sum or blocked) of the last two steps,
+//                reused when the type repeats; other steps scan the boxes
+//   kPointState  one orientation-free state per live point, kept current as
+//                boxes are placed, from which every pair's verdict follows
+enum VerdictKind { kPairCache = 0, kPointState = 1 };
+
+// A point state: Ent v[kPtsWords] (Ent: 16 bits in kNarrow, where every
+// orientation side is at most 65535, else 32 bits), in units of 4 entries:
+//   v[4k .. 4k + 2]     staircase entry k < kPtsStair: q = (qx, qy, qz), the
+//                       offsets max(0, B.lo - p) of a placed box ahead of p
+//                       (B.hi > p on every axis) that a later box could still
+//                       meet; only the componentwise-minimal ones are kept
+//   v[3]                flags: kPtsOverflow when an entry did not fit
+//   v[16 + 4k ..]       support entry k < kPtsSup: (a, a2, b, b2) = the
+//                       x extent [a, a2) and y extent [b, b2) of a placed box
+//                       whose top is at p.z, relative to p (a, b clamped at 0)
+// Unused entries are neutral: staircase (max, max, max), support (0, 0, 0,
+// 0), so they block and support nothing.
+constexpr int kPtsStair = 4, kPtsSup = 4, kPtsWords = 32;
+constexpr uint32_t kPtsOverflow = 1u;
+
+__host__ __device__ constexpr int pts_entry_bytes(int mode) { return mode == kNarrow ? 2 : 4; }
+
+// Dynamic LDS layout of one decoder block with per-point states: two live
+// lists (keys and slot numbers, used alternately), the stack of free slots,
+// ep_cap + 8 point states, the dead list when work is counted, pair masks;
+// FSR reuses the region after the pass; then boxes, MT state and the step
+// table as in lds_layout.
+struct PtsLayout {
+    int32_t off_ep, off_ep2, off_sl, off_sl2, off_free, off_pts, off_dead, off_mask, n_mask;
+    int32_t off_load, off_key, n_key, off_tot;
+    int32_t off_box, off_mt, off_steps, total;
+};
+
+__host__ __device__ inline PtsLayout pts_layout(int32_t ep_cap, int32_t box_cap, int32_t use_rng,
+                                                int32_t count_work, int32_t mode, int32_t step_cap) {
+    PtsLayout l;
+    const int32_t slots = ep_cap + 8;
+    const int32_t list = slots * key_bytes(mode);
+    l.off_ep = 0;
+    l.off_ep2 = list;
+    l.off_sl = 2 * list;
+    l.off_sl2 = l.off_sl + slots * 2;
+    l.off_free = l.off_sl2 + slots * 2;
+    l.off_pts = (l.off_free + slots * 2 + 15) & ~15;
+    l.off_dead = l.off_pts + slots * kPtsWords * pts_entry_bytes(mode);
+    l.off_mask = (l.off_dead + (count_work ? list : 0) + 7) & ~7;
+    l.n_mask = (ep_cap * 6 + 63) / 64;
+    const int32_t pass_end = l.off_mask + l.n_mask * 8;
+    l.n_key = next_pow2(box_cap > 0 ? box_cap : 1);
+    l.off_load = 0;
+    l.off_key = box_cap * 8;
+    l.off_tot = l.off_key + l.n_key * 8;
+    const int32_t fsr_end = l.off_tot + box_cap * (mode == kBig ? 8 : 4);
+    const int32_t region = pass_end > fsr_end ? pass_end : fsr_end;
+    l.off_box = (region + 15) & ~15;
+    l.off_mt = l.off_box + (box_cap + 8) * 32;
+    l.off_steps = l.off_mt + (use_rng ? 624 * 4 : 0);
     l.total = l.off_steps + step_cap * 4;
     return l;
 }

@@ -26,6 +26,7 @@
 #include "decoder_dev.hpp"
 #include "engine_types.hpp"
 #include "mt19937.hpp"
+#include "point_state.hpp"
 #include "wave.hpp"
 
 namespace plpk {
@@ -52,6 +53,9 @@ struct StepShared {
     int quit;       // the decoder stopped (capacity error): leave the pass loop
     int nb[6];      // the box just placed: x, y, z, x2, y2, z2
     Sum cstate[48];        // pair states of its 8 corners x this type's orientations
+    uint16_t cslot[2][8];  // point states: slots for the corners of the box placed
+                           // with n_box even / odd
+    int nfree;             // point states: free slots on the stack
 };
 
 // Persistent decoder: one block of NW waves decodes one (instance, seed)
@@ -77,33 +81,77 @@ struct StepShared {
 // The bound asks for at most 128 VGPRs (four waves per SIMD, i.e. four
 // 4-wave decoders per CU).  M (StateMode, a.mode) sets the widths of the
 // extreme-point keys, support sums and pair-state cache entries.
-template <int NW, int M>
+//
+// V (VerdictKind) chooses how a step's pair verdicts are reached.
+// kPairCache: the pair states of the last two evaluated steps are kept; a
+// step of the same type as the previous one combines them with the new box,
+// and any other step scans the boxes near each 64-pair chunk.  kPointState:
+// every live point carries an orientation-free state (point_state.hpp) in a
+// slot of its own, which the helper wave brings up to date with each new box
+// while wave 0 merges the lists, and builds for the new corners; every step,
+// whatever its type, then reads its verdicts from the states.
+template <int NW, int M, int V>
 __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
     using Key = typename ModeTypes<M>::Key;
     using Sum = typename ModeTypes<M>::Sum;
+    using Ent = typename PtsEntOf<M>::T;
+    constexpr bool PS = V == kPointState;
     constexpr Sum kBlk = kBlockedOf<Sum>;
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int s_ors[18 * NW];
     __shared__ StepShared<Sum> sh;
     __shared__ DecState sst;
-    const LdsLayout lay = lds_layout(a.ep_cap, a.box_cap, a.use_rng, NW, a.count_work, M,
-                                     a.step_cap);
-    Key* const ep_a = (Key*)(lds + lay.off_ep);
-    Key* const ep_b = (Key*)(lds + lay.off_ep2);
-    Key* const dead = (Key*)(lds + lay.off_dead);
-    uint16_t* const oidx = (uint16_t*)(lds + lay.off_oidx);
-    unsigned char* const cache = lds + lay.off_cache;
-    const int cache_bytes = lay.n_cache * cache_entry_bytes(M);
-    uint64_t* mask = (uint64_t*)(lds + lay.off_mask);
-    double* load = (double*)(lds + lay.off_load);
-    uint64_t* skey = (uint64_t*)(lds + lay.off_key);
-    Sum* stot = (Sum*)(lds + lay.off_tot);
-    Box* box = (Box*)(lds + lay.off_box);
-    uint32_t* mt = (uint32_t*)(lds + lay.off_mt);
-    int32_t* s_step = (int32_t*)(lds + lay.off_steps);
     const int lane = lane_id();
     const int wid = (int)(threadIdx.x >> 6);
-    Box* cbox = (Box*)(lds + lay.off_rel) + wid * kCullWindow;   // this wave's culled window
+    Key *ep_a, *ep_b, *dead;
+    uint16_t* oidx = nullptr;
+    unsigned char* cache = nullptr;
+    int cache_bytes = 0;
+    Box* cbox = nullptr;
+    uint16_t *sl_a = nullptr, *sl_b = nullptr, *fstack = nullptr;
+    Pts<Ent>* pts = nullptr;
+    uint64_t* mask;
+    double* load;
+    uint64_t* skey;
+    Sum* stot;
+    Box* box;
+    uint32_t* mt;
+    int32_t* s_step;
+    if constexpr (PS) {
+        const PtsLayout lay = pts_layout(a.ep_cap, a.box_cap, a.use_rng, a.count_work, M,
+                                         a.step_cap);
+        ep_a = (Key*)(lds + lay.off_ep);
+        ep_b = (Key*)(lds + lay.off_ep2);
+        dead = (Key*)(lds + lay.off_dead);
+        sl_a = (uint16_t*)(lds + lay.off_sl);
+        sl_b = (uint16_t*)(lds + lay.off_sl2);
+        fstack = (uint16_t*)(lds + lay.off_free);
+        pts = (Pts<Ent>*)(lds + lay.off_pts);
+        mask = (uint64_t*)(lds + lay.off_mask);
+        load = (double*)(lds + lay.off_load);
+        skey = (uint64_t*)(lds + lay.off_key);
+        stot = (Sum*)(lds + lay.off_tot);
+        box = (Box*)(lds + lay.off_box);
+        mt = (uint32_t*)(lds + lay.off_mt);
+        s_step = (int32_t*)(lds + lay.off_steps);
+    } else {
+        const LdsLayout lay = lds_layout(a.ep_cap, a.box_cap, a.use_rng, NW, a.count_work, M,
+                                         a.step_cap);
+        ep_a = (Key*)(lds + lay.off_ep);
+        ep_b = (Key*)(lds + lay.off_ep2);
+        dead = (Key*)(lds + lay.off_dead);
+        oidx = (uint16_t*)(lds + lay.off_oidx);
+        cache = lds + lay.off_cache;
+        cache_bytes = lay.n_cache * cache_entry_bytes(M);
+        cbox = (Box*)(lds + lay.off_rel) + wid * kCullWindow;   // this wave's culled window
+        mask = (uint64_t*)(lds + lay.off_mask);
+        load = (double*)(lds + lay.off_load);
+        skey = (uint64_t*)(lds + lay.off_key);
+        stot = (Sum*)(lds + lay.off_tot);
+        box = (Box*)(lds + lay.off_box);
+        mt = (uint32_t*)(lds + lay.off_mt);
+        s_step = (int32_t*)(lds + lay.off_steps);
+    }
     const int orw = 18 * wid;   // this wave's orientations: s_ors[orw ..]
 
     for (;;) {
@@ -165,7 +213,19 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
             if (wid == 0) {
                 if (lane == 0) {
                     ep_a[0] = Key(0);   // origin; L, W, H > 0 so it is live
-                    oidx[0] = kNewPoint;
+                    if constexpr (PS) {
+                        sl_a[0] = 0;
+                        pts[0] = pts_empty<Ent>();
+                    } else {
+                        oidx[0] = kNewPoint;
+                    }
+                }
+                if constexpr (PS) {
+                    // slot 0 holds the origin, 1 .. 8 wait for the first box's
+                    // corners, 9 .. ep_cap + 7 are free
+                    for (int i = lane; i < a.ep_cap - 1; i += kWave) fstack[i] = (uint16_t)(i + 9);
+                    if (lane < 8) sh.cslot[0][lane] = (uint16_t)(lane + 1);
+                    if (lane == 0) sh.nfree = a.ep_cap - 1;
                 }
                 for (int i = lane; i < a.box_cap + 8; i += kWave) box[i] = sentinel_box();
             }
@@ -240,7 +300,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 int wm = 0, dm = 0, hm = 0;
                 if (wid == 0) {
                     if (a.count_work) live += (uint64_t)n_act * (uint64_t)no * (uint64_t)n_box;
-                    const int32_t* mn = a.step_min + 4 * (size_t)(I.step_off + s_cur);
+                    const int32_t* mn = a.step_min + 8 * (size_t)(I.step_off + s_cur);
                     wm = mn[0];
                     dm = mn[1];
                     hm = mn[2];
@@ -252,6 +312,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 }
                 wave_sync();
                 const Key* act = cur ? ep_b : ep_a;
+                const uint16_t* sl_act = cur ? sl_b : sl_a;
 
                 // ---- pair verdicts (every wave).  After a placement of
                 // the same type, a pair of a point that was in the previous
@@ -269,6 +330,32 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 const uint32_t magic = orient_recip(no);
                 const int n_chunks = (n_pairs + kWave - 1) / kWave;
                 int cnt_w = 0;
+                if constexpr (PS) {
+                    for (int c = wid; c < n_chunks; c += NW) {
+                        const int p = c * kWave + lane;
+                        const bool valid = p < n_pairs;
+                        const int pv = valid ? p : 0;
+                        const int e = div_orients(pv, no, magic);
+                        const int o = pv - __mul24(e, no);
+                        int x, y, z;
+                        key_xyz(act[e], xb, yb, x, y, z);
+                        const Pts<Ent> ps = pts[sl_act[e]];
+                        const int w = s_ors[orw + 3 * o], dd = s_ors[orw + 3 * o + 1],
+                                  h = s_ors[orw + 3 * o + 2];
+                        const bool c2 = valid & (x + w <= L) & (y + dd <= W) & (z + h <= H);
+                        Sum sup = 0;
+                        const bool c1 = pts_eval(ps, w, dd, h, sup);
+                        const bool ovf = (ps.get(kPtsFlags) & kPtsOverflow) != 0;
+                        uint64_t m = ballot(pair_verdict(c2 & c1 & !ovf, z, w, dd, sup));
+                        if (ballot(c2 & ovf) != 0) {
+                            Sum s2 = 0;
+                            const bool a2 = scan_all(x, y, z, w, dd, h, c2 & ovf, box, nb_w, s2);
+                            m |= ballot((c2 & ovf) & pair_verdict(a2, z, w, dd, s2));
+                        }
+                        mask[c] = m;
+                        cnt_w += __popcll(m);
+                    }
+                } else
                 for (int c = wid; c < n_chunks; c += NW) {
                     const int p = c * kWave + lane;
                     const bool valid = p < n_pairs;
@@ -410,17 +497,95 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 // states of the new box's corners for this type (last wave),
                 // which the next step takes when it has the same type.
                 if (wid == 0) {
+                    SlotMerge smg;
+                    SlotMerge* smp = nullptr;
+                    if constexpr (PS) {
+                        smg.src = sl_act;
+                        smg.dst = cur ? sl_a : sl_b;
+                        smg.cslot = sh.cslot[nb_w & 1][lane & 7];
+                        smp = &smg;
+                    }
                     const int need = merge_points(act, cur ? ep_a : ep_b, oidx, dead, n_act, n_dead,
                                                   box, n_box, x, y, z, X2, Y2, Z2, L, W, H, xb, yb,
                                                   a.ep_cap, a.count_work != 0, uni(wm), uni(dm),
-                                                  uni(hm));
+                                                  uni(hm), smp);
                     max_ep = max(max_ep, (uint32_t)(n_act + n_dead));
                     ep_need = max(ep_need, (uint32_t)need);
                     quit = need > a.ep_cap;   // PLP_E_EPCAP
                     if (lane == 0) sh.quit = quit;
                 }
-                if (wid == NW - 1)
+                if constexpr (PS) {
+                    // One helper wave frees the slots of the points the merge
+                    // drops (the merge's own rule: their smallest later box
+                    // meets the new box or a wall, or they equal an inserted
+                    // corner), brings the other points' states up to date with
+                    // the new box, and takes slots for the next box's corners;
+                    // the last wave builds the inserted corners' states (one
+                    // wave does both when NW <= 2).
+                    constexpr int kFreeWave = NW > 2 ? NW - 2 : NW - 1;
+                    const int32_t* mn = a.step_min + 8 * (size_t)(I.step_off + s_cur);
+                    const int wm_s = mn[0], dm_s = mn[1], hm_s = mn[2];
+                    const int par = nb_w & 1;
+                    uint32_t ins = 0;
+                    if (wid == kFreeWave || wid == NW - 1)
+                        ins = corner_live(sh.nb, box, nb_w + 1, L, W, H, wm_s, dm_s, hm_s);
+                    if (wid == kFreeWave) {
+                        const int WM = mn[4], DM = mn[5], HM = mn[6];
+                        const Box NB = box[nb_w];
+                        Key ck = 0;   // lane j < 8: corner j's key
+                        if (lane < 8)
+                            ck = xyz_key<Key>((lane & 1) ? NB.x2 : NB.x, (lane & 2) ? NB.y2 : NB.y,
+                                              (lane & 4) ? NB.z2 : NB.z, xb, yb);
+                        Key cks[8];
+#pragma unroll
+                        for (int j = 0; j < 8; j++) cks[j] = read_lane(ck, j);
+                        int nf = uni(sh.nfree);
+                        for (int i0 = 0; i0 < na_w; i0 += kWave) {
+                            const int i = i0 + lane;
+                            const bool ok = i < na_w;
+                            const Key v = act[ok ? i : i0];
+                            const int sl = sl_act[ok ? i : i0];
+                            int px, py, pz;
+                            key_xyz(v, xb, yb, px, py, pz);
+                            bool eq = false;
+#pragma unroll
+                            for (int j = 0; j < 8; j++) eq |= (((ins >> j) & 1u) != 0) & (cks[j] == v);
+                            const bool gone =
+                                (px + wm_s > L) | (py + dm_s > W) | (pz + hm_s > H) |
+                                ((px < NB.x2) & (NB.x < px + wm_s) & (py < NB.y2) &
+                                 (NB.y < py + dm_s) & (pz < NB.z2) & (NB.z < pz + hm_s));
+                            const bool drop = ok & (gone | eq);
+                            const uint64_t fm = ballot(drop);
+                            if (drop) fstack[nf + rank_below(fm)] = (uint16_t)sl;
+                            nf += __popcll(fm);
+                            if (ok & !drop && pts_touches(px, py, pz, NB, WM, DM, HM)) {
+                                Pts<Ent> ps = pts[sl];
+                                pts_absorb(ps, px, py, pz, NB, WM, DM, HM);
+                                pts[sl] = ps;
+                            }
+                        }
+                        // the inserted corners keep their slots; the next box's
+                        // corners take new ones in their place (at most ep_cap
+                        // points hold a slot unless the list outgrew the
+                        // capacity, and then the decoder stops)
+                        const int n_ins = __popc(ins);
+                        if (nf >= n_ins) {
+                            if (lane < 8) {
+                                const bool taken = (ins >> lane) & 1u;
+                                const int r = __popc(ins & ((1u << lane) - 1u));
+                                sh.cslot[par ^ 1][lane] =
+                                    taken ? fstack[nf - 1 - r] : sh.cslot[par][lane];
+                            }
+                            nf -= n_ins;
+                        }
+                        if (lane == 0) sh.nfree = nf;
+                    }
+                    if (wid == NW - 1)
+                        pts_corners(pts, sh.cslot[par], sh.nb, box, nb_w + 1, ins, mn[4], mn[5],
+                                    mn[6]);
+                } else if (wid == NW - 1) {
                     corner_states(sh.nb, sh.cstate, s_ors + orw, no, box, nb_w + 1, L, W, H, cbox);
+                }
                 if (wid == 0) advance(s_cur + 1, quit);
                 block_sync();
                 cur ^= 1;
@@ -658,28 +823,45 @@ hipError_t cached_occupancy(const void* kernel, int block_threads, int lds_bytes
     return e;
 }
 
-template <int M>
-static hipError_t launch_decode_w(const DecodeArgs& a, int waves, int grid, int lds_bytes,
-                                  hipStream_t st) {
+using DecodeFn = void (*)(DecodeArgs);
+
+template <int M, int V>
+static DecodeFn decode_fn_w(int waves) {
     switch (waves) {
-        case 1: hipLaunchKernelGGL((decode_kernel<1, M>), dim3(grid), dim3(64), (size_t)lds_bytes, st, a); break;
-        case 2: hipLaunchKernelGGL((decode_kernel<2, M>), dim3(grid), dim3(128), (size_t)lds_bytes, st, a); break;
-        case 4: hipLaunchKernelGGL((decode_kernel<4, M>), dim3(grid), dim3(256), (size_t)lds_bytes, st, a); break;
-        case 8: hipLaunchKernelGGL((decode_kernel<8, M>), dim3(grid), dim3(512), (size_t)lds_bytes, st, a); break;
-        default: return hipErrorInvalidValue;
+        case 1: return decode_kernel<1, M, V>;
+        case 2: return decode_kernel<2, M, V>;
+        case 4: return decode_kernel<4, M, V>;
+        case 8: return decode_kernel<8, M, V>;
+        default: return nullptr;
     }
-    return hipGetLastError();
+}
+
+template <int V>
+static DecodeFn decode_fn_v(int waves, int mode) {
+    switch (mode) {
+        case kNarrow: return decode_fn_w<kNarrow, V>(waves);
+        case kWide: return decode_fn_w<kWide, V>(waves);
+        case kBig: return decode_fn_w<kBig, V>(waves);
+        default: return nullptr;
+    }
+}
+
+// The decode_kernel instance for a launch shape, key width and verdict kind.
+static DecodeFn decode_fn(int waves, int mode, int verdicts) {
+    if (verdicts == kPointState) return decode_fn_v<kPointState>(waves, mode);
+    if (verdicts == kPairCache) return decode_fn_v<kPairCache>(waves, mode);
+    return nullptr;
 }
 
 hipError_t launch_decode(const DecodeArgs& a, int waves, int grid, int lds_bytes,
                          hipStream_t st) {
+    const DecodeFn f = decode_fn(waves, a.mode, a.verdicts);
+    if (!f) return hipErrorInvalidValue;
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, a.work_counter, 0u, 2);
-    switch (a.mode) {
-        case kNarrow: return launch_decode_w<kNarrow>(a, waves, grid, lds_bytes, st);
-        case kWide: return launch_decode_w<kWide>(a, waves, grid, lds_bytes, st);
-        case kBig: return launch_decode_w<kBig>(a, waves, grid, lds_bytes, st);
-        default: return hipErrorInvalidValue;
-    }
+    DecodeArgs arg = a;
+    void* args[] = {&arg};
+    return hipLaunchKernel((const void*)f, dim3(grid), dim3(64 * waves), args, (size_t)lds_bytes,
+                           st);
 }
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st) {
@@ -702,54 +884,16 @@ hipError_t launch_rng_script(const RngArgs& g, int n_seeds, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int NW, int M>
-static hipError_t occupancy_of(int lds_bytes, int* blocks_per_cu) {
-    return cached_occupancy((const void*)decode_kernel<NW, M>, 64 * NW, lds_bytes, blocks_per_cu);
+hipError_t decode_static_lds(int waves, int mode, int verdicts, int* bytes) {
+    const DecodeFn f = decode_fn(waves, mode, verdicts);
+    if (!f) return hipErrorInvalidValue;
+    return cached_static_lds((const void*)f, bytes);
 }
 
-template <int NW, int M>
-static hipError_t static_lds_of(int* bytes) {
-    return cached_static_lds((const void*)decode_kernel<NW, M>, bytes);
-}
-
-template <int M>
-static hipError_t decode_static_lds_w(int waves, int* bytes) {
-    switch (waves) {
-        case 1: return static_lds_of<1, M>(bytes);
-        case 2: return static_lds_of<2, M>(bytes);
-        case 4: return static_lds_of<4, M>(bytes);
-        case 8: return static_lds_of<8, M>(bytes);
-        default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t decode_static_lds(int waves, int mode, int* bytes) {
-    switch (mode) {
-        case kNarrow: return decode_static_lds_w<kNarrow>(waves, bytes);
-        case kWide: return decode_static_lds_w<kWide>(waves, bytes);
-        case kBig: return decode_static_lds_w<kBig>(waves, bytes);
-        default: return hipErrorInvalidValue;
-    }
-}
-
-template <int M>
-static hipError_t decode_occupancy_w(int waves, int lds_bytes, int* blocks_per_cu) {
-    switch (waves) {
-        case 1: return occupancy_of<1, M>(lds_bytes, blocks_per_cu);
-        case 2: return occupancy_of<2, M>(lds_bytes, blocks_per_cu);
-        case 4: return occupancy_of<4, M>(lds_bytes, blocks_per_cu);
-        case 8: return occupancy_of<8, M>(lds_bytes, blocks_per_cu);
-        default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t decode_occupancy(int waves, int mode, int lds_bytes, int* blocks_per_cu) {
-    switch (mode) {
-        case kNarrow: return decode_occupancy_w<kNarrow>(waves, lds_bytes, blocks_per_cu);
-        case kWide: return decode_occupancy_w<kWide>(waves, lds_bytes, blocks_per_cu);
-        case kBig: return decode_occupancy_w<kBig>(waves, lds_bytes, blocks_per_cu);
-        default: return hipErrorInvalidValue;
-    }
+hipError_t decode_occupancy(int waves, int mode, int verdicts, int lds_bytes, int* blocks_per_cu) {
+    const DecodeFn f = decode_fn(waves, mode, verdicts);
+    if (!f) return hipErrorInvalidValue;
+    return cached_occupancy((const void*)f, 64 * waves, lds_bytes, blocks_per_cu);
 }
 
 hipError_t fsr_prepare(int lds_bytes) {

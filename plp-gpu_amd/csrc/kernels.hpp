@@ -50,8 +50,9 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st);
 hipError_t launch_feas(const FeasRowArgs& r, int n_blocks, hipStream_t st);
 hipError_t launch_fsr(const FsrArgs& f, int n_packs, int lds_bytes, hipStream_t st);
 hipError_t launch_rng_script(const RngArgs& g, int n_seeds, hipStream_t st);
-hipError_t decode_occupancy(int waves, int mode, int lds_bytes, int* blocks_per_cu);
-hipError_t decode_static_lds(int waves, int mode, int* bytes);   // the kernel's own __shared__ bytes
+hipError_t decode_occupancy(int waves, int mode, int verdicts, int lds_bytes, int* blocks_per_cu);
+// the kernel's own __shared__ bytes
+hipError_t decode_static_lds(int waves, int mode, int verdicts, int* bytes);
 hipError_t block_occupancy(int waves, int mode, int lds_bytes, int* blocks_per_cu);
 hipError_t block_static_lds(int waves, int mode, int* bytes);
 hipError_t fsr_prepare(int lds_bytes);

@@ -41,6 +41,9 @@ __device__ __forceinline__ int read_lane(int v, int src) {
 __device__ __forceinline__ uint32_t read_lane(uint32_t v, int src) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, src);
 }
+__device__ __forceinline__ uint64_t read_lane(uint64_t v, int src) {
+    return ((uint64_t)read_lane((uint32_t)(v >> 32), src) << 32) | read_lane((uint32_t)v, src);
+}
 
 // Wave-wide integer sums (exact; order-independent).
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {

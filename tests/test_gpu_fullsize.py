@@ -112,8 +112,9 @@ def _check_all(plp, ctx, insts, dec_inst, rows, pl, theta, per_box_support=True)
         _check_packing(insts[dec_inst[k]], r, pl[k], theta, per_box_support)
 
 
-def _run(plp, ctx, texts, dec_inst, seeds, passes, blocks=None, count_work=False):
-    eng = plp.Engine(ctx, use_graph=not count_work, count_work=count_work)
+def _run(plp, ctx, texts, dec_inst, seeds, passes, blocks=None, count_work=False,
+         verdicts=0):
+    eng = plp.Engine(ctx, use_graph=not count_work, count_work=count_work, verdicts=verdicts)
     eng.load([plp.Instance.from_text(tx) for tx in texts], dec_inst, seeds,
              plp.params(eps=EPS, theta=THETA), blocks=blocks)
     eng.run(passes)
@@ -153,6 +154,10 @@ def _workload_test(plp, ctx, orc, name):
     from plp_amd import synth
     texts, dec_inst, seeds, passes = synth.workload(name)
     insts = [orc.parse(tx)[0] for tx in texts]
+    # the item decoder's other verdict kind (the default is point states)
+    # gives the same packings
+    prow, ppl = _run(plp, ctx, texts, dec_inst, seeds, passes, verdicts="pair_cache")
+    _compare_pinned(plp, name, prow, ppl)
     rows, pl = _run(plp, ctx, texts, dec_inst, seeds, passes)
     _check_all(plp, ctx, insts, dec_inst, rows, pl, THETA)
     _compare_pinned(plp, name, rows, pl)

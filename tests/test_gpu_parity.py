@@ -173,7 +173,12 @@ def test_fsr_eval_hand_kats(ctx):
 
 # ------------------------------------------------ device-resident engine --
 
-def test_engine_every_reference_case(plp, ctx, ref_runs):
+# the item decoder's two ways to its verdicts (plp_engine_config.verdicts)
+VERDICTS = ("pair_cache", "point_state")
+
+
+@pytest.mark.parametrize("verdicts", VERDICTS)
+def test_engine_every_reference_case(plp, ctx, ref_runs, verdicts):
     """Each reference run as its own decoder in one multi-instance batch."""
     by_params = {}
     for c in ref_runs:
@@ -182,7 +187,7 @@ def test_engine_every_reference_case(plp, ctx, ref_runs):
         by_params.setdefault(key, []).append(c)
     for key, cases in by_params.items():
         insts = [inst_from(plp, c["instance"]) for c in cases]
-        eng = plp.Engine(ctx, count_work=True)
+        eng = plp.Engine(ctx, count_work=True, verdicts=verdicts)
         eng.load(insts, list(range(len(cases))), [c["args"]["seed"] for c in cases],
                  prm(plp, cases[0]["args"]))
         eng.run(key[0])
@@ -211,14 +216,15 @@ def test_engine_passes_in_pieces_equal_one_call(plp, ctx, ref_runs):
     assert plp.to_json_dict(rows[0], pl[0]) == ref_fields(c["result"])
 
 
+@pytest.mark.parametrize("verdicts", VERDICTS)
 @pytest.mark.parametrize("count_work", [False, True])
-def test_engine_capacity_escalation(plp, ctx, ref_runs, count_work):
+def test_engine_capacity_escalation(plp, ctx, ref_runs, count_work, verdicts):
     """A tiny EP capacity overflows (the live list, and with count_work the
     dead list); the engine reruns those decoders from their seeds with a
     larger capacity and the results are unchanged."""
     cases = [c for c in ref_runs if c["case"] in ("BR1_eps03_s1", "BR15_eps0", "plain_c1")]
     for c in cases:
-        eng = plp.Engine(ctx, ep_cap=16, count_work=count_work)
+        eng = plp.Engine(ctx, ep_cap=16, count_work=count_work, verdicts=verdicts)
         eng.load([inst_from(plp, c["instance"])], [0], [c["args"]["seed"]], prm(plp, c["args"]))
         eng.run(c["args"]["passes"])
         eng.sync()
@@ -232,8 +238,9 @@ def test_engine_capacity_escalation(plp, ctx, ref_runs, count_work):
             assert rows[0].max_ep >= c["result"]["max_cand"], c["case"]
 
 
+@pytest.mark.parametrize("verdicts", VERDICTS)
 @pytest.mark.parametrize("box_cap,ep_cap", [(8, 0), (16, 64), (48, 0)])
-def test_engine_box_capacity_escalation(plp, ctx, box_cap, ep_cap):
+def test_engine_box_capacity_escalation(plp, ctx, box_cap, ep_cap, verdicts):
     """A box capacity below what decoders place stops them; the engine reruns
     them with room for every item (and, with a small extreme-point capacity
     too, more points), then settles on the most boxes any decoder placed.
@@ -245,7 +252,7 @@ def test_engine_box_capacity_escalation(plp, ctx, box_cap, ep_cap):
     with open(golden_path("fullsize_digests.json")) as f:
         pin = _json.load(f)["workloads"]["mixed_256x16"]["per_decoder"][:512]
     insts = [plp.Instance.from_text(t) for t in texts[:32]]
-    eng = plp.Engine(ctx, box_cap=box_cap, ep_cap=ep_cap, use_graph=True)
+    eng = plp.Engine(ctx, box_cap=box_cap, ep_cap=ep_cap, use_graph=True, verdicts=verdicts)
     eng.load(insts, dec_inst[:512], seeds[:512], plp.params(eps=0.01, theta=0.9))
     for _ in range(2):
         eng.reset()
@@ -276,8 +283,9 @@ def test_engine_graph_replay_matches_eager(plp, ctx, ref_runs):
         eng.close()
 
 
+@pytest.mark.parametrize("verdicts", VERDICTS)
 @pytest.mark.parametrize("wpd", [1, 2, 4])
-def test_engine_many_decoders_vs_oracle(plp, ctx, orc, wpd):
+def test_engine_many_decoders_vs_oracle(plp, ctx, orc, wpd, verdicts):
     """K = 256 decoders of the 20-type instance (seeds 1..256, eps 0.01,
     theta 0.9, 2 passes) against the oracle, decoder by decoder, with 1, 2
     and 4 wavefronts per decoder."""
@@ -286,7 +294,7 @@ def test_engine_many_decoders_vs_oracle(plp, ctx, orc, wpd):
     o_inst = orc.parse(text)[0]
     K = 256
     count = wpd == 2     # one width also checks the reference's work count
-    eng = plp.Engine(ctx, waves_per_decoder=wpd, count_work=count)
+    eng = plp.Engine(ctx, waves_per_decoder=wpd, count_work=count, verdicts=verdicts)
     eng.load([inst], [0] * K, list(range(1, K + 1)), plp.params(eps=0.01, theta=0.9))
     eng.run(2)
     eng.sync()
@@ -302,15 +310,16 @@ def test_engine_many_decoders_vs_oracle(plp, ctx, orc, wpd):
             assert rows[k].work_pair_box == 0 and rows[k].work_live_pair_box == 0, k
 
 
+@pytest.mark.parametrize("verdicts", VERDICTS)
 @pytest.mark.parametrize("wpd", [1, 4])
-def test_engine_mixed_batch_vs_oracle(plp, ctx, orc, wpd):
+def test_engine_mixed_batch_vs_oracle(plp, ctx, orc, wpd, verdicts):
     """Uneven instances (3..100 types) x 4 seeds in one persistent launch."""
     from plp_amd import synth
     texts = synth.mixed_batch(24, seed0=777_000)
     insts = [plp.Instance.from_text(t) for t in texts]
     dec_inst = [i for i in range(len(texts)) for _ in range(4)]
     seeds = [100 + 3 * i + s for i in range(len(texts)) for s in range(4)]
-    eng = plp.Engine(ctx, waves_per_decoder=wpd)
+    eng = plp.Engine(ctx, waves_per_decoder=wpd, verdicts=verdicts)
     eng.load(insts, dec_inst, seeds, plp.params(eps=0.3, theta=0.9))
     eng.run(1)
     eng.sync()
@@ -323,8 +332,9 @@ def test_engine_mixed_batch_vs_oracle(plp, ctx, orc, wpd):
                                                       "placements")}, d
 
 
+@pytest.mark.parametrize("verdicts", VERDICTS)
 @pytest.mark.parametrize("wpd", [2, 4])
-def test_engine_many_types_vs_oracle(plp, ctx, orc, wpd):
+def test_engine_many_types_vs_oracle(plp, ctx, orc, wpd, verdicts):
     """130 types of 2-3 copies (131 boxes in a packing, so FSR runs its LDS
     form, beyond 128 boxes), 32 seeds, eps 0.3, 2 passes, against the oracle."""
     from plp_amd import synth
@@ -332,7 +342,7 @@ def test_engine_many_types_vs_oracle(plp, ctx, orc, wpd):
     inst = plp.Instance.from_text(text)
     o_inst = orc.parse(text)[0]
     K = 32
-    eng = plp.Engine(ctx, waves_per_decoder=wpd)
+    eng = plp.Engine(ctx, waves_per_decoder=wpd, verdicts=verdicts)
     eng.load([inst], [0] * K, list(range(1, K + 1)), plp.params(eps=0.3, theta=0.9))
     eng.run(2)
     eng.sync()
@@ -344,8 +354,9 @@ def test_engine_many_types_vs_oracle(plp, ctx, orc, wpd):
         assert got == {x: want[x] for x in got}, k
 
 
+@pytest.mark.parametrize("verdicts", VERDICTS)
 @pytest.mark.parametrize("wpd", [1, 4])
-def test_engine_large_footprints_vs_oracle(plp, ctx, orc, wpd):
+def test_engine_large_footprints_vs_oracle(plp, ctx, orc, wpd, verdicts):
     """Footprints up to 620 x 590 cm (w * d > 131068, so the pair-state cache
     holds 32-bit support sums instead of clamped 16-bit ones) in a
     1300 x 1000 x 900 container, 48 seeds, eps 0.3, 2 passes."""
@@ -359,7 +370,7 @@ def test_engine_large_footprints_vs_oracle(plp, ctx, orc, wpd):
     o_inst = orc.parse(text)[0]
     assert max(a * b for t in o_inst[3] for a, b in ((t[0], t[2]), (t[0], t[4]), (t[2], t[4]))) > 131068
     K = 48
-    eng = plp.Engine(ctx, waves_per_decoder=wpd)
+    eng = plp.Engine(ctx, waves_per_decoder=wpd, verdicts=verdicts)
     eng.load([plp.Instance.from_text(text)], [0] * K, list(range(1, K + 1)),
              plp.params(eps=0.3, theta=0.9))
     eng.run(2)
@@ -417,15 +428,16 @@ def _big_cases():
         return {c["name"]: c for c in json.load(f)["cases"]}
 
 
+@pytest.mark.parametrize("verdicts", VERDICTS)
 @pytest.mark.parametrize("name", ["cube2000", "br1_mm", "hc40_mm", "side_max"])
-def test_engine_big_containers_match_reference_and_oracle(plp, ctx, orc, name):
+def test_engine_big_containers_match_reference_and_oracle(plp, ctx, orc, name, verdicts):
     """The reference CLI's eps 0 and eps 0.3 single runs (its g++ host
     build), and 16 seeds x 2 passes against the oracle, at 1 and 4 waves."""
     case = _big_cases()[name]
     inst = plp.Instance.from_text(case["text"])
     o_inst = orc.parse(case["text"])[0]
     for wpd in (1, 4):
-        eng = plp.Engine(ctx, waves_per_decoder=wpd)
+        eng = plp.Engine(ctx, waves_per_decoder=wpd, verdicts=verdicts)
         for key, prm, seed in (("eps0", plp.params(eps=0.0, use_theta=False), 0),
                                ("eps03_seed5", plp.params(eps=0.3, theta=0.9), 5)):
             eng.load([inst], [0], [seed], prm)
@@ -469,7 +481,8 @@ def test_engine_millimetre_instance_is_the_centimetre_one_scaled(plp, ctx):
         assert (np.asarray(pm[k][:n]) == want).all(), k
 
 
-def test_engine_huge_footprints_64bit_sums(plp, ctx, orc):
+@pytest.mark.parametrize("verdicts", VERDICTS)
+def test_engine_huge_footprints_64bit_sums(plp, ctx, orc, verdicts):
     """Footprints past 2^31 (support sums past 2^32): the oracle keeps
     Python's exact integers (plp/solver.py), as this build does; the
     reference's g++ build would overflow its int product w * d there."""
@@ -478,7 +491,7 @@ def test_engine_huge_footprints_64bit_sums(plp, ctx, orc):
     inst = plp.Instance(100000, 100000, 1000, types)
     o_inst = (100000, 100000, 1000, types)
     seeds = list(range(1, 33))
-    eng = plp.Engine(ctx)
+    eng = plp.Engine(ctx, verdicts=verdicts)
     eng.load([inst], [0] * 32, seeds, plp.params(eps=0.3, theta=0.5))
     eng.run(2)
     eng.sync()
