@@ -226,12 +226,13 @@ def timed_workload(plp_amd, synth, ctx, name: str, steps: int, warmup: int, shap
     prm = plp_amd.params(eps=EPS, theta=THETA)
     blocks = plp_amd.block_params() if synth.is_block_workload(name) else None
     if shape:
-        wpd, cap, bpc, boxes = ([int(v) for v in shape.split(",")] + [0])[:4]
+        wpd, cap, bpc, boxes, verd = ([int(v) for v in shape.split(",")] + [0, 0])[:5]
         eng = plp_amd.Engine(ctx, ep_cap=cap, blocks_per_cu=bpc, use_graph=True,
-                             waves_per_decoder=wpd, box_cap=boxes)
+                             waves_per_decoder=wpd, box_cap=boxes, verdicts=verd)
         eng.load(insts, dec_inst, seeds, prm, blocks=blocks)
+        # automatic verdicts without the autotuner: point states (2)
         tuned = {"waves_per_decoder": wpd, "ep_cap": cap, "blocks_per_cu": bpc, "box_cap": boxes,
-                 "fixed": True}
+                 "verdicts": verd if (verd or blocks is not None) else 2, "fixed": True}
     else:
         eng = plp_amd.Engine(ctx, use_graph=True)
         eng.load(insts, dec_inst, seeds, prm, blocks=blocks)
@@ -282,7 +283,7 @@ def main() -> int:
     ap.add_argument("--workload", default=HEADLINE,
                     help="headline workload (plp_amd.synth.workload names)")
     ap.add_argument("--shape", default="",
-                    help="fixed launch shape WPD,EP_CAP,BLOCKS_PER_CU[,BOX_CAP] instead of "
+                    help="fixed launch shape WPD,EP_CAP,BLOCKS_PER_CU[,BOX_CAP[,VERDICTS]] instead of "
                          "autotuning")
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="length of the Poisson serving run in extras (0 = skip)")
@@ -352,6 +353,11 @@ def main() -> int:
     if os.path.exists(pfile):
         with open(pfile) as f:
             prof = json.load(f)
+    # (only when the profile ran the item decoder with the same verdict kind)
+    if prof.get("verdicts", 0) != head["tuned"].get("verdicts", 0):
+        log(f"{os.path.basename(pfile)} was taken with verdicts {prof.get('verdicts')}, "
+            f"this run uses {head['tuned'].get('verdicts')}: counters not used")
+        prof = {}
     traffic = prof.get("hbm_bytes_per_launch")
     executed = None
     if prof.get("valu_insts_per_launch"):
