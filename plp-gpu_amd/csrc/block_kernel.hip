@@ -234,9 +234,101 @@ __global__ void __launch_bounds__(64 * NW, 3) block_kernel(BlockArgs ba) {
                                 wave_sync();
                             }
                         }
-                        for (int q = 0; q < n_types; q++) {
+                        // Blocks of each type in turn (plp_block.cpp:129-153).
+                        // While no decision makes an eps pick, every type's
+                        // blocks are its greedy chain (the first shape in
+                        // precedence order whose copy count fits the copies
+                        // left), so the chains are walked lane-parallel (lane
+                        // = type) and placed by a prefix sum of their lengths,
+                        // and decision k's random() comes from MT words mti +
+                        // 2k and mti + 2k + 1, all computed at once.  From the
+                        // first decision that makes a pick on -- and for the
+                        // whole plan when those words would cross a
+                        // regeneration or there are more than 64 types -- the
+                        // decisions run one by one, drawing as they go.
+                        int q_ser = 0, r_ser = -1;   // one by one from type q_ser, r_ser copies
+                                                     // left there (-1: all of them)
+                        bool drawn = false;          // that first decision's random() made a pick
+                        if (n_types <= kWave) {
+                            const bool okq = lane < n_types;
+                            const int t = okq ? tlist[lane] : 0;
+                            const int gt = tr[t].gt_off, size = tr[t].shp_cnt;
+                            const int r0 = okq ? rem[t] : 0;
+                            int len = 0;
+                            for (int rr = r0; ballot(rr > 0) != 0;) {
+                                if (rr > 0) {
+                                    const GreedyEntry g = lgt[gt + rr - 1];
+                                    if (g.start == size) {
+                                        rr = 0;
+                                    } else {
+                                        len++;
+                                        rr -= g.cnt0;
+                                    }
+                                }
+                            }
+                            int inc = len;
+#pragma unroll
+                            for (int dl = 1; dl < kWave; dl <<= 1) {
+                                const int up = __shfl_up(inc, dl);
+                                if (lane >= dl) inc += up;
+                            }
+                            const int off = inc - len;
+                            const int n_dec = __shfl(inc, kWave - 1);
+                            int kstar = n_dec;   // first decision with a pick
+                            bool par = true;
+                            if (use_eps) {
+                                if (mti + 2 * n_dec <= 624) {
+                                    for (int k0 = 0; k0 < n_dec; k0 += kWave) {
+                                        const int k = k0 + lane;
+                                        bool trig = false;
+                                        if (k < n_dec) {
+                                            const uint32_t wa = mt_temper(mt[mti + 2 * k]) >> 5;
+                                            const uint32_t wb = mt_temper(mt[mti + 2 * k + 1]) >> 6;
+                                            const double rv = ((double)wa * 67108864.0 + (double)wb) *
+                                                              (1.0 / 9007199254740992.0);
+                                            trig = rv >= 1.0 - a.eps;
+                                        }
+                                        const uint64_t m = ballot(trig);
+                                        if (m != 0) {
+                                            kstar = k0 + __ffsll((unsigned long long)m) - 1;
+                                            break;
+                                        }
+                                    }
+                                    mti += 2 * (kstar < n_dec ? kstar + 1 : n_dec);
+                                } else {
+                                    par = false;
+                                }
+                            }
+                            if (par) {
+                                // the greedy decisions before kstar
+                                int rr = r0, p = 0;
+                                for (;;) {
+                                    const bool go = (p < len) & (off + p < kstar);
+                                    if (ballot(go) == 0) break;
+                                    if (go) {
+                                        const GreedyEntry g = lgt[gt + rr - 1];
+                                        plan[off + p] = (uint32_t)g.s0;
+                                        p++;
+                                        rr -= g.cnt0;
+                                    }
+                                }
+                                n_plan = min(kstar, n_dec);
+                                q_ser = n_types;
+                                if (kstar < n_dec) {
+                                    // decision kstar: its type, and the copies left
+                                    // there (where that lane's walk stopped)
+                                    const bool at = okq & (off <= kstar) & (kstar < off + len);
+                                    const int qs = __ffsll((unsigned long long)ballot(at)) - 1;
+                                    q_ser = qs;
+                                    r_ser = read_lane(rr, qs);
+                                    drawn = true;
+                                }
+                            }
+                            wave_sync();
+                        }
+                        for (int q = q_ser; q < n_types; q++) {
                             const int t = uni(tlist[q]);
-                            int r = uni(rem[t]);
+                            int r = (q == q_ser && r_ser >= 0) ? r_ser : uni(rem[t]);
                             const int off = uni(tr[t].shp_off), size = uni(tr[t].shp_cnt);
                             const int gt = uni(tr[t].gt_off);
                             while (r > 0) {
@@ -247,8 +339,10 @@ __global__ void __launch_bounds__(64 * NW, 3) block_kernel(BlockArgs ba) {
                                 if (start == size) break;
                                 const int tk = min(ba.topk, size - start);
                                 int pk = 0;
-                                if (use_eps && mt_random(mt, mti) >= 1.0 - a.eps)
-                                    pk = mt_randrange(mt, mti, tk);
+                                bool pick = drawn;
+                                drawn = false;
+                                if (!pick) pick = use_eps && mt_random(mt, mti) >= 1.0 - a.eps;
+                                if (pick) pk = mt_randrange(mt, mti, tk);
                                 int s = uni(g.s0), cnt = uni(g.cnt0);
                                 if (pk != 0) {   // an eps pick among the topk largest that fit
                                     s = uni(ba.by_type[off + start + pk]);
@@ -447,31 +541,56 @@ __global__ void __launch_bounds__(64 * NW, 3) block_kernel(BlockArgs ba) {
                     sst.ep_need = need_ep;
                 }
                 if (err == 0) {
-                    // blocks -> items in block order, i (x) outer, j (y), t (z) inner
+                    // blocks -> items in block order, i (x) outer, j (y), t (z)
+                    // inner.  Lane = block: its item sides (the block's sides
+                    // are exact multiples, so the float quotient of these
+                    // integers below 2^24 is exact), its first item from a
+                    // wave prefix sum of the counts, then its items with
+                    // (i, j, t) stepped, no division.
                     int n = 0;
                     int64_t vol = 0;
-                    for (int b = 0; b < n_box; b++) {
-                        const Box B = box[b];
-                        const int ba_ = uni(B.pad) & 0xff, bb = (uni(B.pad) >> 8) & 0xff,
-                                  bk = (uni(B.pad) >> 16) & 0xff;
-                        const int iw = (uni(B.x2) - uni(B.x)) / ba_, id = (uni(B.y2) - uni(B.y)) / bb,
-                                  ih = (uni(B.z2) - uni(B.z)) / bk;
-                        const int cnt_b = ba_ * bb * bk;
-                        for (int l = lane; l < cnt_b; l += kWave) {
-                            const int i = l / (bb * bk), j = (l / bk) % bb, t = l % bk;
-                            Box it;
-                            it.x = uni(B.x) + i * iw; it.y = uni(B.y) + j * id;
-                            it.z = uni(B.z) + t * ih;
-                            it.x2 = it.x + iw; it.y2 = it.y + id; it.z2 = it.z + ih;
-                            it.type = uni(B.type);
-                            // an item below the block's top layer supports the
-                            // one above it; a top-layer item may support
-                            // something only if some block rests on the block
-                            it.pad = ((t < bk - 1) | ((uni(B.pad) & kSupports) != 0)) ? kSupports : 0;
-                            items[n + l] = it;
+                    for (int b0 = 0; b0 < n_box; b0 += kWave) {
+                        const int b = b0 + lane;
+                        const bool okb = b < n_box;
+                        const Box B = box[okb ? b : b0];
+                        const int ba_ = max(B.pad & 0xff, 1), bb = max((B.pad >> 8) & 0xff, 1),
+                                  bk = max((B.pad >> 16) & 0xff, 1);
+                        const int iw = (int)((float)(B.x2 - B.x) / (float)ba_);
+                        const int id = (int)((float)(B.y2 - B.y) / (float)bb);
+                        const int ih = (int)((float)(B.z2 - B.z) / (float)bk);
+                        const int cnt_b = okb ? ba_ * bb * bk : 0;
+                        int inc = cnt_b;
+#pragma unroll
+                        for (int dl = 1; dl < kWave; dl <<= 1) {
+                            const int up = __shfl_up(inc, dl);
+                            if (lane >= dl) inc += up;
                         }
-                        vol += (int64_t)cnt_b * iw * id * ih;
-                        n += cnt_b;
+                        const int off = n + inc - cnt_b;
+                        // an item below the block's top layer supports the one
+                        // above it; a top-layer item may support something
+                        // only if some block rests on the block
+                        const bool top_sup = (B.pad & kSupports) != 0;
+                        const int m_end = wave_max(cnt_b);
+                        int i = 0, j = 0, t = 0;
+                        for (int m = 0; m < m_end; m++) {
+                            if (m < cnt_b) {
+                                Box it;
+                                it.x = B.x + i * iw; it.y = B.y + j * id; it.z = B.z + t * ih;
+                                it.x2 = it.x + iw; it.y2 = it.y + id; it.z2 = it.z + ih;
+                                it.type = B.type;
+                                it.pad = ((t < bk - 1) | top_sup) ? kSupports : 0;
+                                items[off + m] = it;
+                            }
+                            if (++t == bk) {
+                                t = 0;
+                                if (++j == bb) {
+                                    j = 0;
+                                    ++i;
+                                }
+                            }
+                        }
+                        vol += wave_sum_i64(okb ? (int64_t)cnt_b * iw * id * ih : 0);
+                        n += __shfl(inc, kWave - 1);
                     }
                     wave_sync();
                     // D4' (plp_block.cpp:228-275): a packing that beats neither
