@@ -177,7 +177,9 @@ __global__ void __launch_bounds__(64 * NW, 3) block_kernel(BlockArgs ba) {
             wave_sync();
             if (uni(sh.quit) == 0 && a.use_rng) {
                 if (a.reset) {
-                    mt_seed_wave(mt, a.dec_seed[d]);
+                    // the state seeded once at load (mt_seed_kernel)
+                    const uint32_t* g0 = a.mt0 + (size_t)d * kMtWords;
+                    for (int i = lane; i < 624; i += kWave) mt[i] = g0[i];
                 } else {
                     for (int i = lane; i < 624; i += kWave) mt[i] = gmt[i];
                     mti = uni((int)gmt[624]);

@@ -59,7 +59,7 @@ struct plp_engine {
     int passes_done = 0;
     bool need_reset = true;
     bool states_fresh = false;
-    DevBuf d_retry, d_mt, d_state, d_counter;
+    DevBuf d_retry, d_mt, d_mt0, d_state, d_counter;
     HostBuf h_rows, h_rec;   // results, written by the kernels (fine-grained pinned memory)
     std::vector<int32_t> h_order;
     std::vector<int32_t> h_dec_inst;
@@ -243,6 +243,7 @@ plpk::DecodeArgs make_args(plp_engine* g, int n_passes, int reset, const int32_t
     a.n_work = n_work;
     a.n_passes = n_passes;
     a.mt = (uint32_t*)g->d_mt.p;
+    a.mt0 = (const uint32_t*)g->d_mt0.p;
     a.state = (DecState*)g->d_state.p;
     a.out_rows = (int32_t*)g->h_rows.dev;
     a.out_rec = (plpk::RowRecord*)g->h_rec.dev;
@@ -369,7 +370,8 @@ extern "C" void plp_engine_destroy(plp_engine* g) {
     (void)hipSetDevice(g->ctx->device);
     (void)hipStreamSynchronize(g->ctx->stream);
     drop_graph(g);
-    for (DevBuf* b : {&g->d_tab, &g->d_retry, &g->d_mt, &g->d_state, &g->d_counter}) b->release();
+    for (DevBuf* b : {&g->d_tab, &g->d_retry, &g->d_mt, &g->d_mt0, &g->d_state, &g->d_counter})
+        b->release();
     g->h_tab.release();
     g->h_rows.release();
     g->h_rec.release();
@@ -680,6 +682,8 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
         (rc = g->h_tab.reserve(tab_bytes, "pinned table staging")) ||
         (rc = g->d_retry.reserve(4 * (size_t)n_dec, "retry order")) ||
         (rc = g->d_mt.reserve(4 * plpk::kMtWords * (size_t)n_dec, "RNG states")) ||
+        (rc = g->d_mt0.reserve(4 * plpk::kMtWords * (size_t)(g->use_rng ? n_dec : 1),
+                               "seeded RNG states")) ||
         (rc = g->d_state.reserve(sizeof(DecState) * (size_t)n_dec, "decoder states")) ||
         (rc = g->h_rows.reserve(rows_bytes, "pinned result packings")) ||
         (rc = g->h_rec.reserve(sizeof(plpk::RowRecord) * (size_t)n_dec, "pinned result records")))
@@ -690,6 +694,12 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
         if (ups[t].n) std::memcpy((char*)g->h_tab.p + g->tab_off[t], ups[t].src, ups[t].n);
     e = hipMemcpyAsync(g->d_tab.p, g->h_tab.p, tab_bytes, hipMemcpyHostToDevice, g->ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "H2D engine tables");
+    // every decoder's seeded MT19937 state, once per load: a reset copies it
+    if (g->use_rng) {
+        e = plpk::launch_mt_seed((const uint64_t*)table(g, plp_engine::kDecSeed),
+                                 (uint32_t*)g->d_mt0.p, n_dec, g->ctx->stream);
+        if (e != hipSuccess) return hip_fail(e, "RNG seeding");
+    }
     int cap = g->cfg.ep_cap;
     if (cap <= 0) {
         // Automatic capacity: the largest (up to 4 points per item) that

@@ -83,6 +83,7 @@ struct DecodeArgs {
     int32_t n_work;              // queue length
     int32_t n_passes;
     uint32_t* mt;            // n_dec * kMtWords
+    const uint32_t* mt0;     // n_dec * kMtWords: each decoder's freshly seeded state
     DecState* state;         // n_dec
     int32_t* out_rows;       // pinned host: n_dec * max_items * 7, each decoder's selected
                              // packing (JSON row order), rewritten whenever the selection changes

@@ -191,7 +191,9 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
             wave_sync();
             if (uni(sh.quit) == 0 && a.use_rng) {
                 if (a.reset) {
-                    mt_seed_wave(mt, a.dec_seed[d]);
+                    // the state seeded once at load (mt_seed_kernel)
+                    const uint32_t* g0 = a.mt0 + (size_t)d * kMtWords;
+                    for (int i = lane; i < 624; i += kWave) mt[i] = g0[i];
                 } else {
                     for (int i = lane; i < 624; i += kWave) mt[i] = gmt[i];
                     mti = uni((int)gmt[624]);
@@ -742,6 +744,19 @@ __global__ void __launch_bounds__(64) fsr_kernel(FsrArgs f) {
 }
 
 // Device MT19937 probe: one wave per seed runs the op script.
+// CPython's seeding of each decoder's MT19937 (init_by_array of the seed's
+// 32-bit words: a serial chain of 1247 steps), once per engine load; every
+// reset copies the result instead of seeding again.
+__global__ void __launch_bounds__(64) mt_seed_kernel(const uint64_t* seeds, uint32_t* out, int n) {
+    __shared__ uint32_t mt[624];
+    const int d = blockIdx.x;
+    if (d >= n) return;
+    mt_seed_wave(mt, seeds[d]);
+    wave_sync();
+    for (int i = lane_id(); i < 624; i += kWave) out[(size_t)d * kMtWords + i] = mt[i];
+    if (lane_id() == 0) out[(size_t)d * kMtWords + 624] = 624u;
+}
+
 __global__ void __launch_bounds__(64) rng_script_kernel(RngArgs g) {
     __shared__ uint32_t mt[624];
     const int sd = blockIdx.x;
@@ -876,6 +891,12 @@ hipError_t launch_feas(const FeasRowArgs& r, int n_blocks, hipStream_t st) {
 
 hipError_t launch_fsr(const FsrArgs& f, int n_packs, int lds_bytes, hipStream_t st) {
     hipLaunchKernelGGL(fsr_kernel, dim3(n_packs), dim3(64), (size_t)lds_bytes, st, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_mt_seed(const uint64_t* seeds, uint32_t* out, int n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mt_seed_kernel, dim3(n), dim3(64), 0, st, seeds, out, n);
     return hipGetLastError();
 }
 

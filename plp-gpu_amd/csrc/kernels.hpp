@@ -50,6 +50,8 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st);
 hipError_t launch_feas(const FeasRowArgs& r, int n_blocks, hipStream_t st);
 hipError_t launch_fsr(const FsrArgs& f, int n_packs, int lds_bytes, hipStream_t st);
 hipError_t launch_rng_script(const RngArgs& g, int n_seeds, hipStream_t st);
+// Every decoder's seeded MT19937 state (kMtWords words each) into out.
+hipError_t launch_mt_seed(const uint64_t* seeds, uint32_t* out, int n, hipStream_t st);
 hipError_t decode_occupancy(int waves, int mode, int verdicts, int lds_bytes, int* blocks_per_cu);
 // the kernel's own __shared__ bytes
 hipError_t decode_static_lds(int waves, int mode, int verdicts, int* bytes);
