@@ -543,7 +543,8 @@ __device__ __forceinline__ uint32_t corner_present(const Key* src, int n, const 
 // its state instead of oidx: a kept point keeps its slot and an inserted
 // corner j takes the slot lane j holds in sm->cslot.  (The slots of the
 // points that leave are freed by the wave that updates the states, which
-// applies the same rule.)
+// applies the same rule.)  The inserted corners (bit j: corner j) go to
+// *ins_out when given.
 struct SlotMerge {
     const uint16_t* src;   // slot of each src entry
     uint16_t* dst;         // slot of each dst entry
@@ -554,7 +555,8 @@ template <class Key>
 static __device__ int merge_points(const Key* src, Key* dst, uint16_t* oidx, Key* dead,
                             int& n_act, int& n_dead, Box* box, int n_box, int x, int y, int z,
                             int X2, int Y2, int Z2, int L, int W, int H, int xb, int yb, int cap,
-                            bool count_work, int wm, int dm, int hm, SlotMerge* sm = nullptr) {
+                            bool count_work, int wm, int dm, int hm, SlotMerge* sm = nullptr,
+                            uint32_t* ins_out = nullptr) {
     const int lane = lane_id();
     // Corner j: bit 0 -> x side, bit 1 -> y, bit 2 -> z; with the key's field
     // order (z, then y, then x) the keys increase with j.  Lane j < 8 holds
@@ -576,6 +578,7 @@ static __device__ int merge_points(const Key* src, Key* dst, uint16_t* oidx, Key
     // computed against every box, instead of its old ones plus the new box).
     cdead |= corner_kill(box, n_box, x, y, z, X2, Y2, Z2, wm, dm, hm);
     const uint32_t ins = ~cdead & 0xffu;
+    if (ins_out) *ins_out = ins;
 
     // Dead list: drop the strictly inside points in place (a point moves to
     // an index no larger than its own, below the next 64-point chunk).

@@ -183,11 +183,13 @@ __host__ __device__ constexpr int pts_entry_bytes(int mode) { return mode == kNa
 
 // Dynamic LDS layout of one decoder block with per-point states: two live
 // lists (keys and slot numbers, used alternately), the stack of free slots,
-// ep_cap + 8 point states, the dead list when work is counted, pair masks;
-// FSR reuses the region after the pass; then boxes, MT state and the step
-// table as in lds_layout.
+// ep_cap + 8 point states, per slot the verdict bits of the point's pairs
+// for the next step when it repeats the type (bit o: orientation o), the dead
+// list when work is counted, pair masks; FSR reuses the region after the
+// pass; then boxes, MT state and the step table as in lds_layout.
 struct PtsLayout {
-    int32_t off_ep, off_ep2, off_sl, off_sl2, off_free, off_pts, off_dead, off_mask, n_mask;
+    int32_t off_ep, off_ep2, off_sl, off_sl2, off_free, off_pts, off_bits, off_dead, off_mask,
+        n_mask;
     int32_t off_load, off_key, n_key, off_tot;
     int32_t off_box, off_mt, off_steps, total;
 };
@@ -203,7 +205,8 @@ __host__ __device__ inline PtsLayout pts_layout(int32_t ep_cap, int32_t box_cap,
     l.off_sl2 = l.off_sl + slots * 2;
     l.off_free = l.off_sl2 + slots * 2;
     l.off_pts = (l.off_free + slots * 2 + 15) & ~15;
-    l.off_dead = l.off_pts + slots * kPtsWords * pts_entry_bytes(mode);
+    l.off_bits = l.off_pts + slots * kPtsWords * pts_entry_bytes(mode);
+    l.off_dead = (l.off_bits + slots + 15) & ~15;
     l.off_mask = (l.off_dead + (count_work ? list : 0) + 7) & ~7;
     l.n_mask = (ep_cap * 6 + 63) / 64;
     const int32_t pass_end = l.off_mask + l.n_mask * 8;

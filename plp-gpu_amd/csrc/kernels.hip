@@ -88,8 +88,12 @@ struct StepShared {
 // and any other step scans the boxes near each 64-pair chunk.  kPointState:
 // every live point carries an orientation-free state (point_state.hpp) in a
 // slot of its own, which the helper wave brings up to date with each new box
-// while wave 0 merges the lists, and builds for the new corners; every step,
-// whatever its type, then reads its verdicts from the states.
+// while wave 0 merges the lists, and builds for the new corners; a step of a
+// new type reads its verdicts from the states, and a step that repeats the
+// type of the step before it (which placed a box) gathers the verdict bits
+// each slot was given during that merge: the previous bits for the points
+// the new box does not touch, fresh ones for the touched points and the
+// corners.
 template <int NW, int M, int V>
 __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
     using Key = typename ModeTypes<M>::Key;
@@ -110,6 +114,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
     Box* cbox = nullptr;
     uint16_t *sl_a = nullptr, *sl_b = nullptr, *fstack = nullptr;
     Pts<Ent>* pts = nullptr;
+    uint8_t* vbits = nullptr;
     uint64_t* mask;
     double* load;
     uint64_t* skey;
@@ -127,6 +132,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
         sl_b = (uint16_t*)(lds + lay.off_sl2);
         fstack = (uint16_t*)(lds + lay.off_free);
         pts = (Pts<Ent>*)(lds + lay.off_pts);
+        vbits = lds + lay.off_bits;
         mask = (uint64_t*)(lds + lay.off_mask);
         load = (double*)(lds + lay.off_load);
         skey = (uint64_t*)(lds + lay.off_key);
@@ -333,29 +339,38 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 const int n_chunks = (n_pairs + kWave - 1) / kWave;
                 int cnt_w = 0;
                 if constexpr (PS) {
-                    for (int c = wid; c < n_chunks; c += NW) {
-                        const int p = c * kWave + lane;
-                        const bool valid = p < n_pairs;
-                        const int pv = valid ? p : 0;
-                        const int e = div_orients(pv, no, magic);
-                        const int o = pv - __mul24(e, no);
-                        int x, y, z;
-                        key_xyz(act[e], xb, yb, x, y, z);
-                        const Pts<Ent> ps = pts[sl_act[e]];
-                        const int w = s_ors[orw + 3 * o], dd = s_ors[orw + 3 * o + 1],
-                                  h = s_ors[orw + 3 * o + 2];
-                        const bool c2 = valid & (x + w <= L) & (y + dd <= W) & (z + h <= H);
-                        Sum sup = 0;
-                        const bool c1 = pts_eval(ps, w, dd, h, sup);
-                        const bool ovf = (ps.get(kPtsFlags) & kPtsOverflow) != 0;
-                        uint64_t m = ballot(pair_verdict(c2 & c1 & !ovf, z, w, dd, sup));
-                        if (ballot(c2 & ovf) != 0) {
-                            Sum s2 = 0;
-                            const bool a2 = scan_all(x, y, z, w, dd, h, c2 & ovf, box, nb_w, s2);
-                            m |= ballot((c2 & ovf) & pair_verdict(a2, z, w, dd, s2));
+                    if (inc) {
+                        // The previous step had this type and placed a box:
+                        // every point's verdict bits for this step were left
+                        // in vbits[slot] while the lists were merged.
+                        for (int c = wid; c < n_chunks; c += NW) {
+                            const int p = c * kWave + lane;
+                            const bool valid = p < n_pairs;
+                            const int pv = valid ? p : 0;
+                            const int e = div_orients(pv, no, magic);
+                            const int o = pv - __mul24(e, no);
+                            const uint32_t bits = vbits[sl_act[e]];
+                            const uint64_t m = ballot(valid & (((bits >> o) & 1u) != 0));
+                            mask[c] = m;
+                            cnt_w += __popcll(m);
                         }
-                        mask[c] = m;
-                        cnt_w += __popcll(m);
+                    } else {
+                        for (int c = wid; c < n_chunks; c += NW) {
+                            const int p = c * kWave + lane;
+                            const bool valid = p < n_pairs;
+                            const int pv = valid ? p : 0;
+                            const int e = div_orients(pv, no, magic);
+                            const int o = pv - __mul24(e, no);
+                            int x, y, z;
+                            key_xyz(act[e], xb, yb, x, y, z);
+                            const Pts<Ent> ps = pts[sl_act[e]];
+                            const int w = s_ors[orw + 3 * o], dd = s_ors[orw + 3 * o + 1],
+                                      h = s_ors[orw + 3 * o + 2];
+                            const uint64_t m = pts_verdicts<Sum>(ps, valid, x, y, z, w, dd, h, L, W,
+                                                                 H, box, nb_w);
+                            mask[c] = m;
+                            cnt_w += __popcll(m);
+                        }
                     }
                 } else
                 for (int c = wid; c < n_chunks; c += NW) {
@@ -495,6 +510,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 }
                 if (NW > 1) block_sync();   // every wave sees the new box
                 bool quit = false;
+                uint32_t ins_m = 0;   // wave 0: the corners its merge inserted
                 // ---- extreme-point merge (wave 0) and, meanwhile, the pair
                 // states of the new box's corners for this type (last wave),
                 // which the next step takes when it has the same type.
@@ -510,7 +526,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     const int need = merge_points(act, cur ? ep_a : ep_b, oidx, dead, n_act, n_dead,
                                                   box, n_box, x, y, z, X2, Y2, Z2, L, W, H, xb, yb,
                                                   a.ep_cap, a.count_work != 0, uni(wm), uni(dm),
-                                                  uni(hm), smp);
+                                                  uni(hm), smp, &ins_m);
                     max_ep = max(max_ep, (uint32_t)(n_act + n_dead));
                     ep_need = max(ep_need, (uint32_t)need);
                     quit = need > a.ep_cap;   // PLP_E_EPCAP
@@ -521,16 +537,48 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     // drops (the merge's own rule: their smallest later box
                     // meets the new box or a wall, or they equal an inserted
                     // corner), brings the other points' states up to date with
-                    // the new box, and takes slots for the next box's corners;
-                    // the last wave builds the inserted corners' states (one
-                    // wave does both when NW <= 2).
+                    // the new box, and takes slots for the next box's corners.
+                    // The inserted corners' states are built in two halves:
+                    // the corners on the box's bottom face (j < 4) by wave 0
+                    // after its merge (by wave 1 when NW > 2), those on its
+                    // top face by the last wave (which is the helper wave too
+                    // when NW <= 2).
                     constexpr int kFreeWave = NW > 2 ? NW - 2 : NW - 1;
+                    constexpr int kLoWave = NW > 2 ? 1 : 0;
                     const int32_t* mn = a.step_min + 8 * (size_t)(I.step_off + s_cur);
                     const int wm_s = mn[0], dm_s = mn[1], hm_s = mn[2];
                     const int par = nb_w & 1;
-                    uint32_t ins = 0;
-                    if (wid == kFreeWave || wid == NW - 1)
+                    // The next step repeats this type (after a placement the
+                    // memo skips nothing): the new box leaves the verdicts of
+                    // every point it does not touch as they were, so the
+                    // next step's bits are this step's for those points and
+                    // are evaluated afresh only for the touched points and
+                    // the inserted corners, into vbits[slot].
+                    const bool same_next = s_cur + 1 < I.n_steps && uni(s_step[s_cur + 1]) == sv;
+                    uint32_t ins = ins_m;
+                    if (wid == kFreeWave || wid == NW - 1 || (NW > 2 && wid == kLoWave))
                         ins = corner_live(sh.nb, box, nb_w + 1, L, W, H, wm_s, dm_s, hm_s);
+                    // the states of the inserted corners in sub, and their bits
+                    // for the next step when it repeats the type (lane 8j + o:
+                    // orientation o at corner j)
+                    auto build_corners = [&](uint32_t sub) {
+                        pts_corners(pts, sh.cslot[par], sh.nb, box, nb_w + 1, sub, mn[4], mn[5],
+                                    mn[6]);
+                        if (same_next && sub != 0) {
+                            const int j = lane >> 3, o = lane & 7;
+                            const bool on = (((sub >> j) & 1u) != 0) & (o < no);
+                            const int oo = on ? o : 0;
+                            const int csl = sh.cslot[par][j];
+                            const Pts<Ent> ps = pts[csl];
+                            const uint64_t m = pts_verdicts<Sum>(
+                                ps, on, (j & 1) ? sh.nb[3] : sh.nb[0], (j & 2) ? sh.nb[4] : sh.nb[1],
+                                (j & 4) ? sh.nb[5] : sh.nb[2], s_ors[orw + 3 * oo],
+                                s_ors[orw + 3 * oo + 1], s_ors[orw + 3 * oo + 2], L, W, H, box,
+                                nb_w + 1);
+                            if (on & (o == 0)) vbits[csl] = (uint8_t)(m >> (lane & 56));
+                        }
+                    };
+                    if (wid == kLoWave) build_corners(ins & 0x0fu);
                     if (wid == kFreeWave) {
                         const int WM = mn[4], DM = mn[5], HM = mn[6];
                         const Box NB = box[nb_w];
@@ -542,6 +590,12 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
 #pragma unroll
                         for (int j = 0; j < 8; j++) cks[j] = read_lane(ck, j);
                         int nf = uni(sh.nfree);
+                        // The touched points' list indices go to the far end of
+                        // the free-slot array, downwards: the old list's points
+                        // and this box's corner slots hold all but nf of the
+                        // ep_cap + 8 slots, so the stack and these never meet.
+                        int nt = 0;
+                        uint16_t* const touched = fstack + a.ep_cap + 7;
                         for (int i0 = 0; i0 < na_w; i0 += kWave) {
                             const int i = i0 + lane;
                             const bool ok = i < na_w;
@@ -560,10 +614,18 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                             const uint64_t fm = ballot(drop);
                             if (drop) fstack[nf + rank_below(fm)] = (uint16_t)sl;
                             nf += __popcll(fm);
-                            if (ok & !drop && pts_touches(px, py, pz, NB, WM, DM, HM)) {
+                            const bool kept = ok & !drop;
+                            const bool touch = kept && pts_touches(px, py, pz, NB, WM, DM, HM);
+                            if (touch) {
                                 Pts<Ent> ps = pts[sl];
                                 pts_absorb(ps, px, py, pz, NB, WM, DM, HM);
                                 pts[sl] = ps;
+                            }
+                            if (same_next) {
+                                if (kept & !touch) vbits[sl] = (uint8_t)mask_bits(mask, i * no, no);
+                                const uint64_t tm = ballot(touch);
+                                if (touch) touched[-(nt + rank_below(tm))] = (uint16_t)i;
+                                nt += __popcll(tm);
                             }
                         }
                         // the inserted corners keep their slots; the next box's
@@ -581,10 +643,27 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                             nf -= n_ins;
                         }
                         if (lane == 0) sh.nfree = nf;
+                        if (nt != 0) {
+                            // the touched points' bits, eight points at a time:
+                            // lane 8k + o evaluates orientation o of the k-th
+                            wave_sync();   // their states and indices are in LDS
+                            for (int k0 = 0; k0 < nt; k0 += 8) {
+                                const int k = k0 + (lane >> 3), o = lane & 7;
+                                const bool on = (k < nt) & (o < no);
+                                const int ti = touched[-(k < nt ? k : 0)];
+                                const int tsl = sl_act[ti];
+                                int tx, ty, tz;
+                                key_xyz(act[ti], xb, yb, tx, ty, tz);
+                                const Pts<Ent> ps = pts[tsl];
+                                const int oo = on ? o : 0;
+                                const uint64_t m = pts_verdicts<Sum>(
+                                    ps, on, tx, ty, tz, s_ors[orw + 3 * oo], s_ors[orw + 3 * oo + 1],
+                                    s_ors[orw + 3 * oo + 2], L, W, H, box, nb_w + 1);
+                                if (on & (o == 0)) vbits[tsl] = (uint8_t)(m >> (lane & 56));
+                            }
+                        }
                     }
-                    if (wid == NW - 1)
-                        pts_corners(pts, sh.cslot[par], sh.nb, box, nb_w + 1, ins, mn[4], mn[5],
-                                    mn[6]);
+                    if (wid == NW - 1) build_corners(ins & 0xf0u);
                 } else if (wid == NW - 1) {
                     corner_states(sh.nb, sh.cstate, s_ors + orw, no, box, nb_w + 1, L, W, H, cbox);
                 }

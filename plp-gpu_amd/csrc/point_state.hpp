@@ -208,6 +208,37 @@ __device__ __forceinline__ bool scan_all(int x, int y, int z, int w, int d, int 
     return (am >> lane_id()) & 1ull;
 }
 
+// The verdicts of this lane's pair -- a w x d x h box at the point (x, y, z)
+// whose state is ps -- for the lanes in act: C2, C1 and the support from the
+// state (or, for a point whose state overflowed, from a scan of every placed
+// box), C3 on top (pair_verdict).  Returns the wave's ballot of feasible
+// pairs.
+template <class Sum, class Ent>
+__device__ __forceinline__ uint64_t pts_verdicts(const Pts<Ent>& ps, bool act, int x, int y, int z,
+                                                 int w, int d, int h, int L, int W, int H,
+                                                 const Box* __restrict__ box, int n_box) {
+    const bool c2 = act & (x + w <= L) & (y + d <= W) & (z + h <= H);
+    Sum sup = 0;
+    const bool c1 = pts_eval(ps, w, d, h, sup);
+    const bool ovf = (ps.get(kPtsFlags) & kPtsOverflow) != 0;
+    uint64_t m = ballot(pair_verdict(c2 & c1 & !ovf, z, w, d, sup));
+    if (ballot(c2 & ovf) != 0) {
+        Sum s2 = 0;
+        const bool a2 = scan_all(x, y, z, w, d, h, c2 & ovf, box, n_box, s2);
+        m |= ballot((c2 & ovf) & pair_verdict(a2, z, w, d, s2));
+    }
+    return m;
+}
+
+// Bits b0 .. b0 + n - 1 (n <= 8) of the verdict masks, as the low bits of
+// the result: the verdicts of one point's pairs.
+__device__ __forceinline__ uint32_t mask_bits(const uint64_t* mask, int b0, int n) {
+    const int wi = b0 >> 6, sh = b0 & 63;
+    uint64_t v = mask[wi] >> sh;
+    if (sh + n > 64) v |= mask[wi + 1] << (64 - sh);
+    return (uint32_t)v & ((1u << n) - 1u);
+}
+
 // Per-axis corner masks (corner j: bit 0 of j -> x side, bit 1 -> y, bit 2
 // -> z; corners on the low side of an axis are 0x55, 0x33 or 0x0f, on the
 // high side 0xaa, 0xcc or 0xf0): the corners c whose axis value c_a has
