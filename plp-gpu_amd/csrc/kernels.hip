@@ -437,11 +437,12 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 block_sync();
                 // wave 0 reads its next random() words and the first mask
                 // with the counts: one LDS round trip for the three
+                // (lane l of wave 0 holds mask word l of the first 64 chunks)
                 MtPeek pk{0u, 0u};
-                uint64_t m0 = 0;
+                uint64_t mw = 0;
                 if (wid == 0) {
                     if (a.use_rng) pk = mt_peek2(mt, mti);
-                    m0 = mask[0];
+                    if (lane < n_chunks) mw = mask[lane];
                 }
                 const int cnt = uni(sum_counts<NW>(sh.cnt[par]));
                 if (cnt == 0) {
@@ -475,18 +476,28 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     if (a.use_rng) {
                         if (mt_random(mt, mti, pk) >= 1.0 - a.eps) idx = mt_randrange(mt, mti, cnt);
                     }
-                    // The idx-th feasible pair.
-                    int c = 0;
-                    uint64_t m = uni(m0);
-                    for (;;) {
-                        const int pc = __popcll(m);
-                        if (idx < pc) break;
-                        idx -= pc;
-                        c++;
-                        m = uni(mask[c]);
+                    // The idx-th feasible pair: for idx 0 (every step that
+                    // draws no eps pick) the first set bit of the first
+                    // non-zero word the lanes hold, else a walk over the words.
+                    const uint64_t nz = ballot(mw != 0);
+                    int c = 0, bit;
+                    uint64_t m;
+                    if (idx == 0 && nz != 0) {
+                        c = __ffsll((unsigned long long)nz) - 1;
+                        m = read_lane(mw, c);
+                        bit = __ffsll((unsigned long long)m) - 1;
+                    } else {
+                        m = read_lane(mw, 0);
+                        for (;;) {
+                            const int pc = __popcll(m);
+                            if (idx < pc) break;
+                            idx -= pc;
+                            c++;
+                            m = c < kWave ? read_lane(mw, c) : uni(mask[c]);
+                        }
+                        const bool hit = ((m >> lane) & 1ull) && rank_below(m) == idx;
+                        bit = __ffsll((unsigned long long)ballot(hit)) - 1;
                     }
-                    const bool hit = ((m >> lane) & 1ull) && rank_below(m) == idx;
-                    const int bit = __ffsll((unsigned long long)ballot(hit)) - 1;
                     const int p = c * kWave + bit;
                     const int e = div_orients(p, no, magic);
                     const int o = p - __mul24(e, no);
