@@ -160,6 +160,18 @@ typedef struct {
                                     point, kept current as boxes are placed; 0 = automatic
                                     (2, and plp_engine_autotune tries both).  Results do not
                                     depend on it. */
+    int32_t share_prefix;        /* item decoder: 2 = every run first decodes each instance's
+                                    trajectory (feas[0] at every step) once, and each decoder
+                                    replays its own RNG along it and resumes where its first
+                                    eps pick with an index other than 0 leaves it (or takes
+                                    its packing when none does); 1 = every decoder decodes
+                                    from the origin; 0 = automatic: 2 when the RNG is unused
+                                    or eps * (largest item count) <= 8, and the decoders are
+                                    at least three rounds of the persistent grid (with
+                                    fewer, a launch lasts as long as its decoders that
+                                    diverge early, which the recording only delays), else 1;
+                                    plp_engine_autotune tries both.  Never with count_work.
+                                    Results do not depend on it. */
 } plp_engine_config;
 
 typedef struct {

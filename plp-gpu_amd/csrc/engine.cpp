@@ -50,7 +50,7 @@ struct plp_engine {
     // Read-only tables of a load, in one device buffer uploaded from one
     // pinned staging buffer (one copy per load).
     enum Tab { kInst, kStep, kStepMin, kShape, kTinfo, kByType, kVorder, kGtab, kOrs, kDecInst,
-               kDecSeed, kOrder, kNumTabs };
+               kDecSeed, kOrder, kTrajOrder, kNumTabs };
     DevBuf d_tab;
     HostBuf h_tab;
     size_t tab_off[kNumTabs] = {};
@@ -70,7 +70,21 @@ struct plp_engine {
     float last_ms = 0.0f;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int g_passes = -1, g_reset = -1, g_grid = -1, g_cap = -1, g_wpd = -1, g_verd = -1;
+    int g_passes = -1, g_reset = -1, g_grid = -1, g_cap = -1, g_wpd = -1, g_verd = -1,
+        g_share = -1;
+    const void* g_tbuf = nullptr;
+    // Shared prefixes (plpk::TrajHdr): the mode (plp_engine_config
+    // share_prefix; the autotuner sets 1 or 2), whether the automatic mode
+    // expects decoders to follow their trajectories for long (no RNG, or eps
+    // small against the item count), whether a block can hold a trajectory,
+    // the recording launch's shape (waves, list capacity, LDS bytes, grid),
+    // and one device buffer holding per instance the header, the placing
+    // steps, the boxes and the kept lists.
+    int share_mode = 0;
+    bool share_eps = false, share_fit = true;
+    int t_wpd = 4, t_cap = 0, t_lds = 0, t_grid = 0;
+    size_t t_key_cap = 0;   // keys per instance
+    DevBuf d_traj;
 };
 
 // The kernels write plp_row_result records as plpk::RowRecord.
@@ -162,7 +176,7 @@ void drop_graph(plp_engine* g) {
     if (g->graph) (void)hipGraphDestroy(g->graph);
     g->gexec = nullptr;
     g->graph = nullptr;
-    g->g_passes = g->g_reset = g->g_grid = g->g_cap = g->g_wpd = g->g_verd = -1;
+    g->g_passes = g->g_reset = g->g_grid = g->g_cap = g->g_wpd = g->g_verd = g->g_share = -1;
 }
 
 // Dynamic LDS bytes of one decoder block for a capacity, for the loaded
@@ -261,7 +275,80 @@ plpk::DecodeArgs make_args(plp_engine* g, int n_passes, int reset, const int32_t
     a.theta = g->params.theta;
     a.mu = g->params.mu;
     a.work_counter = (uint32_t*)g->d_counter.p;
+    a.traj = 0;
+    a.traj_key_cap = 0;
+    a.traj_list_cap = 0;
+    a.pad_traj = 0;
+    a.traj_hdr = nullptr;
+    a.traj_step = nullptr;
+    a.traj_keys = nullptr;
+    a.traj_box = nullptr;
     return a;
+}
+
+// Shared prefixes: the recording launch's shape for the current decoder
+// capacity -- the decoders' waves and verdict kind with at least four
+// waves, room for every item, and a list capacity of at least the
+// decoders' (twice it, up to what one block's LDS holds) -- and the buffer
+// of the recorded trajectories.  Clears g->share_fit when no block can hold
+// a trajectory at the decoders' capacity.
+int traj_prepare(plp_engine* g) {
+    g->t_wpd = std::max(4, g->wpd);
+    const int wide_cap = round_up(std::max(2 * g->ep_cap, 256), 32);
+    auto lds_of = [&](int cap) {
+        return g->verd == plpk::kPointState
+                   ? plpk::pts_layout(cap, g->full_box, 0, 0, g->mode, g->step_cap).total
+                   : plpk::lds_layout(cap, g->full_box, 0, g->t_wpd, 0, g->mode, g->step_cap)
+                         .total;
+    };
+    int st = 0;
+    if (plpk::decode_static_lds(g->t_wpd, g->mode, g->verd, &st) != hipSuccess) {
+        (void)hipGetLastError();
+        st = 4096;
+    }
+    int cap = wide_cap;
+    while (cap > g->ep_cap && lds_of(cap) + st > g->ctx->max_lds) cap -= 32;
+    if (cap < g->ep_cap || lds_of(cap) + st > g->ctx->max_lds) {
+        g->share_fit = false;
+        return PLP_OK;
+    }
+    int occ = 0;
+    hipError_t e = plpk::decode_occupancy(g->t_wpd, g->mode, g->verd, lds_of(cap), &occ);
+    if (e != hipSuccess) return hip_fail(e, "trajectory occupancy");
+    if (occ < 1) {
+        g->share_fit = false;
+        return PLP_OK;
+    }
+    g->t_cap = cap;
+    g->t_lds = lds_of(cap);
+    g->t_grid = std::max(1, std::min(g->n_inst, occ * g->ctx->n_cu));
+    g->t_key_cap = (size_t)g->step_cap * (size_t)(g->ep_cap + 8);
+    const size_t kb = g->mode == plpk::kBig ? 8 : 4;
+    const size_t per = sizeof(plpk::TrajHdr) + (size_t)g->step_cap * sizeof(plpk::TrajStep) +
+                       (size_t)g->step_cap * 32 + g->t_key_cap * kb;
+    return g->d_traj.reserve(per * (size_t)g->n_inst, "recorded trajectories");
+}
+
+// Whether this run records the trajectories first (before traj_prepare).
+bool share_wanted(const plp_engine* g) {
+    if (g->blocks || g->cfg.count_work || g->share_mode == 1) return false;
+    if (g->share_mode == 2) return true;
+    return g->share_eps && g->n_dec >= 3 * g->grid;
+}
+
+// The trajectory fields of a launch's arguments (traj: 1 record, 2 resume).
+void traj_args(plp_engine* g, plpk::DecodeArgs& a, int traj) {
+    const size_t n = (size_t)g->n_inst;
+    char* p = (char*)g->d_traj.p;
+    a.traj = traj;
+    a.traj_key_cap = (int32_t)g->t_key_cap;
+    a.traj_list_cap = g->ep_cap;
+    a.traj_hdr = (plpk::TrajHdr*)p;
+    a.traj_step = (plpk::TrajStep*)(p + n * sizeof(plpk::TrajHdr));
+    a.traj_box = (int32_t*)(p + n * (sizeof(plpk::TrajHdr) +
+                                     (size_t)g->step_cap * sizeof(plpk::TrajStep)));
+    a.traj_keys = p + n * (sizeof(plpk::TrajHdr) + (size_t)g->step_cap * sizeof(plpk::TrajStep) +
+                           (size_t)g->step_cap * 32);
 }
 
 // One launch of the loaded decoder kind over `grid` blocks.
@@ -285,13 +372,43 @@ int launch_run(plp_engine* g, int n_passes, int reset) {
     plpk::DecodeArgs a = make_args(g, n_passes, reset,
                                    (const int32_t*)table(g, plp_engine::kOrder), g->n_dec);
     hipError_t e;
+    int rc;
+    // Shared prefixes: one recording launch over the instances, then the
+    // decoders, which resume from the recorded trajectories.  Automatic mode:
+    // when the decoders are expected to follow their trajectories for long and
+    // the queue holds at least three rounds of the persistent grid (with
+    // fewer, the launch lasts as long as its slowest decoders, whose early
+    // divergence sharing does not shorten, and the recording launch only adds
+    // to it).
+    plpk::DecodeArgs t{};
+    const bool share = share_wanted(g);
+    if (share) {
+        if ((rc = traj_prepare(g)) != PLP_OK) return rc;
+    }
+    const bool rec = share && g->share_fit;
+    if (rec) {
+        t = make_args(g, 1, 1, (const int32_t*)table(g, plp_engine::kTrajOrder), g->n_inst);
+        t.ep_cap = g->t_cap;
+        t.box_cap = g->full_box;
+        t.use_rng = 0;
+        traj_args(g, t, 1);
+        traj_args(g, a, 2);
+    }
+    auto launch_all = [&]() -> hipError_t {
+        if (rec) {
+            const hipError_t te = plpk::launch_decode(t, g->t_wpd, g->t_grid, g->t_lds, c->stream);
+            if (te != hipSuccess) return te;
+        }
+        return launch(g, a, g->grid, c->stream);
+    };
     if (g->cfg.use_graph) {
         if (!g->gexec || g->g_passes != n_passes || g->g_reset != reset || g->g_grid != g->grid ||
-            g->g_cap != g->ep_cap || g->g_wpd != g->wpd || g->g_verd != g->verd) {
+            g->g_cap != g->ep_cap || g->g_wpd != g->wpd || g->g_verd != g->verd ||
+            g->g_share != (rec ? 1 : 0) || g->g_tbuf != g->d_traj.p) {
             drop_graph(g);
             e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
             if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-            hipError_t le = launch(g, a, g->grid, c->stream);
+            hipError_t le = launch_all();
             e = hipStreamEndCapture(c->stream, &g->graph);
             if (le != hipSuccess) return hip_fail(le, "decode capture");
             if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
@@ -303,12 +420,14 @@ int launch_run(plp_engine* g, int n_passes, int reset) {
             g->g_cap = g->ep_cap;
             g->g_wpd = g->wpd;
             g->g_verd = g->verd;
+            g->g_share = rec ? 1 : 0;
+            g->g_tbuf = g->d_traj.p;
         }
         e = hipGraphLaunch(g->gexec, c->stream);
         if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
         return PLP_OK;
     }
-    e = launch(g, a, g->grid, c->stream);
+    e = launch_all();
     if (e != hipSuccess) return hip_fail(e, "decoder launch");
     return PLP_OK;
 }
@@ -337,10 +456,10 @@ extern "C" int plp_engine_create(plp_ctx* ctx, const plp_engine_config* cfg, plp
     const int wq = g->cfg.waves_per_decoder;
     if ((wq != 0 && wq != 1 && wq != 2 && wq != 4 && wq != 8) || g->cfg.ep_cap < 0 ||
         g->cfg.box_cap < 0 || g->cfg.blocks_per_cu < 0 || g->cfg.verdicts < 0 ||
-        g->cfg.verdicts > 2) {
+        g->cfg.verdicts > 2 || g->cfg.share_prefix < 0 || g->cfg.share_prefix > 2) {
         delete g;
         return fail(PLP_E_ARG, "config: waves_per_decoder must be 0, 1, 2, 4 or 8; caps >= 0; "
-                               "verdicts 0, 1 or 2");
+                               "verdicts and share_prefix 0, 1 or 2");
     }
     g->wpd = wq > 0 ? wq : 4;
     g->verd = g->cfg.verdicts == 1 ? plpk::kPairCache : plpk::kPointState;
@@ -370,7 +489,8 @@ extern "C" void plp_engine_destroy(plp_engine* g) {
     (void)hipSetDevice(g->ctx->device);
     (void)hipStreamSynchronize(g->ctx->stream);
     drop_graph(g);
-    for (DevBuf* b : {&g->d_tab, &g->d_retry, &g->d_mt, &g->d_mt0, &g->d_state, &g->d_counter})
+    for (DevBuf* b : {&g->d_tab, &g->d_retry, &g->d_mt, &g->d_mt0, &g->d_state, &g->d_counter,
+                      &g->d_traj})
         b->release();
     g->h_tab.release();
     g->h_rows.release();
@@ -654,6 +774,17 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
     std::stable_sort(g->h_order.begin(), g->h_order.end(), [&](int a, int b) {
         return cost[(size_t)dec_inst[a]] > cost[(size_t)dec_inst[b]];
     });
+    // Shared prefixes: the instances' trajectories are recorded in the same
+    // order; the automatic mode expects a decoder to follow its trajectory
+    // for a good part of a pass when the RNG is unused or 1/eps placing steps
+    // (before its first eps pick) are at least an eighth of the item count
+    std::vector<int32_t> traj_order((size_t)n_inst);
+    for (int i = 0; i < n_inst; i++) traj_order[(size_t)i] = i;
+    std::stable_sort(traj_order.begin(), traj_order.end(),
+                     [&](int a, int b) { return cost[(size_t)a] > cost[(size_t)b]; });
+    g->share_mode = g->cfg.share_prefix;
+    g->share_eps = g->use_rng == 0 || params->eps * (double)max_items <= 8.0;
+    g->share_fit = true;
     const size_t rows_bytes = (size_t)n_dec * (size_t)max_items * 7 * sizeof(int32_t);
     struct Up {
         const void* src;
@@ -671,6 +802,7 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
         {dec_inst, 4 * (size_t)n_dec},
         {dec_seed, 8 * (size_t)n_dec},
         {g->h_order.data(), 4 * (size_t)n_dec},
+        {traj_order.data(), 4 * (size_t)n_inst},
     };
     size_t tab_bytes = 0;
     for (int t = 0; t < plp_engine::kNumTabs; t++) {
@@ -964,6 +1096,25 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
     rc = shape(g, bcfg.ep_cap, bcfg.blocks_per_cu, g->n_dec);
     if (rc != PLP_OK) return rc;
     drop_graph(g);
+    if (!g->blocks && !g->cfg.count_work && saved.share_prefix == 0) {
+        // shared prefixes on and off at the chosen shape, the faster of three
+        // runs each
+        float t[2] = {-1.0f, -1.0f};
+        for (int k = 0; k < 2; k++) {
+            g->share_mode = k == 0 ? 1 : 2;
+            drop_graph(g);
+            for (int r = 0; r < 3; r++) {
+                g->need_reset = true;
+                rc = plp_engine_run(g, n_passes);
+                if (rc == PLP_OK) rc = plp_engine_sync(g);
+                if (rc != PLP_OK) return rc;
+                if (t[k] < 0.0f || g->last_ms < t[k]) t[k] = g->last_ms;
+            }
+        }
+        g->share_mode = t[1] < t[0] ? 2 : 1;
+        bms = std::min(t[0], t[1]);
+        drop_graph(g);
+    }
     g->need_reset = true;
     g->passes_done = 0;
     g->states_fresh = false;
@@ -971,6 +1122,7 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
         *best = g->cfg;
         best->waves_per_decoder = g->wpd;
         best->verdicts = g->blocks ? 0 : (g->verd == plpk::kPairCache ? 1 : 2);
+        best->share_prefix = g->blocks ? 0 : g->share_mode;
     }
     if (best_ms) *best_ms = bms;
     return PLP_OK;

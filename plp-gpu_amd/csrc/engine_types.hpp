@@ -71,6 +71,37 @@ struct RowRecord {
 static_assert(sizeof(RowRecord) == 64, "RowRecord layout");
 
 
+// Shared prefixes.  Every decoder of an instance makes the same decisions as
+// the instance's trajectory -- its pass with feas[0] picked at every step --
+// until its first eps pick with an index other than 0: the verdicts depend
+// only on the boxes placed, and along the way each placing step draws one
+// random() whatever the seed.  A run first decodes each instance's
+// trajectory once and records, per placing step j (box j is placed there):
+// the step, the feasible-pair count, the live list it started from and the
+// largest list held so far; then every decoder replays its own RNG along the
+// trajectory and resumes from the list and boxes of the step where it
+// diverges, or takes the trajectory's packing when it never does.
+struct TrajHdr {
+    int32_t valid;    // 1: recorded without a capacity stop
+    int32_t n_box;    // boxes placed (= placing steps)
+    int32_t need;     // largest live list held during the pass
+    int32_t pad;
+    int64_t volume;
+    double score;     // stability_score of the packing (theta-independent)
+};
+static_assert(sizeof(TrajHdr) == 32, "TrajHdr layout");
+
+struct TrajStep {
+    int32_t s;        // index in the item sequence
+    int32_t cnt;      // feasible pairs
+    int32_t n_act;    // live points the step starts from (its list: key_off ..)
+    int32_t need;     // largest live list held before this step's merge
+    int32_t key_off;  // first key of the list in the instance's key area (-1: not kept,
+                      // the list exceeds the decoders' capacity)
+    int32_t pad[3];
+};
+static_assert(sizeof(TrajStep) == 32, "TrajStep layout");
+
 struct DecodeArgs {
     const InstHdr* inst;
     const int32_t* step_type;
@@ -100,6 +131,18 @@ struct DecodeArgs {
     int32_t verdicts;        // VerdictKind of the kernel
     double eps, theta, mu;
     uint32_t* work_counter;  // [0] persistent-grid work queue head, [1] decoders ending in error
+    // Shared prefixes: 0 off; 1 the queue items are instances whose
+    // trajectories are recorded; 2 decoders resume from the recorded ones.
+    int32_t traj;
+    int32_t traj_key_cap;    // keys per instance in traj_keys (placing steps x (list cap + 8))
+    int32_t traj_list_cap;   // record: lists longer than this are not kept (the decoders'
+                             // capacity; a decoder reaching such a list stops anyway)
+    int32_t pad_traj;
+    TrajHdr* traj_hdr;       // per instance
+    TrajStep* traj_step;     // per instance: step_cap entries, by placing step
+    void* traj_keys;         // per instance: traj_key_cap keys (Key of the mode)
+    int32_t* traj_box;       // per instance: step_cap boxes of 8 ints (x, y, z, x2, y2, z2,
+                             // type, index of the first box resting on it or 2^30)
 };
 
 // Dynamic LDS layout of one decoder block.  The pass-time region (two

@@ -56,7 +56,168 @@ struct StepShared {
     uint16_t cslot[2][8];  // point states: slots for the corners of the box placed
                            // with n_box even / odd
     int nfree;             // point states: free slots on the stack
+    int rs_n, rs_box, rs_s;   // a pass resumed from a trajectory: its list length, boxes
+                              // and step (rs_n 0: the pass starts from the origin)
 };
+
+// ------------------------------------------------------ shared prefixes --
+
+// The largest w, d and h over the steps from s on (step s's type and every
+// later step's): the reach of the point states that serve step s.
+__device__ __forceinline__ void largest_from(const DecodeArgs& a, const InstHdr& I, int s, int& WM,
+                                             int& DM, int& HM) {
+    const int32_t* mn = a.step_min + 8 * (size_t)(I.step_off + s);
+    WM = mn[4];
+    DM = mn[5];
+    HM = mn[6];
+    const int t = a.step_type[I.step_off + s];
+    const int32_t* orow = a.ors_tab + (size_t)(I.ors_off + t) * kOrsRow;
+    const int no = orow[0];
+    for (int o = 0; o < no; o++) {
+        WM = max(WM, orow[1 + 3 * o]);
+        DM = max(DM, orow[2 + 3 * o]);
+        HM = max(HM, orow[3 + 3 * o]);
+    }
+}
+
+// Wave 0 at the start of a pass of a decoder whose instance has a recorded
+// trajectory (engine_types.hpp, TrajHdr).  Replays the decoder's RNG along
+// the trajectory's placing steps: step j draws random() -- from MT words
+// mti + 2j and mti + 2j + 1 while no regeneration falls between, so up to 64
+// steps are tested at once, lane k taking step j + k -- and, when it is at
+// least 1 - eps, randrange(cnt_j); the first index other than 0 is where the
+// decoder leaves the trajectory.  Then either
+//   * it never does: the pass is the trajectory's (traj_pass), or
+//   * it does at placing step j: step j's list goes to ep_a, boxes 0 .. j-1
+//     to box[] (with their kSupports marks as of step j), n_act / n_box are
+//     set, the step's pick is forced to the drawn index, and the step is
+//     returned (loaded set);
+// unless a capacity of this decoder would have stopped the pass on the way
+// (its list or its boxes outgrow the LDS), which sets quit (and box_over) as
+// the decode itself would.  Returns the step to resume at, or -1 when the
+// pass does not decode.  An instance without a valid trajectory decodes from
+// the origin (step 0, nothing loaded).
+template <class Key>
+__device__ int resume_from_trajectory(const DecodeArgs& a, int inst_i, uint32_t* mt, int& mti,
+                                      Box* box, Key* ep_a, TrajHdr& th, bool& traj_pass,
+                                      int& forced, bool& loaded, int& n_act, int& n_box,
+                                      uint32_t& ep_need, bool& box_over, int& quit) {
+    const int lane = lane_id();
+    wave_sync();   // after the pass's list and box initialisation
+    loaded = false;
+    const TrajHdr h = a.traj_hdr[inst_i];
+    th.valid = uni(h.valid);
+    th.n_box = uni(h.n_box);
+    th.need = uni(h.need);
+    th.volume = uni(h.volume);
+    th.score = __longlong_as_double((long long)uni((uint64_t)__double_as_longlong(h.score)));
+    if (!th.valid) return 0;
+    const TrajStep* ts = a.traj_step + (size_t)inst_i * a.step_cap;
+    const int nd = th.n_box;
+    int j = 0, idx = 0;
+    bool div = false;
+    if (a.use_rng) {
+        while (j < nd) {
+            const int room = (624 - mti) >> 1;   // random() calls before a regeneration
+            if (room == 0) {
+                if (mt_random(mt, mti) >= 1.0 - a.eps) {
+                    idx = mt_randrange(mt, mti, uni(ts[j].cnt));
+                    if (idx != 0) {
+                        div = true;
+                        break;
+                    }
+                }
+                j++;
+                continue;
+            }
+            const int w = min(min(room, kWave), nd - j);
+            bool hit = false;
+            if (lane < w) {
+                const uint32_t hi = mt_temper(mt[mti + 2 * lane]) >> 5;
+                const uint32_t lo = mt_temper(mt[mti + 2 * lane + 1]) >> 6;
+                hit = ((double)hi * 67108864.0 + (double)lo) * (1.0 / 9007199254740992.0) >=
+                      1.0 - a.eps;
+            }
+            const uint64_t hm = ballot(hit);
+            if (hm == 0) {
+                mti += 2 * w;
+                j += w;
+                continue;
+            }
+            const int k = __ffsll((unsigned long long)hm) - 1;
+            mti += 2 * (k + 1);
+            j += k;
+            idx = mt_randrange(mt, mti, uni(ts[j].cnt));
+            if (idx != 0) {
+                div = true;
+                break;
+            }
+            j++;
+        }
+    }
+    if (!div) {
+        ep_need = max(ep_need, (uint32_t)th.need);
+        n_box = 0;
+        if (th.need > a.ep_cap) {
+            quit = 1;
+        } else if (nd > a.box_cap) {
+            box_over = true;
+            quit = 1;
+        } else {
+            traj_pass = true;
+        }
+        return -1;
+    }
+    const int need_j = uni(ts[j].need), na = uni(ts[j].n_act), koff = uni(ts[j].key_off);
+    const int s_j = uni(ts[j].s);
+    ep_need = max(ep_need, (uint32_t)need_j);
+    if (need_j > a.ep_cap || koff < 0) {
+        quit = 1;
+        return -1;
+    }
+    if (j >= a.box_cap) {
+        box_over = true;
+        quit = 1;
+        return -1;
+    }
+    const Key* tk = (const Key*)a.traj_keys + (size_t)inst_i * a.traj_key_cap + koff;
+    for (int i = lane; i < na; i += kWave) ep_a[i] = tk[i];
+    const int32_t* tb = a.traj_box + (size_t)inst_i * a.step_cap * 8;
+    for (int i = lane; i < j; i += kWave) {
+        const int4 p = *(const int4*)(tb + 8 * i);
+        const int4 q = *(const int4*)(tb + 8 * i + 4);
+        Box B;
+        B.x = p.x; B.y = p.y; B.z = p.z; B.x2 = p.w; B.y2 = q.x; B.z2 = q.y;
+        B.type = q.z;
+        B.pad = q.w < j ? kSupports : 0;   // a box of the first j rests on it
+        box[i] = B;
+    }
+    n_act = na;
+    n_box = j;
+    forced = idx;
+    loaded = true;
+    wave_sync();
+    return s_j;
+}
+
+// The rows (JSON layout) of n boxes of a recorded trajectory.
+__device__ __forceinline__ void write_rows_traj(int32_t* dst, const int32_t* tb, int n) {
+    for (int t = lane_id(); t < 7 * n; t += kWave) {
+        const int i = t / 7, c = t - (t / 7) * 7;
+        const int32_t* b = tb + 8 * i;
+        int v;
+        switch (c) {
+            case 0: v = b[6]; break;
+            case 1: v = b[0]; break;
+            case 2: v = b[1]; break;
+            case 3: v = b[2]; break;
+            case 4: v = b[3] - b[0]; break;
+            case 5: v = b[4] - b[1]; break;
+            default: v = b[5] - b[2]; break;
+        }
+        dst[t] = v;
+    }
+}
 
 // Persistent decoder: one block of NW waves decodes one (instance, seed)
 // for n_passes passes (plp_common.hpp:404-456 per pass, FSR and the D4'
@@ -167,7 +328,9 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
         block_sync();
         if (item >= a.n_work) break;
         const int d = uni(a.work_order[item]);
-        const InstHdr I = a.inst[uni(a.dec_inst[d])];
+        // recording trajectories, the queue items are instances
+        const int inst_i = a.traj == 1 ? d : uni(a.dec_inst[d]);
+        const InstHdr I = a.inst[inst_i];
         const int L = I.L, W = I.W, H = I.H, xb = I.xbits, yb = I.ybits;
         const int32_t* steps = a.step_type + I.step_off;
         // Step table in LDS: type | n_ors << 24 (types < 2^24 at load time).
@@ -295,7 +458,62 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 }
                 if (more) prev_ti = ti;
             };
-            if (wid == 0) advance(0, false);
+            // ---- shared prefix (decoders of a recorded trajectory)
+            int forced = -1;          // wave 0: the pick of the step the pass resumes at
+            bool traj_pass = false;   // wave 0: the pass is the trajectory's, whole
+            int koff = 0;             // wave 0, recording: keys kept so far
+            if (a.traj == 2) {
+                if (wid == 0) {
+                    int quit0 = 0;
+                    bool loaded = false;
+                    TrajHdr th;
+                    const int rs_s = resume_from_trajectory<Key>(
+                        a, inst_i, mt, mti, box, ep_a, th, traj_pass, forced, loaded, n_act,
+                        n_box, ep_need, box_over, quit0);
+                    if (lane == 0) {
+                        sh.rs_n = loaded ? n_act : 0;
+                        sh.rs_box = n_box;
+                        sh.rs_s = rs_s;
+                        sh.quit = quit0;
+                    }
+                }
+                block_sync();
+                if constexpr (PS) {
+                    // the resumed list's states, built from its boxes: slot
+                    // i for entry i, this box's corners next, the rest free
+                    const int rn = uni(sh.rs_n);
+                    if (rn > 0) {
+                        const int rb = uni(sh.rs_box), rss = uni(sh.rs_s);
+                        int WM, DM, HM;
+                        largest_from(a, I, rss, WM, DM, HM);
+                        for (int i = threadIdx.x; i < rn; i += 64 * NW) {
+                            int px, py, pz;
+                            key_xyz(ep_a[i], xb, yb, px, py, pz);
+                            Pts<Ent> ps = pts_empty<Ent>();
+                            for (int b = 0; b < rb; b++) {
+                                const Box B = box_xyz(box[b]);
+                                if (pts_touches(px, py, pz, B, WM, DM, HM))
+                                    pts_absorb(ps, px, py, pz, B, WM, DM, HM);
+                            }
+                            pts[i] = ps;
+                            sl_a[i] = (uint16_t)i;
+                        }
+                        if (wid == 0) {
+                            for (int i = lane; i < a.ep_cap - rn; i += kWave)
+                                fstack[i] = (uint16_t)(rn + 8 + i);
+                            if (lane < 8) sh.cslot[rb & 1][lane] = (uint16_t)(rn + lane);
+                            if (lane == 0) sh.nfree = a.ep_cap - rn;
+                        }
+                    }
+                }
+                if (wid == 0) {
+                    const int rss = uni(sh.rs_s);
+                    if (rss >= 0) advance(rss, false);
+                    else advance(I.n_steps, true);   // the trajectory's pass, or a capacity stop
+                }
+            } else if (wid == 0) {
+                advance(0, false);
+            }
             block_sync();
             for (;;) {
                 const StepDesc sd = sh.step;
@@ -473,7 +691,13 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     // feas[0], or feas[randrange(|feas|)] when random() >=
                     // 1 - eps (plp_common.hpp:427-435).
                     int idx = 0;
-                    if (a.use_rng) {
+                    if (forced >= 0) {
+                        // drawn while replaying the trajectory (this step's count
+                        // equals the trajectory's there; the bound only keeps the
+                        // walk below within the masks)
+                        idx = min(forced, cnt - 1);
+                        forced = -1;
+                    } else if (a.use_rng) {
                         if (mt_random(mt, mti, pk) >= 1.0 - a.eps) idx = mt_randrange(mt, mti, cnt);
                     }
                     // The idx-th feasible pair: for idx 0 (every step that
@@ -508,6 +732,26 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     X2 = x + uni(s_ors[orw + 3 * o]);
                     Y2 = y + uni(s_ors[orw + 3 * o + 1]);
                     Z2 = z + uni(s_ors[orw + 3 * o + 2]);
+                    if (a.traj == 1) {
+                        // recording: placing step n_box -- its step, pair count,
+                        // list and the largest list held before it (a list the
+                        // decoders' capacity cannot hold is not kept)
+                        TrajStep* ts = a.traj_step + (size_t)inst_i * a.step_cap + n_box;
+                        const bool keep =
+                            na_w <= a.traj_list_cap && koff + na_w <= a.traj_key_cap;
+                        if (lane == 0) {
+                            ts->s = s_cur;
+                            ts->cnt = cnt;
+                            ts->n_act = na_w;
+                            ts->need = (int32_t)ep_need;
+                            ts->key_off = keep ? koff : -1;
+                        }
+                        if (keep) {
+                            Key* tk = (Key*)a.traj_keys + (size_t)inst_i * a.traj_key_cap + koff;
+                            for (int i = lane; i < na_w; i += kWave) tk[i] = act[i];
+                            koff += na_w;
+                        }
+                    }
                     if (lane == 0) {
                         Box nbx;
                         nbx.x = x; nbx.y = y; nbx.z = z; nbx.x2 = X2; nbx.y2 = Y2; nbx.z2 = Z2;
@@ -684,15 +928,68 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
             }
 
             // ---- pass end (wave 0): stability score and D4' selection
-            if (wid == 0) {
+            if (wid == 0 && a.traj == 1) {
+                // recording: the trajectory's packing, its score, and per box
+                // the first box that rests on it (from which a decoder resuming
+                // at placing step j knows the boxes marked kSupports then)
                 const bool overflow = uni(sh.quit) != 0;
-                const bool scored = !overflow && uni(sst.err) == 0;
                 const int n = n_box;
                 int64_t vpart = 0;
                 for (int i = lane; i < n; i += kWave)
                     vpart += (int64_t)(box[i].x2 - box[i].x) * (box[i].y2 - box[i].y) *
                              (box[i].z2 - box[i].z);
                 const int64_t vol = wave_sum_i64(vpart);
+                const double score =
+                    (!overflow && n > 0) ? fsr_score(box, n, a.mu, load, skey, stot, nullptr) : 0.0;
+                constexpr int32_t kNone = 1 << 30;
+                int32_t* tb = a.traj_box + (size_t)inst_i * a.step_cap * 8;
+                for (int i0 = 0; i0 < n; i0 += kWave) {
+                    const int i = i0 + lane;
+                    const bool ok = i < n;
+                    const Box Bi = box[ok ? i : 0];
+                    int fr = kNone;
+                    for (int k = i0 + 1; k < n; k++) {
+                        const Box Bk = box_xyz(box[k]);
+                        if (ok & (k > i) & (fr == kNone) & (Bi.z2 == Bk.z) & (Bi.x < Bk.x2) &
+                            (Bk.x < Bi.x2) & (Bi.y < Bk.y2) & (Bk.y < Bi.y2))
+                            fr = k;
+                        if ((k & 15) == 0 && ballot(ok & (fr == kNone)) == 0) break;
+                    }
+                    if (ok) {
+                        *(int4*)(tb + 8 * i) = make_int4(Bi.x, Bi.y, Bi.z, Bi.x2);
+                        *(int4*)(tb + 8 * i + 4) = make_int4(Bi.y2, Bi.z2, Bi.type, fr);
+                    }
+                }
+                if (lane == 0) {
+                    TrajHdr h;
+                    h.valid = overflow ? 0 : 1;
+                    h.n_box = n;
+                    h.need = (int32_t)ep_need;
+                    h.pad = 0;
+                    h.volume = vol;
+                    h.score = score;
+                    a.traj_hdr[inst_i] = h;
+                }
+            } else if (wid == 0) {
+                const bool overflow = uni(sh.quit) != 0;
+                const bool scored = !overflow && uni(sst.err) == 0;
+                // a pass that followed the trajectory to its end has its packing
+                int n = n_box;
+                int64_t vol;
+                double traj_score = 0.0;
+                if (traj_pass) {
+                    const TrajHdr h = a.traj_hdr[inst_i];
+                    n = uni(h.n_box);
+                    vol = uni(h.volume);
+                    traj_score = __longlong_as_double(
+                        (long long)uni((uint64_t)__double_as_longlong(h.score)));
+                } else {
+                    int64_t vpart = 0;
+                    for (int i = lane; i < n; i += kWave)
+                        vpart += (int64_t)(box[i].x2 - box[i].x) * (box[i].y2 - box[i].y) *
+                                 (box[i].z2 - box[i].z);
+                    vol = wave_sum_i64(vpart);
+                }
                 // D4' (plp_common.hpp:493-511), kept incrementally: the best of
                 // all packings and the best accepted one, strict > on (volume,
                 // n_boxes) so the earliest winner stays.  A packing that beats
@@ -706,11 +1003,12 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 // count of boxes that can reach FSR >= 1 keeps its score below
                 // theta, it cannot, and its score is never needed.
                 bool need = scored && (better_all || better_acc);
-                if (need && !better_all && a.use_theta && n > 0 &&
+                if (need && !better_all && a.use_theta && n > 0 && !traj_pass &&
                     (double)fsr_count_bound(box, n, a.mu) / (double)n < a.theta)
                     need = false;
                 double score = 0.0;
-                if (need) score = fsr_score(box, n, a.mu, load, skey, stot, nullptr);
+                if (need)
+                    score = traj_pass ? traj_score : fsr_score(box, n, a.mu, load, skey, stot, nullptr);
                 DecState st = sst;
                 if (overflow) st.err = 4;   // PLP_E_EPCAP
                 st.work_pair_box += work;
@@ -730,7 +1028,11 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     // The selection only ever moves to the packing just decoded
                     // (now the best accepted, or the best of all while nothing is
                     // accepted), so the host copy is rewritten from LDS then.
-                    if (accept || (st.n_acc < 0 && better_all)) write_rows(out_sel, box, n);
+                    if (accept || (st.n_acc < 0 && better_all)) {
+                        if (traj_pass)
+                            write_rows_traj(out_sel, a.traj_box + (size_t)inst_i * a.step_cap * 8, n);
+                        else write_rows(out_sel, box, n);
+                    }
                     st.passes++;
                 }
                 wave_sync();
@@ -739,17 +1041,17 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
             block_sync();
             if (uni(sh.quit) != 0) break;
         }
-        if (wid == 0 && !skip_decoder) {
+        if (wid == 0 && !skip_decoder && a.traj != 1) {
             if (a.use_rng) {
                 for (int i = lane; i < 624; i += kWave) gmt[i] = mt[i];
                 if (lane == 0) gmt[624] = (uint32_t)mti;
             }
             if (lane == 0) a.state[d] = sst;
         }
-        if (wid == 0 && lane == 0) a.out_rec[d] = row_record(sst, a.dec_seed[d]);
+        if (wid == 0 && lane == 0 && a.traj != 1) a.out_rec[d] = row_record(sst, a.dec_seed[d]);
         // work_counter[1] counts the decoders that end with an error, so the
         // host reads one word, not every state, when none did
-        if (wid == 0 && lane == 0 && sst.err != 0) atomicAdd(a.work_counter + 1, 1u);
+        if (wid == 0 && lane == 0 && sst.err != 0 && a.traj != 1) atomicAdd(a.work_counter + 1, 1u);
         block_sync();
     }
 }
