@@ -23,7 +23,8 @@ Printed: one JSON line with the contract fields plus
   cpu_baseline  the reference's g++ build (oracle/_ref/ref_driver) on the
                 same 4096 decoders over the host cores of this job
   extras        the 20-type instance x 1024 decoders x 4 passes, the 100-type
-                instance (2048 decoders) in 587x233x220 and in 880x349x330,
+                instance (2048 decoders) in 587x233x220 and in 880x349x330, the
+                256 instances x 64 seeds (16384 decoders, shared prefixes),
                 the plain one-step-per-launch GPU form, the block-building
                 engine (GPU and reference CPU), JSON emission, Poisson serving
 usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu] [--no-extras]
@@ -427,7 +428,7 @@ def main() -> int:
             extras["cpu_single_core"] = single
     eng.close()
     if not a.no_extras:
-        for name in ("twenty_1024x4", "hundred_2048", "hundred_fit_2048"):
+        for name in ("twenty_1024x4", "hundred_2048", "hundred_fit_2048", "mixed_256x64"):
             if name == a.workload:
                 continue
             r = timed_workload(plp_amd, synth, ctx, name, 5 if name != "hundred_fit_2048" else 3,

@@ -1097,13 +1097,13 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
     if (rc != PLP_OK) return rc;
     drop_graph(g);
     if (!g->blocks && !g->cfg.count_work && saved.share_prefix == 0) {
-        // shared prefixes on and off at the chosen shape, the faster of three
-        // runs each
+        // shared prefixes on and off at the chosen shape, the fastest of five
+        // runs each; on only when it wins by more than the run-to-run noise
         float t[2] = {-1.0f, -1.0f};
         for (int k = 0; k < 2; k++) {
             g->share_mode = k == 0 ? 1 : 2;
             drop_graph(g);
-            for (int r = 0; r < 3; r++) {
+            for (int r = 0; r < 5; r++) {
                 g->need_reset = true;
                 rc = plp_engine_run(g, n_passes);
                 if (rc == PLP_OK) rc = plp_engine_sync(g);
@@ -1111,8 +1111,8 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
                 if (t[k] < 0.0f || g->last_ms < t[k]) t[k] = g->last_ms;
             }
         }
-        g->share_mode = t[1] < t[0] ? 2 : 1;
-        bms = std::min(t[0], t[1]);
+        g->share_mode = t[1] < 0.97f * t[0] ? 2 : 1;
+        bms = g->share_mode == 2 ? t[1] : t[0];
         drop_graph(g);
     }
     g->need_reset = true;

@@ -99,6 +99,8 @@ def workload(name: str):
     """(texts, dec_inst, seeds, passes) of a named full-size workload:
       twenty_1024x4     the 20-type instance, 1024 decoders seeds 1..1024, 4 passes
       mixed_256x16      256 instances of 3..100 types x 16 seeds (4096 decoders), 1 pass
+      mixed_256x64      the same 256 instances x 64 seeds (16384 decoders), 1 pass: a queue
+                        of eight rounds of the persistent grid
       hundred_2048      the 100-type instance (587x233x220), 2048 decoders, 1 pass
       hundred_fit_2048  the 100-type instance in 880x349x330, 2048 decoders, 1 pass
       blocks_twenty_1024x4  twenty_1024x4 decoded by the block-building engine
@@ -108,9 +110,10 @@ def workload(name: str):
         return workload(name[len("blocks_"):])
     if name == "twenty_1024x4":
         return [multi_instance()], [0] * 1024, list(range(1, 1025)), 4
-    if name == "mixed_256x16":
-        return (mixed_batch(256), [i for i in range(256) for _ in range(16)],
-                [1 + s for _ in range(256) for s in range(16)], 1)
+    if name in ("mixed_256x16", "mixed_256x64"):
+        k = 16 if name == "mixed_256x16" else 64
+        return (mixed_batch(256), [i for i in range(256) for _ in range(k)],
+                [1 + s for _ in range(256) for s in range(k)], 1)
     if name == "hundred_2048":
         return [large_instance()], [0] * 2048, list(range(1, 2049)), 1
     if name == "hundred_fit_2048":
@@ -119,7 +122,7 @@ def workload(name: str):
 
 
 WORKLOADS = ("twenty_1024x4", "mixed_256x16", "hundred_2048", "hundred_fit_2048",
-             "blocks_twenty_1024x4")
+             "blocks_twenty_1024x4", "mixed_256x64")
 
 
 def is_block_workload(name: str) -> bool:

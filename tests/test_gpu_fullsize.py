@@ -8,7 +8,8 @@ reference's g++ build agree on every decoder), and, except for the
 880x349x330 workload whose host decode takes minutes, the C oracle decodes
 every decoder again here on a thread pool and must match field for field.
 Workloads (plp_amd.synth.workload): the 20-type instance with 1024 decoders
-x 4 passes, 256 uneven instances x 16 seeds in one launch, the 100-type
+x 4 passes, 256 uneven instances x 16 seeds in one launch (and x 64 seeds,
+a queue of eight grid rounds, with and without shared prefixes), the 100-type
 instance (600 items, about 100 placed) with 2048 decoders, the same 600 items
 in 880x349x330 (all placed, reference lists past 3000 points) with 2048
 decoders, and the block-building engine on the 20-type instance with 1024
@@ -172,6 +173,25 @@ def test_twenty_types_1024_decoders_4_passes(plp, ctx, orc):
 def test_256_instances_16_seeds(plp, ctx, orc):
     _, insts, dec_inst, seeds, passes, rows, pl = _workload_test(plp, ctx, orc, "mixed_256x16")
     _compare(plp, rows, pl, _oracle_all(orc, insts, dec_inst, seeds, passes))
+
+
+def test_256_instances_64_seeds_deep_queue(plp, ctx):
+    """16384 decoders, eight rounds of the persistent grid: the automatic
+    mode records the instances' trajectories and the decoders resume from
+    them (shared prefixes); every decoder equals its pinned digest, and so
+    does every decoder of a run with sharing off."""
+    from plp_amd import synth
+    texts, dec_inst, seeds, passes = synth.workload("mixed_256x64")
+    insts = [plp.Instance.from_text(t) for t in texts]
+    for share in ("auto", "off"):
+        eng = plp.Engine(ctx, use_graph=True, share_prefix=share)
+        eng.load(insts, dec_inst, seeds, plp.params(eps=EPS, theta=THETA))
+        eng.run(passes)
+        eng.sync()
+        rows, pl = eng.fetch()
+        eng.close()
+        assert all(r.err == 0 for r in rows)
+        _compare_pinned(plp, "mixed_256x64", rows, pl)
 
 
 def test_hundred_types_2048_decoders(plp, ctx, orc):

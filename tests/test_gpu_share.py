@@ -214,7 +214,7 @@ def test_edge_instances_shared(plp, ctx, orc, eps):
 
 
 @pytest.mark.parametrize("name", ["twenty_1024x4", "mixed_256x16", "hundred_2048",
-                                  "hundred_fit_2048"])
+                                  "hundred_fit_2048", "mixed_256x64"])
 def test_full_size_workloads_shared_match_pins(plp, ctx, name):
     """Every decoder of the benchmark workloads, shared, for both verdict
     kinds, against the digests the C oracle and the reference's g++ build
