@@ -138,6 +138,9 @@ struct DecodeArgs {
     int32_t traj_list_cap;   // record: lists longer than this are not kept (the decoders'
                              // capacity; a decoder reaching such a list stops anyway)
     int32_t pad_traj;
+    uint32_t* ticks;         // per decoder, or null: the time its work took in the run
+                             // (10 ns ticks of the constant-rate counter), for ordering the
+                             // queue by measured length
     TrajHdr* traj_hdr;       // per instance
     TrajStep* traj_step;     // per instance: step_cap entries, by placing step
     void* traj_keys;         // per instance: traj_key_cap keys (Key of the mode)

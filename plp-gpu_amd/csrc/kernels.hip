@@ -328,6 +328,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
         block_sync();
         if (item >= a.n_work) break;
         const int d = uni(a.work_order[item]);
+        const uint64_t t_item = __builtin_amdgcn_s_memrealtime();
         // recording trajectories, the queue items are instances
         const int inst_i = a.traj == 1 ? d : uni(a.dec_inst[d]);
         const InstHdr I = a.inst[inst_i];
@@ -1048,7 +1049,10 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
             }
             if (lane == 0) a.state[d] = sst;
         }
-        if (wid == 0 && lane == 0 && a.traj != 1) a.out_rec[d] = row_record(sst, a.dec_seed[d]);
+        if (wid == 0 && lane == 0 && a.traj != 1) {
+            a.out_rec[d] = row_record(sst, a.dec_seed[d]);
+            if (a.ticks) a.ticks[d] = (uint32_t)(__builtin_amdgcn_s_memrealtime() - t_item);
+        }
         // work_counter[1] counts the decoders that end with an error, so the
         // host reads one word, not every state, when none did
         if (wid == 0 && lane == 0 && sst.err != 0 && a.traj != 1) atomicAdd(a.work_counter + 1, 1u);
