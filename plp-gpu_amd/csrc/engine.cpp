@@ -82,6 +82,9 @@ struct plp_engine {
     // steps, the boxes and the kept lists.
     int share_mode = 0;
     bool share_eps = false, share_fit = true;
+    // runs since the load, and whether the queue was ordered by measured times
+    int runs = 0;
+    bool ordered = false;
     int t_wpd = 4, t_cap = 0, t_lds = 0, t_grid = 0;
     size_t t_key_cap = 0;   // keys per instance
     DevBuf d_traj;
@@ -362,6 +365,7 @@ int order_by_run_time(plp_engine* g) {
                        4 * (size_t)g->n_dec, hipMemcpyHostToDevice, g->ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g->ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "H2D work order");
+    g->ordered = true;
     return PLP_OK;
 }
 
@@ -889,6 +893,8 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
     g->passes_done = 0;
     g->need_reset = true;
     g->states_fresh = false;
+    g->runs = 0;
+    g->ordered = false;
     g->loaded = true;
     return PLP_OK;
 }
@@ -909,6 +915,14 @@ extern "C" int plp_engine_run(plp_engine* g, int32_t n_passes) {
     hipError_t e = hipSetDevice(g->ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     const int reset = g->need_reset ? 1 : 0;
+    // the second run of a load takes its instances in the order of the times
+    // the first run measured (a batch decoded once, like a served request,
+    // never pays for it)
+    if (g->runs == 1 && !g->ordered && !g->blocks && g->n_inst > 1) {
+        const int rc0 = order_by_run_time(g);
+        if (rc0 != PLP_OK) return rc0;
+    }
+    g->runs++;
     if ((e = hipEventRecord(g->ev0, g->ctx->stream)) != hipSuccess) return hip_fail(e, "event");
     int rc = launch_run(g, n_passes, reset);
     if (rc != PLP_OK) return rc;
