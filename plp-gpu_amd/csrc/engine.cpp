@@ -73,6 +73,7 @@ struct plp_engine {
     int g_passes = -1, g_reset = -1, g_grid = -1, g_cap = -1, g_wpd = -1, g_verd = -1,
         g_share = -1;
     const void* g_tbuf = nullptr;
+    int g_tshape = -1;
     // Shared prefixes (plpk::TrajHdr): the mode (plp_engine_config
     // share_prefix; the autotuner sets 1 or 2), whether the automatic mode
     // expects decoders to follow their trajectories for long (no RNG, or eps
@@ -86,6 +87,8 @@ struct plp_engine {
     int runs = 0;
     bool ordered = false;
     int t_wpd = 4, t_cap = 0, t_lds = 0, t_grid = 0;
+    int t_wpd_req = 0;   // recording waves, 0: at least 4 and the decoders'
+    int t_verd = -1;     // recording verdict kind, -1: the decoders'
     size_t t_key_cap = 0;   // keys per instance
     DevBuf d_traj;
 };
@@ -297,16 +300,17 @@ plpk::DecodeArgs make_args(plp_engine* g, int n_passes, int reset, const int32_t
 // of the recorded trajectories.  Clears g->share_fit when no block can hold
 // a trajectory at the decoders' capacity.
 int traj_prepare(plp_engine* g) {
-    g->t_wpd = std::max(4, g->wpd);
+    g->t_wpd = g->t_wpd_req > 0 ? g->t_wpd_req : std::max(4, g->wpd);
+    const int tv = g->t_verd >= 0 ? g->t_verd : g->verd;
     const int wide_cap = round_up(std::max(2 * g->ep_cap, 256), 32);
     auto lds_of = [&](int cap) {
-        return g->verd == plpk::kPointState
+        return tv == plpk::kPointState
                    ? plpk::pts_layout(cap, g->full_box, 0, 0, g->mode, g->step_cap).total
                    : plpk::lds_layout(cap, g->full_box, 0, g->t_wpd, 0, g->mode, g->step_cap)
                          .total;
     };
     int st = 0;
-    if (plpk::decode_static_lds(g->t_wpd, g->mode, g->verd, &st) != hipSuccess) {
+    if (plpk::decode_static_lds(g->t_wpd, g->mode, tv, &st) != hipSuccess) {
         (void)hipGetLastError();
         st = 4096;
     }
@@ -317,7 +321,7 @@ int traj_prepare(plp_engine* g) {
         return PLP_OK;
     }
     int occ = 0;
-    hipError_t e = plpk::decode_occupancy(g->t_wpd, g->mode, g->verd, lds_of(cap), &occ);
+    hipError_t e = plpk::decode_occupancy(g->t_wpd, g->mode, tv, lds_of(cap), &occ);
     if (e != hipSuccess) return hip_fail(e, "trajectory occupancy");
     if (occ < 1) {
         g->share_fit = false;
@@ -431,6 +435,7 @@ int launch_run(plp_engine* g, int n_passes, int reset) {
         t.ep_cap = g->t_cap;
         t.box_cap = g->full_box;
         t.use_rng = 0;
+        t.verdicts = g->t_verd >= 0 ? g->t_verd : g->verd;
         traj_args(g, t, 1);
         traj_args(g, a, 2);
     }
@@ -444,7 +449,8 @@ int launch_run(plp_engine* g, int n_passes, int reset) {
     if (g->cfg.use_graph) {
         if (!g->gexec || g->g_passes != n_passes || g->g_reset != reset || g->g_grid != g->grid ||
             g->g_cap != g->ep_cap || g->g_wpd != g->wpd || g->g_verd != g->verd ||
-            g->g_share != (rec ? 1 : 0) || g->g_tbuf != g->d_traj.p) {
+            g->g_share != (rec ? 1 : 0) || g->g_tbuf != g->d_traj.p ||
+            g->g_tshape != g->t_wpd * 4 + t.verdicts) {
             drop_graph(g);
             e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
             if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
@@ -462,6 +468,7 @@ int launch_run(plp_engine* g, int n_passes, int reset) {
             g->g_verd = g->verd;
             g->g_share = rec ? 1 : 0;
             g->g_tbuf = g->d_traj.p;
+            g->g_tshape = g->t_wpd * 4 + t.verdicts;
         }
         e = hipGraphLaunch(g->gexec, c->stream);
         if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
@@ -895,6 +902,8 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
     g->states_fresh = false;
     g->runs = 0;
     g->ordered = false;
+    g->t_wpd_req = 0;
+    g->t_verd = -1;
     g->loaded = true;
     return PLP_OK;
 }
@@ -1211,6 +1220,35 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
         g->share_mode = t[1] < 0.97f * t[0] ? 2 : 1;
         bms = g->share_mode == 2 ? t[1] : t[0];
         drop_graph(g);
+        if (g->share_mode == 2) {
+            // the recording launch's waves and verdict kind (its own, apart
+            // from the decoders'), the fastest of three runs each
+            float best_t = t[1];
+            int bw = g->t_wpd_req, bv = g->t_verd;
+            for (int tw : {4, 8})
+                for (int tv : {plpk::kPointState, plpk::kPairCache}) {
+                    g->t_wpd_req = tw;
+                    g->t_verd = tv;
+                    float ms = -1.0f;
+                    drop_graph(g);
+                    for (int r = 0; r < 3; r++) {
+                        g->need_reset = true;
+                        rc = plp_engine_run(g, n_passes);
+                        if (rc == PLP_OK) rc = plp_engine_sync(g);
+                        if (rc != PLP_OK) return rc;
+                        if (ms < 0.0f || g->last_ms < ms) ms = g->last_ms;
+                    }
+                    if (ms < best_t) {
+                        best_t = ms;
+                        bw = tw;
+                        bv = tv;
+                    }
+                }
+            g->t_wpd_req = bw;
+            g->t_verd = bv;
+            bms = best_t;
+            drop_graph(g);
+        }
     }
     g->need_reset = true;
     g->passes_done = 0;
