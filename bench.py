@@ -229,15 +229,16 @@ def timed_workload(plp_amd, synth, ctx, name: str, steps: int, warmup: int, shap
     prm = plp_amd.params(eps=EPS, theta=THETA)
     blocks = plp_amd.block_params() if synth.is_block_workload(name) else None
     if shape:
-        wpd, cap, bpc, boxes, verd, share = ([int(v) for v in shape.split(",")] + [0, 0, 0])[:6]
+        wpd, cap, bpc, boxes, verd, share, twpd, tverd = \
+            ([int(v) for v in shape.split(",")] + [0] * 5)[:8]
         eng = plp_amd.Engine(ctx, ep_cap=cap, blocks_per_cu=bpc, use_graph=True,
                              waves_per_decoder=wpd, box_cap=boxes, verdicts=verd,
-                             share_prefix=share)
+                             share_prefix=share, trace_waves=twpd, trace_verdicts=tverd)
         eng.load(insts, dec_inst, seeds, prm, blocks=blocks)
         # automatic verdicts without the autotuner: point states (2)
         tuned = {"waves_per_decoder": wpd, "ep_cap": cap, "blocks_per_cu": bpc, "box_cap": boxes,
                  "verdicts": verd if (verd or blocks is not None) else 2, "share_prefix": share,
-                 "fixed": True}
+                 "trace_waves": twpd, "trace_verdicts": tverd, "fixed": True}
     else:
         eng = plp_amd.Engine(ctx, use_graph=True)
         eng.load(insts, dec_inst, seeds, prm, blocks=blocks)
@@ -288,8 +289,9 @@ def main() -> int:
     ap.add_argument("--workload", default=HEADLINE,
                     help="headline workload (plp_amd.synth.workload names)")
     ap.add_argument("--shape", default="",
-                    help="fixed launch shape WPD,EP_CAP,BLOCKS_PER_CU[,BOX_CAP[,VERDICTS[,SHARE]]] "
-                         "instead of autotuning (SHARE: plp_engine_config.share_prefix)")
+                    help="fixed launch shape WPD,EP_CAP,BLOCKS_PER_CU[,BOX_CAP[,VERDICTS[,SHARE"
+                         "[,TRACE_WAVES,TRACE_VERDICTS]]]] instead of autotuning (the "
+                         "plp_engine_config fields of those names)")
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="length of the Poisson serving run in extras (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)

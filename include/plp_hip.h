@@ -172,6 +172,14 @@ typedef struct {
                                     diverge early, which the recording only delays), else 1;
                                     plp_engine_autotune tries both.  Never with count_work.
                                     Results do not depend on it. */
+    int32_t trace_waves;         /* with shared prefixes: wavefronts of the recording launch's
+                                    blocks (one block per instance), 4 or 8; 0 = the
+                                    decoders' count, at least 4 */
+    int32_t trace_verdicts;      /* with shared prefixes: the recording launch's verdict kind,
+                                    as verdicts (1 pair states, 2 point states); 0 = the
+                                    decoders'.  plp_engine_autotune picks both when
+                                    share_prefix, trace_waves and trace_verdicts are 0 and it
+                                    keeps sharing on, and returns them. */
 } plp_engine_config;
 
 typedef struct {

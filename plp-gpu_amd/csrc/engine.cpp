@@ -492,6 +492,15 @@ int read_states(plp_engine* g) {
 
 }  // namespace
 
+static bool valid_waves(int w) { return w == 0 || w == 1 || w == 2 || w == 4 || w == 8; }
+
+// The recording launch's requested waves and verdict kind from the config.
+static void trace_shape_from_cfg(plp_engine* g) {
+    g->t_wpd_req = g->cfg.trace_waves;
+    g->t_verd = g->cfg.trace_verdicts == 0 ? -1
+              : g->cfg.trace_verdicts == 1 ? plpk::kPairCache : plpk::kPointState;
+}
+
 extern "C" int plp_engine_create(plp_ctx* ctx, const plp_engine_config* cfg, plp_engine** out) {
     if (!ctx || !out) return fail(PLP_E_ARG, "NULL argument");
     *out = nullptr;
@@ -501,12 +510,16 @@ extern "C" int plp_engine_create(plp_ctx* ctx, const plp_engine_config* cfg, plp
     g->ctx = ctx;
     if (cfg) g->cfg = *cfg;
     const int wq = g->cfg.waves_per_decoder;
-    if ((wq != 0 && wq != 1 && wq != 2 && wq != 4 && wq != 8) || g->cfg.ep_cap < 0 ||
+    if (!valid_waves(wq) || g->cfg.ep_cap < 0 ||
         g->cfg.box_cap < 0 || g->cfg.blocks_per_cu < 0 || g->cfg.verdicts < 0 ||
-        g->cfg.verdicts > 2 || g->cfg.share_prefix < 0 || g->cfg.share_prefix > 2) {
+        g->cfg.verdicts > 2 || g->cfg.share_prefix < 0 || g->cfg.share_prefix > 2 ||
+        (g->cfg.trace_waves != 0 && g->cfg.trace_waves != 4 && g->cfg.trace_waves != 8) ||
+        g->cfg.trace_verdicts < 0 ||
+        g->cfg.trace_verdicts > 2) {
         delete g;
-        return fail(PLP_E_ARG, "config: waves_per_decoder must be 0, 1, 2, 4 or 8; caps >= 0; "
-                               "verdicts and share_prefix 0, 1 or 2");
+        return fail(PLP_E_ARG, "config: waves_per_decoder must be 0, 1, 2, 4 or 8, "
+                               "trace_waves 0, 4 or 8; caps >= 0; verdicts, share_prefix and "
+                               "trace_verdicts 0, 1 or 2");
     }
     g->wpd = wq > 0 ? wq : 4;
     g->verd = g->cfg.verdicts == 1 ? plpk::kPairCache : plpk::kPointState;
@@ -902,8 +915,7 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
     g->states_fresh = false;
     g->runs = 0;
     g->ordered = false;
-    g->t_wpd_req = 0;
-    g->t_verd = -1;
+    trace_shape_from_cfg(g);
     g->loaded = true;
     return PLP_OK;
 }
@@ -1220,7 +1232,7 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
         g->share_mode = t[1] < 0.97f * t[0] ? 2 : 1;
         bms = g->share_mode == 2 ? t[1] : t[0];
         drop_graph(g);
-        if (g->share_mode == 2) {
+        if (g->share_mode == 2 && saved.trace_waves == 0 && saved.trace_verdicts == 0) {
             // the recording launch's waves and verdict kind (its own, apart
             // from the decoders'), the fastest of three runs each
             float best_t = t[1];
@@ -1258,6 +1270,10 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
         best->waves_per_decoder = g->wpd;
         best->verdicts = g->blocks ? 0 : (g->verd == plpk::kPairCache ? 1 : 2);
         best->share_prefix = g->blocks ? 0 : g->share_mode;
+        const bool tr = !g->blocks && g->share_mode == 2;
+        best->trace_waves = tr ? (g->t_wpd_req > 0 ? g->t_wpd_req : std::max(4, g->wpd)) : 0;
+        const int tv = g->t_verd >= 0 ? g->t_verd : g->verd;
+        best->trace_verdicts = tr ? (tv == plpk::kPairCache ? 1 : 2) : 0;
     }
     if (best_ms) *best_ms = bms;
     return PLP_OK;

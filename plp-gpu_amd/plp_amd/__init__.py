@@ -58,7 +58,8 @@ class Params(C.Structure):
 class EngineConfig(C.Structure):
     _fields_ = [("waves_per_decoder", C.c_int32), ("ep_cap", C.c_int32),
                 ("blocks_per_cu", C.c_int32), ("use_graph", C.c_int32), ("count_work", C.c_int32),
-                ("box_cap", C.c_int32), ("verdicts", C.c_int32), ("share_prefix", C.c_int32)]
+                ("box_cap", C.c_int32), ("verdicts", C.c_int32), ("share_prefix", C.c_int32),
+                ("trace_waves", C.c_int32), ("trace_verdicts", C.c_int32)]
 
 
 VERDICTS = {"auto": 0, "pair_cache": 1, "point_state": 2}
@@ -295,19 +296,25 @@ class Engine:
     """Device-resident decoders (seam 3)."""
 
     def __init__(self, ctx: Ctx, ep_cap=0, blocks_per_cu=0, use_graph=False, waves_per_decoder=0,
-                 count_work=False, box_cap=0, verdicts=0, share_prefix=0):
+                 count_work=False, box_cap=0, verdicts=0, share_prefix=0, trace_waves=0,
+                 trace_verdicts=0):
         """verdicts: 0 / "auto", 1 / "pair_cache" or 2 / "point_state"
         (plp_engine_config.verdicts); share_prefix: 0 / "auto", 1 / "off" or
-        2 / "on" (plp_engine_config.share_prefix)."""
+        2 / "on" (plp_engine_config.share_prefix); trace_waves (0, 4, 8) and
+        trace_verdicts (as verdicts, 0 = the decoders'): the recording
+        launch's shape when prefixes are shared."""
         self.ctx = ctx
         self.h = C.c_void_p()
         if isinstance(verdicts, str):
             verdicts = VERDICTS[verdicts]
         if isinstance(share_prefix, str):
             share_prefix = SHARE[share_prefix]
+        if isinstance(trace_verdicts, str):
+            trace_verdicts = VERDICTS[trace_verdicts]
         cfg = EngineConfig(int(waves_per_decoder), int(ep_cap), int(blocks_per_cu),
                            int(bool(use_graph)), int(bool(count_work)), int(box_cap),
-                           int(verdicts), int(share_prefix))
+                           int(verdicts), int(share_prefix), int(trace_waves),
+                           int(trace_verdicts))
         _check(_lib.plp_engine_create(ctx.h, C.byref(cfg), C.byref(self.h)))
         ctx._children.add(self)
         self.n_dec = 0
@@ -363,7 +370,8 @@ class Engine:
         _check(_lib.plp_engine_autotune(self.h, n_passes, C.byref(cfg), C.byref(ms)))
         return {"waves_per_decoder": cfg.waves_per_decoder, "ep_cap": cfg.ep_cap,
                 "blocks_per_cu": cfg.blocks_per_cu, "box_cap": cfg.box_cap,
-                "verdicts": cfg.verdicts, "share_prefix": cfg.share_prefix}, ms.value
+                "verdicts": cfg.verdicts, "share_prefix": cfg.share_prefix,
+                "trace_waves": cfg.trace_waves, "trace_verdicts": cfg.trace_verdicts}, ms.value
 
     def max_items(self) -> int:
         mi = C.c_int32()

@@ -232,3 +232,21 @@ def test_full_size_workloads_shared_match_pins(plp, ctx, name):
         bad = [k for k in range(len(rows)) if per[k] != pin["per_decoder"][k]]
         assert not bad, f"{verdicts}: {len(bad)} decoders differ, first {bad[:8]}"
         assert digest == pin["digest"]
+
+
+@pytest.mark.parametrize("trace_waves,trace_verdicts", [(4, "pair_cache"), (8, "point_state"),
+                                                        (8, "pair_cache")])
+def test_recording_launch_shapes(plp, ctx, trace_waves, trace_verdicts):
+    """The recording launch's own waves and verdict kind (trace_waves,
+    trace_verdicts) against 2-wave point-state decoders: the first 512
+    decoders of the uneven 256-instance batch equal their pinned digests."""
+    from plp_amd import synth
+    texts, dec_inst, seeds, passes = synth.workload("mixed_256x16")
+    with open(golden_path("fullsize_digests.json")) as f:
+        pin = json.load(f)["workloads"]["mixed_256x16"]["per_decoder"][:512]
+    rows, pl = _decode(plp, ctx, [plp.Instance.from_text(t) for t in texts[:32]],
+                       dec_inst[:512], seeds[:512], plp.params(eps=0.01, theta=0.9), passes,
+                       waves_per_decoder=2, verdicts="point_state", use_graph=True,
+                       trace_waves=trace_waves, trace_verdicts=trace_verdicts)
+    assert all(r.err == 0 for r in rows)
+    assert plp.results_digest(rows, pl)[1] == pin
