@@ -248,18 +248,18 @@ def timed_workload(plp_amd, synth, ctx, name: str, steps: int, warmup: int, shap
     n = len(seeds)
     rows = (plp_amd.RowResult * n)()
     pl = np.zeros((n, eng.max_items(), 7), np.int32)
+    # each step: reset, run, then sync + fetch in one call whose copy of
+    # every decoder's row and packing overlaps the decode
     for _ in range(warmup):
         eng.reset()
         eng.run(passes)
-        eng.sync()
-        eng.fetch(out=(rows, pl))
+        eng.sync_fetch(out=(rows, pl))
     kernel_ms = []
     t0 = time.perf_counter()
     for _ in range(steps):
         eng.reset()
         eng.run(passes)
-        eng.sync()
-        eng.fetch(out=(rows, pl))
+        eng.sync_fetch(out=(rows, pl))
         kernel_ms.append(eng.last_ms())
     elapsed = time.perf_counter() - t0
     assert all(r.err == 0 for r in rows)

@@ -270,6 +270,16 @@ int plp_engine_max_items(plp_engine *eng, int32_t *max_items);
  * their results into the engine's pinned host memory while they run, so this
  * is a host copy after the stream synchronises, with no transfer of its own. */
 int plp_engine_fetch(plp_engine *eng, plp_row_result *rows, int32_t *placements);
+/* plp_engine_sync then plp_engine_fetch, with the same results and errors,
+ * but the copy overlaps the decode: each decoder publishes a completion word
+ * in pinned memory after its row and packing, and host threads copy every
+ * decoder's results as soon as its word is set, so that only the last
+ * decoders' copies follow the kernel.  Replaces the per-pass slot copy of
+ * run_device's loop (gpu/plp_gpu.cu:1353-1379, 1395-1437), overlapped.  With
+ * more than one run enqueued since the last sync, count_work, the block
+ * engine or decoders rerun at a larger capacity it copies after the sync,
+ * as plp_engine_fetch does. */
+int plp_engine_sync_fetch(plp_engine *eng, plp_row_result *rows, int32_t *placements);
 /* The same results without a copy: *rows points at n_dec records and
  * *placements (may be NULL) at n_dec * row_stride ints (decoder d's packing
  * at d * row_stride, JSON row order), both in the engine's pinned host

@@ -61,6 +61,10 @@ struct plp_engine {
     bool states_fresh = false;
     DevBuf d_retry, d_mt, d_mt0, d_state, d_counter, d_ticks;
     HostBuf h_rows, h_rec;   // results, written by the kernels (fine-grained pinned memory)
+    // per decoder: 1 once its results are in h_rows / h_rec (item decoder
+    // launches; cleared by every sync), and the runs enqueued since the last sync
+    HostBuf h_done;
+    int runs_unsynced = 0;
     std::vector<int32_t> h_order;
     std::vector<int32_t> h_dec_inst;
     std::vector<uint64_t> h_seed;
@@ -121,6 +125,8 @@ public:
         cv_.notify_all();
         for (std::thread& t : th_) t.join();
     }
+    // Threads that take parts: the workers and the caller.
+    int threads() const { return (int)th_.size() + 1; }
     // fn(i) for every i in [0, parts); returns when all have run.
     void run(int parts, const std::function<void(int)>& fn) {
         std::lock_guard<std::mutex> job(job_mu_);
@@ -290,6 +296,7 @@ plpk::DecodeArgs make_args(plp_engine* g, int n_passes, int reset, const int32_t
     a.traj_step = nullptr;
     a.traj_keys = nullptr;
     a.traj_box = nullptr;
+    a.done = (uint32_t*)g->h_done.dev;
     return a;
 }
 
@@ -524,7 +531,7 @@ extern "C" int plp_engine_create(plp_ctx* ctx, const plp_engine_config* cfg, plp
     g->wpd = wq > 0 ? wq : 4;
     g->verd = g->cfg.verdicts == 1 ? plpk::kPairCache : plpk::kPointState;
     // fine-grained: the kernels' stores reach host memory while they run
-    g->h_rows.flags = g->h_rec.flags = hipHostMallocMapped | hipHostMallocCoherent;
+    g->h_rows.flags = g->h_rec.flags = g->h_done.flags = hipHostMallocMapped | hipHostMallocCoherent;
     if ((e = hipEventCreate(&g->ev0)) != hipSuccess || (e = hipEventCreate(&g->ev1)) != hipSuccess) {
         delete g;
         return hip_fail(e, "hipEventCreate");
@@ -555,6 +562,7 @@ extern "C" void plp_engine_destroy(plp_engine* g) {
     g->h_tab.release();
     g->h_rows.release();
     g->h_rec.release();
+    g->h_done.release();
     if (g->ev0) (void)hipEventDestroy(g->ev0);
     if (g->ev1) (void)hipEventDestroy(g->ev1);
     if (g->h_failed) (void)hipHostFree(g->h_failed);
@@ -879,8 +887,11 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
         (rc = g->d_state.reserve(sizeof(DecState) * (size_t)n_dec, "decoder states")) ||
         (rc = g->d_ticks.reserve(4 * (size_t)n_dec, "decoder run times")) ||
         (rc = g->h_rows.reserve(rows_bytes, "pinned result packings")) ||
-        (rc = g->h_rec.reserve(sizeof(plpk::RowRecord) * (size_t)n_dec, "pinned result records")))
+        (rc = g->h_rec.reserve(sizeof(plpk::RowRecord) * (size_t)n_dec, "pinned result records")) ||
+        (rc = g->h_done.reserve(4 * (size_t)n_dec, "pinned completion words")))
         return rc;
+    std::memset(g->h_done.p, 0, 4 * (size_t)n_dec);
+    g->runs_unsynced = 0;
     // One copy: the stream has drained (above), so the staging buffer is free;
     // the decode launches that read the tables follow the copy on the stream.
     for (int t = 0; t < plp_engine::kNumTabs; t++)
@@ -944,6 +955,7 @@ extern "C" int plp_engine_run(plp_engine* g, int32_t n_passes) {
         if (rc0 != PLP_OK) return rc0;
     }
     g->runs++;
+    g->runs_unsynced++;
     if ((e = hipEventRecord(g->ev0, g->ctx->stream)) != hipSuccess) return hip_fail(e, "event");
     int rc = launch_run(g, n_passes, reset);
     if (rc != PLP_OK) return rc;
@@ -958,8 +970,7 @@ extern "C" int plp_engine_run(plp_engine* g, int32_t n_passes) {
     return PLP_OK;
 }
 
-extern "C" int plp_engine_sync(plp_engine* g) {
-    if (!g || !g->loaded) return fail(PLP_E_ARG, "engine not loaded");
+static int sync_impl(plp_engine* g) {
     plp_ctx* c = g->ctx;
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
@@ -1026,6 +1037,81 @@ extern "C" int plp_engine_sync(plp_engine* g) {
             return rc;
         drop_graph(g);
     }
+    return PLP_OK;
+}
+
+extern "C" int plp_engine_sync(plp_engine* g) {
+    if (!g || !g->loaded) return fail(PLP_E_ARG, "engine not loaded");
+    const int rc = sync_impl(g);
+    // the launches of the runs just synchronised (reruns included) have
+    // ended: their completion words start the next run at 0
+    std::memset(g->h_done.p, 0, 4 * (size_t)g->n_dec);
+    g->runs_unsynced = 0;
+    return rc;
+}
+
+namespace {
+
+// One decoder's row and packing from the pinned result memory.
+void copy_result(const plp_engine* g, int d, plp_row_result* rows, int32_t* placements) {
+    const plpk::RowRecord* rec = (const plpk::RowRecord*)g->h_rec.p;
+    std::memcpy(rows + d, rec + d, sizeof(plp_row_result));
+    if (placements) {
+        const size_t stride = (size_t)g->max_items * 7;
+        std::memcpy(placements + stride * (size_t)d,
+                    (const int32_t*)g->h_rows.p + stride * (size_t)d,
+                    4 * 7 * (size_t)std::max(0, rows[d].n_boxes));
+    }
+}
+
+}  // namespace
+
+extern "C" int plp_engine_sync_fetch(plp_engine* g, plp_row_result* rows, int32_t* placements) {
+    if (!g || !g->loaded || !rows) return fail(PLP_E_ARG, "engine not loaded");
+    if (g->passes_done == 0) return fail(PLP_E_ARG, "nothing decoded since the last load or reset");
+    hipError_t e = hipSetDevice(g->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    // Overlapped only for one outstanding item-decoder run: the completion
+    // words then tell exactly which decoders of this run have finished.
+    if (g->blocks || g->cfg.count_work || g->runs_unsynced != 1) {
+        const int rc = plp_engine_sync(g);
+        return rc != PLP_OK ? rc : plp_engine_fetch(g, rows, placements);
+    }
+    const uint32_t* done = (const uint32_t*)g->h_done.p;
+    std::atomic<bool> missed{false};
+    // one part per pool thread (the caller included), each polling its
+    // decoders' completion words and copying a decoder's results as soon as
+    // its word is set; after the run's end event has completed, one more scan
+    // finds every remaining word set
+    const int parts = copy_pool().threads();
+    const int per = (g->n_dec + parts - 1) / parts;
+    copy_pool().run(parts, [&](int i) {
+        const int d0 = i * per, d1 = std::min(g->n_dec, d0 + per);
+        std::vector<int> left;
+        for (int d = d0; d < d1; d++) left.push_back(d);
+        bool ended = false;
+        while (!left.empty()) {
+            size_t k = 0;
+            for (const int d : left) {
+                if (__atomic_load_n(done + d, __ATOMIC_ACQUIRE) != 0u) copy_result(g, d, rows, placements);
+                else left[k++] = d;
+            }
+            const bool progress = k < left.size();
+            left.resize(k);
+            if (left.empty() || progress) continue;
+            if (ended) {
+                missed.store(true);
+                break;
+            }
+            if (hipEventQuery(g->ev1) == hipSuccess) ended = true;
+            else
+                for (int s = 0; s < 64; s++) __builtin_ia32_pause();
+        }
+    });
+    const int rc = plp_engine_sync(g);
+    if (rc != PLP_OK) return rc;
+    // decoders rerun at a larger capacity (or a word not seen): copy all again
+    if (missed.load() || *g->h_failed != 0) return plp_engine_fetch(g, rows, placements);
     return PLP_OK;
 }
 

@@ -146,6 +146,9 @@ struct DecodeArgs {
     void* traj_keys;         // per instance: traj_key_cap keys (Key of the mode)
     int32_t* traj_box;       // per instance: step_cap boxes of 8 ints (x, y, z, x2, y2, z2,
                              // type, index of the first box resting on it or 2^30)
+    uint32_t* done;          // per decoder, or null: set to 1 (in host memory) once its row
+                             // record and packing are there, for a host copy that overlaps
+                             // the decode
 };
 
 // Dynamic LDS layout of one decoder block.  The pass-time region (two

@@ -1053,6 +1053,13 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
             a.out_rec[d] = row_record(sst, a.dec_seed[d]);
             if (a.ticks) a.ticks[d] = (uint32_t)(__builtin_amdgcn_s_memrealtime() - t_item);
         }
+        if (wid == 0 && a.traj != 1 && a.done != nullptr) {
+            // wave 0 wrote the record and every packing row: once they have
+            // reached host memory, the decoder's completion word
+            __threadfence_system();
+            if (lane == 0)
+                __hip_atomic_store(a.done + d, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
         // work_counter[1] counts the decoders that end with an error, so the
         // host reads one word, not every state, when none did
         if (wid == 0 && lane == 0 && sst.err != 0 && a.traj != 1) atomicAdd(a.work_counter + 1, 1u);

@@ -31,7 +31,8 @@ EXPORTS = [
     "plp_feas_check_batch",
     "plp_fsr_eval", "plp_rng_script", "plp_plain_run", "plp_engine_create",
     "plp_engine_destroy", "plp_engine_load", "plp_engine_load_blocks", "plp_engine_reset", "plp_engine_run", "plp_engine_sync",
-    "plp_engine_max_items", "plp_engine_fetch", "plp_engine_results", "plp_engine_last_ms",
+    "plp_engine_max_items", "plp_engine_fetch", "plp_engine_sync_fetch", "plp_engine_results",
+    "plp_engine_last_ms",
     "plp_engine_describe",
     "plp_engine_autotune", "plp_server_create", "plp_server_destroy", "plp_server_submit",
     "plp_server_poll", "plp_server_inflight", "plp_server_query", "plp_server_stats",
@@ -117,6 +118,7 @@ _sig = {
     "plp_engine_sync": (C.c_int, [_P]),
     "plp_engine_max_items": (C.c_int, [_P, _I32P]),
     "plp_engine_fetch": (C.c_int, [_P, C.POINTER(RowResult), _P]),
+    "plp_engine_sync_fetch": (C.c_int, [_P, C.POINTER(RowResult), _P]),
     "plp_engine_results": (C.c_int, [_P, C.POINTER(C.POINTER(RowResult)), C.POINTER(_I32P),
                                      _I32P]),
     "plp_engine_last_ms": (C.c_int, [_P, C.POINTER(C.c_float)]),
@@ -387,6 +389,18 @@ class Engine:
         else:
             rows, pl = out
         _check(_lib.plp_engine_fetch(self.h, rows, _ptr(pl) if placements else None))
+        return (list(rows) if out is None else rows), pl
+
+    def sync_fetch(self, placements=True, out=None):
+        """sync() and fetch() in one call (plp_engine_sync_fetch): each
+        decoder's results are copied as soon as it finishes, overlapping the
+        decode; same arguments and results as fetch()."""
+        if out is None:
+            rows = (RowResult * self.n_dec)()
+            pl = np.zeros((self.n_dec, self.max_items(), 7), np.int32) if placements else None
+        else:
+            rows, pl = out
+        _check(_lib.plp_engine_sync_fetch(self.h, rows, _ptr(pl) if placements else None))
         return (list(rows) if out is None else rows), pl
 
     def results(self):
