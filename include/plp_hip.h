@@ -276,9 +276,9 @@ int plp_engine_fetch(plp_engine *eng, plp_row_result *rows, int32_t *placements)
  * decoder's results as soon as its word is set, so that only the last
  * decoders' copies follow the kernel.  Replaces the per-pass slot copy of
  * run_device's loop (gpu/plp_gpu.cu:1353-1379, 1395-1437), overlapped.  With
- * more than one run enqueued since the last sync, count_work, the block
- * engine or decoders rerun at a larger capacity it copies after the sync,
- * as plp_engine_fetch does. */
+ * more than one run enqueued since the last sync, count_work or decoders
+ * rerun at a larger capacity it copies after the sync, as plp_engine_fetch
+ * does. */
 int plp_engine_sync_fetch(plp_engine *eng, plp_row_result *rows, int32_t *placements);
 /* The same results without a copy: *rows points at n_dec records and
  * *placements (may be NULL) at n_dec * row_stride ints (decoder d's packing

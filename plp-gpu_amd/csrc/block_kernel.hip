@@ -638,6 +638,13 @@ __global__ void __launch_bounds__(64 * NW, 3) block_kernel(BlockArgs ba) {
             if (lane == 0) a.state[d] = sst;
         }
         if (wid == 0 && lane == 0) a.out_rec[d] = row_record(sst, a.dec_seed[d]);
+        if (wid == 0 && a.done != nullptr) {
+            // wave 0 wrote the record and every packing row: once they have
+            // reached host memory, the decoder's completion word
+            __threadfence_system();
+            if (lane == 0)
+                __hip_atomic_store(a.done + d, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
         // work_counter[1] counts the decoders that end with an error, so the
         // host reads one word, not every state, when none did
         if (wid == 0 && lane == 0 && sst.err != 0) atomicAdd(a.work_counter + 1, 1u);

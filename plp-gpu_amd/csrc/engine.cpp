@@ -1071,9 +1071,9 @@ extern "C" int plp_engine_sync_fetch(plp_engine* g, plp_row_result* rows, int32_
     if (g->passes_done == 0) return fail(PLP_E_ARG, "nothing decoded since the last load or reset");
     hipError_t e = hipSetDevice(g->ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    // Overlapped only for one outstanding item-decoder run: the completion
-    // words then tell exactly which decoders of this run have finished.
-    if (g->blocks || g->cfg.count_work || g->runs_unsynced != 1) {
+    // Overlapped only for one outstanding run: the completion words then
+    // tell exactly which decoders of this run have finished.
+    if (g->cfg.count_work || g->runs_unsynced != 1) {
         const int rc = plp_engine_sync(g);
         return rc != PLP_OK ? rc : plp_engine_fetch(g, rows, placements);
     }
