@@ -255,8 +255,12 @@ __device__ __forceinline__ void write_rows_traj(int32_t* dst, const int32_t* tb,
 // each slot was given during that merge: the previous bits for the points
 // the new box does not touch, fresh ones for the touched points and the
 // corners.
-template <int NW, int M, int V>
+template <int NW, int M, int V, bool F>
 __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
+    // F: the full kernel, with the work count (count_work) and the shared
+    // prefixes' recording and resumption; without F they are compiled out
+    const bool cw = F && a.count_work != 0;
+    const int traj = F ? a.traj : 0;
     using Key = typename ModeTypes<M>::Key;
     using Sum = typename ModeTypes<M>::Sum;
     using Ent = typename PtsEntOf<M>::T;
@@ -284,7 +288,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
     uint32_t* mt;
     int32_t* s_step;
     if constexpr (PS) {
-        const PtsLayout lay = pts_layout(a.ep_cap, a.box_cap, a.use_rng, a.count_work, M,
+        const PtsLayout lay = pts_layout(a.ep_cap, a.box_cap, a.use_rng, cw, M,
                                          a.step_cap);
         ep_a = (Key*)(lds + lay.off_ep);
         ep_b = (Key*)(lds + lay.off_ep2);
@@ -302,7 +306,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
         mt = (uint32_t*)(lds + lay.off_mt);
         s_step = (int32_t*)(lds + lay.off_steps);
     } else {
-        const LdsLayout lay = lds_layout(a.ep_cap, a.box_cap, a.use_rng, NW, a.count_work, M,
+        const LdsLayout lay = lds_layout(a.ep_cap, a.box_cap, a.use_rng, NW, cw, M,
                                          a.step_cap);
         ep_a = (Key*)(lds + lay.off_ep);
         ep_b = (Key*)(lds + lay.off_ep2);
@@ -330,7 +334,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
         const int d = uni(a.work_order[item]);
         const uint64_t t_item = __builtin_amdgcn_s_memrealtime();
         // recording trajectories, the queue items are instances
-        const int inst_i = a.traj == 1 ? d : uni(a.dec_inst[d]);
+        const int inst_i = traj == 1 ? d : uni(a.dec_inst[d]);
         const InstHdr I = a.inst[inst_i];
         const int L = I.L, W = I.W, H = I.H, xb = I.xbits, yb = I.ybits;
         const int32_t* steps = a.step_type + I.step_off;
@@ -425,7 +429,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                         s++;
                         continue;
                     }
-                    if (a.count_work)
+                    if (cw)
                         work += (uint64_t)(n_act + n_dead) * (uint64_t)no * (uint64_t)n_box;
                     n_steps_eval++;
                     if (ti != fail_ti) break;
@@ -440,7 +444,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                         }
                         run += kWave;
                     }
-                    if (a.count_work)
+                    if (cw)
                         work += (uint64_t)(run - 1) * (uint64_t)(n_act + n_dead) *
                                 (uint64_t)no * (uint64_t)n_box;
                     n_steps_eval += (uint32_t)(run - 1);
@@ -463,7 +467,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
             int forced = -1;          // wave 0: the pick of the step the pass resumes at
             bool traj_pass = false;   // wave 0: the pass is the trajectory's, whole
             int koff = 0;             // wave 0, recording: keys kept so far
-            if (a.traj == 2) {
+            if (traj == 2) {
                 if (wid == 0) {
                     int quit0 = 0;
                     bool loaded = false;
@@ -526,7 +530,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 const int na_w = uni(sd.n_act), nb_w = uni(sd.n_box);
                 int wm = 0, dm = 0, hm = 0;
                 if (wid == 0) {
-                    if (a.count_work) live += (uint64_t)n_act * (uint64_t)no * (uint64_t)n_box;
+                    if (cw) live += (uint64_t)n_act * (uint64_t)no * (uint64_t)n_box;
                     const int32_t* mn = a.step_min + 8 * (size_t)(I.step_off + s_cur);
                     wm = mn[0];
                     dm = mn[1];
@@ -733,7 +737,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     X2 = x + uni(s_ors[orw + 3 * o]);
                     Y2 = y + uni(s_ors[orw + 3 * o + 1]);
                     Z2 = z + uni(s_ors[orw + 3 * o + 2]);
-                    if (a.traj == 1) {
+                    if (traj == 1) {
                         // recording: placing step n_box -- its step, pair count,
                         // list and the largest list held before it (a list the
                         // decoders' capacity cannot hold is not kept)
@@ -781,7 +785,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     }
                     const int need = merge_points(act, cur ? ep_a : ep_b, oidx, dead, n_act, n_dead,
                                                   box, n_box, x, y, z, X2, Y2, Z2, L, W, H, xb, yb,
-                                                  a.ep_cap, a.count_work != 0, uni(wm), uni(dm),
+                                                  a.ep_cap, cw, uni(wm), uni(dm),
                                                   uni(hm), smp, &ins_m);
                     max_ep = max(max_ep, (uint32_t)(n_act + n_dead));
                     ep_need = max(ep_need, (uint32_t)need);
@@ -929,7 +933,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
             }
 
             // ---- pass end (wave 0): stability score and D4' selection
-            if (wid == 0 && a.traj == 1) {
+            if (wid == 0 && traj == 1) {
                 // recording: the trajectory's packing, its score, and per box
                 // the first box that rests on it (from which a decoder resuming
                 // at placing step j knows the boxes marked kSupports then)
@@ -1015,7 +1019,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 st.work_pair_box += work;
                 st.work_live += live;
                 st.work_steps += n_steps_eval;
-                if (a.count_work) st.max_ep = max(st.max_ep, max_ep);
+                if (cw) st.max_ep = max(st.max_ep, max_ep);
                 st.ep_need = max(st.ep_need, ep_need);
                 st.box_need = max(st.box_need, box_over ? (uint32_t)a.box_cap + 1u : (uint32_t)n);
                 if (st.err == 0) {
@@ -1042,18 +1046,18 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
             block_sync();
             if (uni(sh.quit) != 0) break;
         }
-        if (wid == 0 && !skip_decoder && a.traj != 1) {
+        if (wid == 0 && !skip_decoder && traj != 1) {
             if (a.use_rng) {
                 for (int i = lane; i < 624; i += kWave) gmt[i] = mt[i];
                 if (lane == 0) gmt[624] = (uint32_t)mti;
             }
             if (lane == 0) a.state[d] = sst;
         }
-        if (wid == 0 && lane == 0 && a.traj != 1) {
+        if (wid == 0 && lane == 0 && traj != 1) {
             a.out_rec[d] = row_record(sst, a.dec_seed[d]);
             if (a.ticks) a.ticks[d] = (uint32_t)(__builtin_amdgcn_s_memrealtime() - t_item);
         }
-        if (wid == 0 && a.traj != 1 && a.done != nullptr) {
+        if (wid == 0 && traj != 1 && a.done != nullptr) {
             // wave 0 wrote the record and every packing row: once they have
             // reached host memory, the decoder's completion word
             __threadfence_system();
@@ -1062,7 +1066,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
         }
         // work_counter[1] counts the decoders that end with an error, so the
         // host reads one word, not every state, when none did
-        if (wid == 0 && lane == 0 && sst.err != 0 && a.traj != 1) atomicAdd(a.work_counter + 1, 1u);
+        if (wid == 0 && lane == 0 && sst.err != 0 && traj != 1) atomicAdd(a.work_counter + 1, 1u);
         block_sync();
     }
 }
@@ -1243,37 +1247,44 @@ hipError_t cached_occupancy(const void* kernel, int block_threads, int lds_bytes
 
 using DecodeFn = void (*)(DecodeArgs);
 
-template <int M, int V>
+template <int M, int V, bool F>
 static DecodeFn decode_fn_w(int waves) {
     switch (waves) {
-        case 1: return decode_kernel<1, M, V>;
-        case 2: return decode_kernel<2, M, V>;
-        case 4: return decode_kernel<4, M, V>;
-        case 8: return decode_kernel<8, M, V>;
+        case 1: return decode_kernel<1, M, V, F>;
+        case 2: return decode_kernel<2, M, V, F>;
+        case 4: return decode_kernel<4, M, V, F>;
+        case 8: return decode_kernel<8, M, V, F>;
         default: return nullptr;
     }
 }
 
-template <int V>
+template <int V, bool F>
 static DecodeFn decode_fn_v(int waves, int mode) {
     switch (mode) {
-        case kNarrow: return decode_fn_w<kNarrow, V>(waves);
-        case kWide: return decode_fn_w<kWide, V>(waves);
-        case kBig: return decode_fn_w<kBig, V>(waves);
+        case kNarrow: return decode_fn_w<kNarrow, V, F>(waves);
+        case kWide: return decode_fn_w<kWide, V, F>(waves);
+        case kBig: return decode_fn_w<kBig, V, F>(waves);
         default: return nullptr;
     }
 }
 
-// The decode_kernel instance for a launch shape, key width and verdict kind.
-static DecodeFn decode_fn(int waves, int mode, int verdicts) {
-    if (verdicts == kPointState) return decode_fn_v<kPointState>(waves, mode);
-    if (verdicts == kPairCache) return decode_fn_v<kPairCache>(waves, mode);
+// The decode_kernel instance for a launch shape, key width and verdict kind,
+// full (work count, shared prefixes) or not.
+static DecodeFn decode_fn(int waves, int mode, int verdicts, bool full = true) {
+    if (verdicts == kPointState)
+        return full ? decode_fn_v<kPointState, true>(waves, mode)
+                    : decode_fn_v<kPointState, false>(waves, mode);
+    if (verdicts == kPairCache)
+        return full ? decode_fn_v<kPairCache, true>(waves, mode)
+                    : decode_fn_v<kPairCache, false>(waves, mode);
     return nullptr;
 }
 
 hipError_t launch_decode(const DecodeArgs& a, int waves, int grid, int lds_bytes,
                          hipStream_t st) {
-    const DecodeFn f = decode_fn(waves, a.mode, a.verdicts);
+    // the instance without the work count and the shared prefixes when the
+    // launch uses neither (it keeps fewer values live: fewer spills)
+    const DecodeFn f = decode_fn(waves, a.mode, a.verdicts, a.count_work != 0 || a.traj != 0);
     if (!f) return hipErrorInvalidValue;
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, a.work_counter, 0u, 2);
     DecodeArgs arg = a;
