@@ -366,9 +366,17 @@ int order_by_run_time(plp_engine* g) {
     for (int i = 0; i < g->n_inst; i++)
         mean[(size_t)i] = cnt[(size_t)i] ? sum[(size_t)i] / cnt[(size_t)i] : 0.0;
     for (int d = 0; d < g->n_dec; d++) g->h_order[(size_t)d] = d;
-    std::stable_sort(g->h_order.begin(), g->h_order.end(), [&](int a, int b) {
-        return mean[(size_t)g->h_dec_inst[(size_t)a]] > mean[(size_t)g->h_dec_inst[(size_t)b]];
-    });
+    if (g->n_dec <= g->grid) {
+        // one round of the grid: every decoder starts at once, and the first
+        // blocks to take work sit on different CUs, so longest first gives
+        // each CU one decoder of each length class
+        std::stable_sort(g->h_order.begin(), g->h_order.end(),
+                         [&](int a, int b) { return t[(size_t)a] > t[(size_t)b]; });
+    } else {
+        std::stable_sort(g->h_order.begin(), g->h_order.end(), [&](int a, int b) {
+            return mean[(size_t)g->h_dec_inst[(size_t)a]] > mean[(size_t)g->h_dec_inst[(size_t)b]];
+        });
+    }
     // the table sits in the pinned staging copy of the load's tables too
     char* stage = (char*)g->h_tab.p + g->tab_off[plp_engine::kOrder];
     std::memcpy(stage, g->h_order.data(), 4 * (size_t)g->n_dec);
@@ -950,7 +958,7 @@ extern "C" int plp_engine_run(plp_engine* g, int32_t n_passes) {
     // the second run of a load takes its instances in the order of the times
     // the first run measured (a batch decoded once, like a served request,
     // never pays for it)
-    if (g->runs == 1 && !g->ordered && !g->blocks && g->n_inst > 1) {
+    if (g->runs == 1 && !g->ordered && !g->blocks && (g->n_inst > 1 || g->n_dec <= g->grid)) {
         const int rc0 = order_by_run_time(g);
         if (rc0 != PLP_OK) return rc0;
     }
