@@ -1356,6 +1356,16 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
             drop_graph(g);
         }
     }
+    if (!g->blocks && g->n_dec <= g->grid) {
+        // one round of the grid at the chosen shape: the queue takes the
+        // decoders by their own times measured at that shape (the probe ran at
+        // another one, often with a smaller grid)
+        g->need_reset = true;
+        rc = plp_engine_run(g, n_passes);
+        if (rc == PLP_OK) rc = plp_engine_sync(g);
+        if (rc == PLP_OK) rc = order_by_run_time(g);
+        if (rc != PLP_OK) return rc;
+    }
     g->need_reset = true;
     g->passes_done = 0;
     g->states_fresh = false;
