@@ -387,7 +387,8 @@ def main() -> int:
                                f"(plp_amd.synth.workload), eps {EPS}, theta {THETA}, "
                                f"{passes} pass(es) per decoder per step, one persistent "
                                f"hipGraph launch; step = reset + run + sync + fetch of every "
-                               f"row and placement",
+                               f"row and placement (plp_engine_sync_fetch: the copy overlaps "
+                               f"the decode)",
                    "decoders": n_dec, "passes_per_step": passes, "parallelism": "none (1 GPU)",
                    "launch": shape, "autotuned": head["tuned"]},
         "parity": parity,
@@ -406,6 +407,7 @@ def main() -> int:
                      "frac": None if (executed or achieved) is None
                      else round((executed if executed is not None else achieved) / VALU_PEAK_TOPS, 4),
                      "live_test_rate_tops": None if achieved is None else round(achieved, 3),
+                     "valu_lane_utilisation": prof.get("valu_lane_utilisation"),
                      "traffic": traffic,
                      "kernel": kernel_name, "kernel_ms": round(ms_launch, 4),
                      "live_pair_box_tests_per_launch": live, "ops_per_test": OPS_PER_TEST,
