@@ -544,7 +544,8 @@ __device__ __forceinline__ uint32_t corner_present(const Key* src, int n, const 
 // corner j takes the slot lane j holds in sm->cslot.  (The slots of the
 // points that leave are freed by the wave that updates the states, which
 // applies the same rule.)  The inserted corners (bit j: corner j) go to
-// *ins_out when given.
+// *ins_out when given, and to the LDS word *ins_post as 0x100 | ins as soon
+// as they are known, before the list is merged.
 struct SlotMerge {
     const uint16_t* src;   // slot of each src entry
     uint16_t* dst;         // slot of each dst entry
@@ -556,7 +557,7 @@ static __device__ int merge_points(const Key* src, Key* dst, uint16_t* oidx, Key
                             int& n_act, int& n_dead, Box* box, int n_box, int x, int y, int z,
                             int X2, int Y2, int Z2, int L, int W, int H, int xb, int yb, int cap,
                             bool count_work, int wm, int dm, int hm, SlotMerge* sm = nullptr,
-                            uint32_t* ins_out = nullptr) {
+                            uint32_t* ins_out = nullptr, int* ins_post = nullptr) {
     const int lane = lane_id();
     // Corner j: bit 0 -> x side, bit 1 -> y, bit 2 -> z; with the key's field
     // order (z, then y, then x) the keys increase with j.  Lane j < 8 holds
@@ -579,6 +580,10 @@ static __device__ int merge_points(const Key* src, Key* dst, uint16_t* oidx, Key
     cdead |= corner_kill(box, n_box, x, y, z, X2, Y2, Z2, wm, dm, hm);
     const uint32_t ins = ~cdead & 0xffu;
     if (ins_out) *ins_out = ins;
+    // posted for another wave of the block as soon as known: 0x100 | ins
+    if (ins_post != nullptr && lane == 0)
+        __hip_atomic_store(ins_post, (int)(0x100u | ins), __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
 
     // Dead list: drop the strictly inside points in place (a point moves to
     // an index no larger than its own, below the next 64-point chunk).

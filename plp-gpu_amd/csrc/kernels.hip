@@ -56,6 +56,8 @@ struct StepShared {
     uint16_t cslot[2][8];  // point states: slots for the corners of the box placed
                            // with n_box even / odd
     int nfree;             // point states: free slots on the stack
+    int ins_post;          // point states: 0x100 | the corners wave 0's merge inserts,
+                           // posted by it, taken and reset to 0 by the slot wave
     int rs_n, rs_box, rs_s;   // a pass resumed from a trajectory: its list length, boxes
                               // and step (rs_n 0: the pass starts from the origin)
 };
@@ -325,6 +327,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
     }
     const int orw = 18 * wid;   // this wave's orientations: s_ors[orw ..]
 
+    if (threadIdx.x == 0) sh.ins_post = 0;
     for (;;) {
         if (threadIdx.x == 0) sh.item = (int)atomicAdd(a.work_counter, 1u);
         block_sync();
@@ -771,6 +774,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 if (NW > 1) block_sync();   // every wave sees the new box
                 bool quit = false;
                 uint32_t ins_m = 0;   // wave 0: the corners its merge inserted
+                bool late_post = false;   // the slot wave stopped waiting for wave 0's post
                 // ---- extreme-point merge (wave 0) and, meanwhile, the pair
                 // states of the new box's corners for this type (last wave),
                 // which the next step takes when it has the same type.
@@ -786,7 +790,8 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     const int need = merge_points(act, cur ? ep_a : ep_b, oidx, dead, n_act, n_dead,
                                                   box, n_box, x, y, z, X2, Y2, Z2, L, W, H, xb, yb,
                                                   a.ep_cap, cw, uni(wm), uni(dm),
-                                                  uni(hm), smp, &ins_m);
+                                                  uni(hm), smp, &ins_m,
+                                                  (PS && NW > 1) ? &sh.ins_post : nullptr);
                     max_ep = max(max_ep, (uint32_t)(n_act + n_dead));
                     ep_need = max(ep_need, (uint32_t)need);
                     quit = need > a.ep_cap;   // PLP_E_EPCAP
@@ -816,7 +821,11 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                     // the inserted corners, into vbits[slot].
                     const bool same_next = s_cur + 1 < I.n_steps && uni(s_step[s_cur + 1]) == sv;
                     uint32_t ins = ins_m;
-                    if (wid == kFreeWave || wid == NW - 1 || (NW > 2 && wid == kLoWave))
+                    // the slot wave (when it is not wave 0) takes the inserted
+                    // corners from wave 0's merge, which posts them before
+                    // merging the list, after its own pass over the list
+                    constexpr bool kDefer = NW > 1;
+                    if ((wid == NW - 1 && !(kDefer && wid == kFreeWave)) || (NW > 2 && wid == kLoWave))
                         ins = corner_live(sh.nb, box, nb_w + 1, L, W, H, wm_s, dm_s, hm_s);
                     // the states of the inserted corners in sub, and their bits
                     // for the next step when it repeats the type (lane 8j + o:
@@ -856,6 +865,11 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                         // ep_cap + 8 slots, so the stack and these never meet.
                         int nt = 0;
                         uint16_t* const touched = fstack + a.ep_cap + 7;
+                        // deferred: lane j < 8 keeps the list index of the point
+                        // equal to corner j (the corners are distinct, so a point
+                        // equals at most one); such a point gives way to the corner
+                        // only if it is inserted, known after this loop
+                        int eq_at = -1;
                         for (int i0 = 0; i0 < na_w; i0 += kWave) {
                             const int i = i0 + lane;
                             const bool ok = i < na_w;
@@ -864,8 +878,22 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                             int px, py, pz;
                             key_xyz(v, xb, yb, px, py, pz);
                             bool eq = false;
+                            if constexpr (kDefer) {
+                                uint32_t em = 0;
 #pragma unroll
-                            for (int j = 0; j < 8; j++) eq |= (((ins >> j) & 1u) != 0) & (cks[j] == v);
+                                for (int j = 0; j < 8; j++) em |= (cks[j] == v) ? (1u << j) : 0u;
+                                uint64_t hit = ballot(ok & (em != 0));
+                                while (hit != 0) {
+                                    const int hl = __ffsll((unsigned long long)hit) - 1;
+                                    hit &= hit - 1;
+                                    const int hj = __ffs(read_lane(em, hl)) - 1;
+                                    if (lane == hj) eq_at = i0 + hl;
+                                }
+                            } else {
+#pragma unroll
+                                for (int j = 0; j < 8; j++)
+                                    eq |= (((ins >> j) & 1u) != 0) & (cks[j] == v);
+                            }
                             const bool gone =
                                 (px + wm_s > L) | (py + dm_s > W) | (pz + hm_s > H) |
                                 ((px < NB.x2) & (NB.x < px + wm_s) & (py < NB.y2) &
@@ -887,6 +915,36 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                                 if (touch) touched[-(nt + rank_below(tm))] = (uint16_t)i;
                                 nt += __popcll(tm);
                             }
+                        }
+                        if constexpr (kDefer) {
+                            // the inserted corners from wave 0's post; past a bound
+                            // (never reached: wave 0 posts early in its merge, which
+                            // it always runs) they are computed here, and the late
+                            // post is cleared after the step's barrier
+                            int pv = 0;
+                            for (int k = 0; k < (1 << 20); k++) {
+                                pv = __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                                    &sh.ins_post, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+                                if (pv != 0) break;
+                                __builtin_amdgcn_s_sleep(1);
+                            }
+                            if (pv != 0) {
+                                ins = (uint32_t)pv & 0xffu;
+                                if (lane == 0)
+                                    __hip_atomic_store(&sh.ins_post, 0, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                            } else {
+                                ins = corner_live(sh.nb, box, nb_w + 1, L, W, H, wm_s, dm_s, hm_s);
+                                late_post = true;
+                            }
+                            // a listed point equal to an inserted corner gives way
+                            // to it: its slot joins the free stack (the stack, these
+                            // at most 8 slots included, stays below the touched list)
+                            const bool leave =
+                                (lane < 8) && (((ins >> lane) & 1u) != 0) && (eq_at >= 0);
+                            const uint64_t lm = ballot(leave);
+                            if (leave) fstack[nf + rank_below(lm)] = sl_act[eq_at];
+                            nf += __popcll(lm);
                         }
                         // the inserted corners keep their slots; the next box's
                         // corners take new ones in their place (at most ep_cap
@@ -929,6 +987,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                 }
                 if (wid == 0) advance(s_cur + 1, quit);
                 block_sync();
+                if (late_post && lane == 0) sh.ins_post = 0;
                 cur ^= 1;
             }
 
