@@ -1111,7 +1111,9 @@ extern "C" int plp_engine_sync_fetch(plp_engine* g, plp_row_result* rows, int32_
                 missed.store(true);
                 break;
             }
-            if (hipEventQuery(g->ev1) == hipSuccess) ended = true;
+            // the run has ended, or failed (any answer but "not ready"):
+            // one more scan, then the sync reports what happened
+            if (hipEventQuery(g->ev1) != hipErrorNotReady) ended = true;
             else
                 for (int s = 0; s < 64; s++) __builtin_ia32_pause();
         }
