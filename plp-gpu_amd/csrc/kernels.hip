@@ -903,17 +903,29 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                             if (drop) fstack[nf + rank_below(fm)] = (uint16_t)sl;
                             nf += __popcll(fm);
                             const bool kept = ok & !drop;
+                            // a touched point's state takes the new box after the
+                            // loop, 64 touched points a pass
                             const bool touch = kept && pts_touches(px, py, pz, NB, WM, DM, HM);
-                            if (touch) {
-                                Pts<Ent> ps = pts[sl];
-                                pts_absorb(ps, px, py, pz, NB, WM, DM, HM);
-                                pts[sl] = ps;
-                            }
-                            if (same_next) {
-                                if (kept & !touch) vbits[sl] = (uint8_t)mask_bits(mask, i * no, no);
-                                const uint64_t tm = ballot(touch);
-                                if (touch) touched[-(nt + rank_below(tm))] = (uint16_t)i;
-                                nt += __popcll(tm);
+                            if (same_next & kept & !touch)
+                                vbits[sl] = (uint8_t)mask_bits(mask, i * no, no);
+                            const uint64_t tm = ballot(touch);
+                            if (touch) touched[-(nt + rank_below(tm))] = (uint16_t)i;
+                            nt += __popcll(tm);
+                        }
+                        if (nt != 0) {
+                            wave_sync();   // the touched indices are in LDS
+                            for (int k0 = 0; k0 < nt; k0 += kWave) {
+                                const int k = k0 + lane;
+                                const bool on = k < nt;
+                                const int ti = touched[-(on ? k : k0)];
+                                const int tsl = sl_act[ti];
+                                int tx, ty, tz;
+                                key_xyz(act[ti], xb, yb, tx, ty, tz);
+                                if (on) {
+                                    Pts<Ent> ps = pts[tsl];
+                                    pts_absorb(ps, tx, ty, tz, NB, WM, DM, HM);
+                                    pts[tsl] = ps;
+                                }
                             }
                         }
                         if constexpr (kDefer) {
@@ -961,7 +973,7 @@ __global__ void __launch_bounds__(64 * NW, 4) decode_kernel(DecodeArgs a) {
                             nf -= n_ins;
                         }
                         if (lane == 0) sh.nfree = nf;
-                        if (nt != 0) {
+                        if (same_next && nt != 0) {
                             // the touched points' bits, eight points at a time:
                             // lane 8k + o evaluates orientation o of the k-th
                             wave_sync();   // their states and indices are in LDS
