@@ -151,9 +151,10 @@ typedef struct {
     int32_t count_work;          /* 1: also keep the dead extreme points so that rows report the
                                     reference's list size (max_ep) and brute-force work
                                     (work_pair_box); costs a third list of ep_cap entries */
-    int32_t box_cap;             /* placed boxes a decoder holds in LDS, 0 = every item of the
-                                    largest instance; a decoder that would place one more is
-                                    rerun with room for all (results do not depend on it) */
+    int32_t box_cap;             /* placed boxes a decoder holds in LDS (block engine: items
+                                    its placed blocks hold), 0 = every item of the largest
+                                    instance; a decoder that would pass it is rerun with room
+                                    for all (results do not depend on it) */
     int32_t verdicts;            /* item decoder's verdicts: 1 = pair states of the last two
                                     steps, reused when the type repeats, other steps scanning
                                     the boxes; 2 = one orientation-free state per extreme

@@ -115,7 +115,8 @@ __global__ void __launch_bounds__(64 * NW, 3) block_kernel(BlockArgs ba) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ BlockShared sh;
     __shared__ DecState sst;
-    const BlockLds lay = block_lds_layout(a.ep_cap, a.box_cap, ba.max_types, a.use_rng, NW, BIG);
+    const BlockLds lay =
+        block_lds_layout(a.ep_cap, a.box_cap, a.step_cap, ba.max_types, a.use_rng, NW, BIG);
     Key* const ep_a = (Key*)(lds + lay.off_ep);
     Key* const ep_b = (Key*)(lds + lay.off_ep2);
     uint16_t* const oidx = (uint16_t*)(lds + lay.off_oidx);
@@ -195,6 +196,10 @@ __global__ void __launch_bounds__(64 * NW, 3) block_kernel(BlockArgs ba) {
             // ---- pass start (wave 0): all copies left, origin only, no blocks
             int n_act = 1, n_dead = 0, n_box = 0;
             uint32_t ep_need = 1;
+            // wave 0: the items the placed blocks hold, and whether a block
+            // would have passed the item capacity (box_cap)
+            int n_it = 0;
+            bool item_over = false;
             int wm = 0, dm = 0, hm = 0;
             if (wid == 0) {
                 for (int t = lane; t < nt; t += kWave) rem[t] = tr[t].qty;
@@ -430,7 +435,13 @@ __global__ void __launch_bounds__(64 * NW, 3) block_kernel(BlockArgs ba) {
                     }
                     fail_s = -1;
                     // ---- pick, place, merge (wave 0)
-                    if (wid == 0) {
+                    if (wid == 0 && n_it + cnt_s > a.box_cap) {
+                        // one block past the item capacity: this decoder stops
+                        // and the host reruns it with room for every item
+                        item_over = true;
+                        if (lane == 0) sh.quit = 1;
+                    } else if (wid == 0) {
+                        n_it += cnt_s;
                         int pe = -1;   // picked extreme point
                         if (use_eps && mt_random(mt, mti) >= 1.0 - a.eps) {
                             int idx = mt_randrange(mt, mti, cnt);
@@ -541,6 +552,8 @@ __global__ void __launch_bounds__(64 * NW, 3) block_kernel(BlockArgs ba) {
                 if (lane == 0) {
                     sst.err = err;
                     sst.ep_need = need_ep;
+                    sst.box_need = max(uni(sst.box_need),
+                                       item_over ? (uint32_t)a.box_cap + 1u : (uint32_t)n_it);
                 }
                 if (err == 0) {
                     // blocks -> items in block order, i (x) outer, j (y), t (z)

@@ -195,7 +195,8 @@ void drop_graph(plp_engine* g) {
 // decoder kind.
 int dyn_lds(plp_engine* g, int ep_cap) {
     if (g->blocks)
-        return plpk::block_lds_layout(ep_cap, g->box_cap, g->max_types, g->use_rng, g->wpd,
+        return plpk::block_lds_layout(ep_cap, g->box_cap, g->full_box, g->max_types, g->use_rng,
+                                      g->wpd,
                                       g->mode == plpk::kBig)
             .total;
     if (g->verd == plpk::kPointState)
@@ -827,8 +828,8 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
     g->max_items = max_items;
     g->full_box = g->step_cap = round_up(max_items, 8);   // sentinel-padded groups of 8
     // the block engine keeps every item (its blocks decompose into items)
-    g->box_cap = (bp || g->cfg.box_cap <= 0) ? g->full_box
-                                             : std::min(g->full_box, round_up(g->cfg.box_cap, 8));
+    g->box_cap = g->cfg.box_cap <= 0 ? g->full_box
+                                     : std::min(g->full_box, round_up(g->cfg.box_cap, 8));
     g->mode = big ? plpk::kBig : wide ? plpk::kWide : plpk::kNarrow;
     g->params = *params;
     g->blocks = bp != nullptr;
@@ -1040,7 +1041,7 @@ static int sync_impl(plp_engine* g) {
             need_all = std::max(need_all, (int)st.ep_need);
             box_all = std::max(box_all, (int)st.box_need);
         }
-        if (!g->blocks) g->box_cap = std::min(g->full_box, round_up(box_all, 8));
+        g->box_cap = std::min(g->full_box, round_up(box_all, 8));
         if ((rc = shape(g, round_up(need_all, 32), g->cfg.blocks_per_cu, g->n_dec)) != PLP_OK)
             return rc;
         drop_graph(g);
@@ -1209,7 +1210,7 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
         box_need = std::max(box_need, (int)s.box_need);
     }
     if (!g->blocks && (rc = order_by_run_time(g)) != PLP_OK) return rc;
-    if (!g->blocks) g->box_cap = std::min(g->full_box, round_up(box_need, 8));
+    g->box_cap = std::min(g->full_box, round_up(box_need, 8));
     std::vector<int> caps;
     for (int num : {4, 5, 6})   // 1.0x, 1.25x, 1.5x the observed peak
         caps.push_back(std::max(64, round_up(need * num / 4, 32)));
@@ -1237,7 +1238,7 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
         g->verd = c.verd;
         g->cfg.ep_cap = c.cfg.ep_cap;
         g->cfg.blocks_per_cu = c.cfg.blocks_per_cu;
-        if (!g->blocks) g->box_cap = c.cfg.box_cap;
+        g->box_cap = c.cfg.box_cap;
         return shape(g, c.cfg.ep_cap, c.cfg.blocks_per_cu, g->n_dec);
     };
     auto time_runs = [&](int runs, float* ms) {
@@ -1261,14 +1262,14 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
         for (int cap : caps) {
             g->cfg.ep_cap = cap;
             g->cfg.blocks_per_cu = 0;
-            if (!g->blocks) g->box_cap = box_probe;
+            g->box_cap = box_probe;
             if (shape(g, cap, 0, g->n_dec) != PLP_OK) continue;
             const int occ = g->occ;
             std::vector<int> bpcs = {occ, std::max(1, occ * 3 / 4), std::max(1, occ / 2)};
             bpcs.erase(std::unique(bpcs.begin(), bpcs.end()), bpcs.end());
             for (int bpc : bpcs) {
                 g->cfg.blocks_per_cu = bpc;
-                if (!g->blocks) g->box_cap = box_probe;
+                g->box_cap = box_probe;
                 if (shape(g, cap, bpc, g->n_dec) != PLP_OK) continue;
                 float ms = 0.0f;
                 if ((rc = time_runs(1, &ms)) != PLP_OK) return rc;

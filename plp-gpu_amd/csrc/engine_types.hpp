@@ -321,9 +321,13 @@ struct BlockLds {
     int32_t off_box, off_mt, off_types, off_vord, off_gtab, total;
 };
 
+// box_cap: items the placed blocks may hold (the blocks, their decomposed
+// items and FSR's arrays); full_items: every item of the largest instance
+// (the sweep plan and the greedy-plan entries).
 __host__ __device__ inline BlockLds block_lds_layout(int32_t ep_cap, int32_t box_cap,
-                                                     int32_t max_types, int32_t use_rng,
-                                                     int32_t n_waves, int32_t big = 0) {
+                                                     int32_t full_items, int32_t max_types,
+                                                     int32_t use_rng, int32_t n_waves,
+                                                     int32_t big = 0) {
     BlockLds l;
     const int32_t list = (ep_cap + 8) * (big ? 8 : 4);
     l.off_ep = 0;
@@ -333,7 +337,7 @@ __host__ __device__ inline BlockLds block_lds_layout(int32_t ep_cap, int32_t box
     l.off_rem = l.off_mask + ((ep_cap + 63) / 64) * 8;
     l.off_tlist = l.off_rem + max_types * 4;
     l.off_plan = l.off_tlist + max_types * 4;
-    l.n_plan = next_pow2(box_cap > 0 ? box_cap : 1);
+    l.n_plan = next_pow2(full_items > 0 ? full_items : 1);
     l.off_rel = (l.off_plan + l.n_plan * 4 + 15) & ~15;
     const int32_t pass_end = l.off_rel + n_waves * (64 + 8) * 32;
     l.n_key = next_pow2(box_cap > 0 ? box_cap : 1);
@@ -350,7 +354,7 @@ __host__ __device__ inline BlockLds block_lds_layout(int32_t ep_cap, int32_t box
     l.off_types = l.off_mt + (use_rng ? 624 * 4 : 0);
     l.off_vord = l.off_types + max_types * (int32_t)sizeof(BlockTypeRec);
     l.off_gtab = l.off_vord + max_types * 4;
-    l.total = l.off_gtab + box_cap * (int32_t)sizeof(GreedyEntry);
+    l.total = l.off_gtab + full_items * (int32_t)sizeof(GreedyEntry);
     return l;
 }
 

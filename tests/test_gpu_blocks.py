@@ -157,6 +157,36 @@ def test_block_capacity_escalation(plp, ctx, orc):
         assert {x: got[x] for x in FIELDS} == {x: want[x] for x in FIELDS}, k
 
 
+@pytest.mark.parametrize("box_cap", [8, 24, 64])
+def test_block_item_capacity_rerun(plp, ctx, box_cap):
+    """Room for fewer items than the placed blocks hold: decoders stop when a
+    block would pass it and are rerun with room for every item; every result
+    equals the pinned digests of the 20-type instance x 1024 x 4 passes, run
+    after run, and the autotuner's fitted capacity gives them too."""
+    import json
+    from conftest import golden_path
+    from plp_amd import synth
+    with open(golden_path("fullsize_digests.json")) as f:
+        pin = json.load(f)["workloads"]["blocks_twenty_1024x4"]["per_decoder"]
+    texts, dec_inst, seeds, passes = synth.workload("blocks_twenty_1024x4")
+    eng = plp.Engine(ctx, box_cap=box_cap, use_graph=True)
+    eng.load([plp.Instance.from_text(t) for t in texts], dec_inst, seeds,
+             plp.params(eps=0.01, theta=0.9), blocks=plp.block_params())
+    for _ in range(2):
+        eng.reset()
+        eng.run(passes)
+        rows, pl = eng.sync_fetch()
+        assert all(r.err == 0 for r in rows)
+        assert plp.results_digest(rows, pl)[1] == pin
+    tuned, _ = eng.autotune(passes)
+    assert 0 < tuned["box_cap"] <= eng.max_items()
+    eng.reset()
+    eng.run(passes)
+    rows, pl = eng.sync_fetch()
+    assert plp.results_digest(rows, pl)[1] == pin
+    eng.close()
+
+
 BLOCK_CLI = __import__("os").path.join(__import__("conftest").REPO, "plp-gpu_amd", "build",
                                        "plp_block")
 
