@@ -392,48 +392,108 @@ __global__ void __launch_bounds__(64 * NW, 3) block_kernel(BlockArgs ba) {
                 // each test's shape record is read one test ahead
                 BlockShapeRec S_next = shapes[uni((int)plan[0])];
                 for (int q = 0; q < n_plan; q++) {
-                    const int s = uni((int)plan[q]);
-                    const BlockShapeRec S = S_next;
-                    if (q + 1 < n_plan) S_next = shapes[uni((int)plan[q + 1])];
-                    const int ti = uni(S.ti), cnt_s = uni(S.cnt);
-                    const int bw = uni(S.bw), bd = uni(S.bd), bh = uni(S.bh);
-                    if (uni(rem[ti]) < cnt_s) continue;   // planned before, stale
-                    if (s == fail_s) continue;            // same test, same (empty) answer
                     const Key* act = cur ? ep_b : ep_a;
                     const int n_chunks = (na_w + kWave - 1) / kWave;
-                    int cnt_w = 0;
-                    for (int c = wid; c < n_chunks; c += NW) {
-                        const int e = c * kWave + lane;
-                        const bool ok = e < na_w;
+                    BlockShapeRec S;   // the test that places
+                    int cnt;           // its feasible points
+                    if (NW == 1 && na_w <= kWave / 2) {
+                        // Short list, one wave: the tests of plan[q .. q+G) run
+                        // together, test g on lanes [g*seg, (g+1)*seg), lane =
+                        // point.  The tests see the same list (nothing is placed
+                        // before the first that finds a point), so the first
+                        // group with a feasible point is the serial loop's next
+                        // placement, and the groups before it found nothing.
+                        const int lg = na_w <= kWave / 4 ? 4 : 5;   // log2(seg)
+                        const int g = lane >> lg;
+                        const int qg = q + g;
+                        const bool inr = qg < n_plan;
+                        const int sg = inr ? (int)plan[qg] : 0;
+                        const int sp = g == 0 ? fail_s : (inr ? (int)plan[qg - 1] : -1);
+                        const BlockShapeRec Sg = shapes[sg];
+                        // stale tests and repeats of the test before (the same
+                        // answer) test nothing
+                        const bool tv = inr && rem[Sg.ti] >= Sg.cnt && sg != sp;
+                        const int e = lane & ((1 << lg) - 1);
+                        const bool ok = tv & (e < na_w);
                         int x, y, z;
-                        key_xyz(act[ok ? e : 0], xb, yb, x, y, z);   // in bounds; ok lanes count
-                        const bool c2 = ok & (x + bw <= L) & (y + bd <= W) & (z + bh <= H);
+                        key_xyz(act[ok ? e : 0], xb, yb, x, y, z);
+                        const bool c2 = ok & (x + Sg.bw <= L) & (y + Sg.bd <= W) & (z + Sg.bh <= H);
                         uint64_t m = 0;
                         if (ballot(c2) != 0) {
+                            // the region of all groups' pairs (a box outside it
+                            // meets none of them)
                             const int xlo = wave_min(c2 ? x : 0x7fffffff);
-                            const int xhi = wave_max(c2 ? x + bw : -1);
+                            const int xhi = wave_max(c2 ? x + Sg.bw : -1);
                             const int ylo = wave_min(c2 ? y : 0x7fffffff);
-                            const int yhi = wave_max(c2 ? y + bd : -1);
+                            const int yhi = wave_max(c2 ? y + Sg.bd : -1);
                             const int zlo = wave_min(c2 ? z : 0x7fffffff);
-                            const int zhi = wave_max(c2 ? z + bh : -1);
+                            const int zhi = wave_max(c2 ? z + Sg.bh : -1);
                             Sum sup = 0;
-                            const bool c12 = pair_scan(x, y, z, bw, bd, bh, c2, box, nb_w, xlo, xhi,
-                                                       ylo, yhi, zlo, zhi, cbox, sup);
-                            m = ballot(pair_verdict(c12, z, bw, bd, sup));
+                            const bool c12 = pair_scan(x, y, z, Sg.bw, Sg.bd, Sg.bh, c2, box, nb_w,
+                                                       xlo, xhi, ylo, yhi, zlo, zhi, cbox, sup);
+                            m = ballot(pair_verdict(c12, z, Sg.bw, Sg.bd, sup));
                         }
-                        mask[c] = m;   // every lane writes the same (uniform) mask
-                        cnt_w += __popcll(m);
-                    }
-                    const int par = ev & 1;
-                    ev++;
-                    sh.cnt[par][wid] = cnt_w;   // every lane: the same value
-                    block_sync();
-                    const int cnt = uni(sum_counts<NW>(sh.cnt[par]));
-                    if (cnt == 0) {
-                        fail_s = s;
-                        continue;
+                        const int q_end = min(q + (kWave >> lg), n_plan);
+                        if (m == 0) {
+                            // every test of the round found nothing
+                            fail_s = uni((int)plan[q_end - 1]);
+                            q = q_end - 1;
+                            if (q + 1 < n_plan) S_next = shapes[uni((int)plan[q + 1])];
+                            continue;
+                        }
+                        const int gw = (__ffsll((unsigned long long)m) - 1) >> lg;
+                        const uint64_t mw = (m >> (gw << lg)) & ((1ull << (1 << lg)) - 1);
+                        q += gw;
+                        if (lane == 0) mask[0] = mw;
+                        S = shapes[uni((int)plan[q])];
+                        if (q + 1 < n_plan) S_next = shapes[uni((int)plan[q + 1])];
+                        cnt = __popcll(mw);
+                        wave_sync();
+                    } else {
+                        const int s = uni((int)plan[q]);
+                        S = S_next;
+                        if (q + 1 < n_plan) S_next = shapes[uni((int)plan[q + 1])];
+                        const int ti = uni(S.ti), cnt_s = uni(S.cnt);
+                        const int bw = uni(S.bw), bd = uni(S.bd), bh = uni(S.bh);
+                        if (uni(rem[ti]) < cnt_s) continue;   // planned before, stale
+                        if (s == fail_s) continue;            // same test, same (empty) answer
+                        int cnt_w = 0;
+                        for (int c = wid; c < n_chunks; c += NW) {
+                            const int e = c * kWave + lane;
+                            const bool ok = e < na_w;
+                            int x, y, z;
+                            // in bounds; ok lanes count
+                            key_xyz(act[ok ? e : 0], xb, yb, x, y, z);
+                            const bool c2 = ok & (x + bw <= L) & (y + bd <= W) & (z + bh <= H);
+                            uint64_t m = 0;
+                            if (ballot(c2) != 0) {
+                                const int xlo = wave_min(c2 ? x : 0x7fffffff);
+                                const int xhi = wave_max(c2 ? x + bw : -1);
+                                const int ylo = wave_min(c2 ? y : 0x7fffffff);
+                                const int yhi = wave_max(c2 ? y + bd : -1);
+                                const int zlo = wave_min(c2 ? z : 0x7fffffff);
+                                const int zhi = wave_max(c2 ? z + bh : -1);
+                                Sum sup = 0;
+                                const bool c12 = pair_scan(x, y, z, bw, bd, bh, c2, box, nb_w, xlo,
+                                                           xhi, ylo, yhi, zlo, zhi, cbox, sup);
+                                m = ballot(pair_verdict(c12, z, bw, bd, sup));
+                            }
+                            mask[c] = m;   // every lane writes the same (uniform) mask
+                            cnt_w += __popcll(m);
+                        }
+                        const int par = ev & 1;
+                        ev++;
+                        sh.cnt[par][wid] = cnt_w;   // every lane: the same value
+                        block_sync();
+                        cnt = uni(sum_counts<NW>(sh.cnt[par]));
+                        if (cnt == 0) {
+                            fail_s = s;
+                            continue;
+                        }
                     }
                     fail_s = -1;
+                    const int ti = uni(S.ti), cnt_s = uni(S.cnt);
+                    const int bw = uni(S.bw), bd = uni(S.bd), bh = uni(S.bh);
                     // ---- pick, place, merge (wave 0)
                     if (wid == 0 && n_it + cnt_s > a.box_cap) {
                         // one block past the item capacity: this decoder stops
