@@ -41,7 +41,13 @@ def _params(plp, a):
             plp.block_params(a["amax"], a["bmax"], a["kmax"], a["topk"], a["pick"]))
 
 
-def test_every_reference_block_run(plp, ctx):
+# one wave per decoder (the autotuned shape: short lists test several plan
+# entries in one pass) and four (one test at a time)
+WAVES = [1, 4]
+
+
+@pytest.mark.parametrize("wpd", WAVES)
+def test_every_reference_block_run(plp, ctx, wpd):
     """One engine per parameter set, one decoder per fixture case."""
     with open(golden_path("block_runs.json")) as f:
         cases = json.load(f)
@@ -53,7 +59,7 @@ def test_every_reference_block_run(plp, ctx):
     for key, cs in groups.items():
         prm, bp = _params(plp, cs[0]["args"])
         insts = [plp.Instance.from_text(load_text(c["instance"])) for c in cs]
-        eng = plp.Engine(ctx)
+        eng = plp.Engine(ctx, waves_per_decoder=wpd)
         eng.load(insts, list(range(len(cs))), [c["args"]["seed"] for c in cs], prm, blocks=bp)
         eng.run(key[0])
         eng.sync()
@@ -83,12 +89,13 @@ def test_many_block_decoders_vs_oracle(plp, ctx, orc, wpd):
         assert {x: got[x] for x in FIELDS} == {x: want[x] for x in FIELDS}, k
 
 
+@pytest.mark.parametrize("wpd", WAVES)
 @pytest.mark.parametrize("pick", ["support", "contact"])
-def test_contact_pick_rules_vs_oracle(plp, ctx, orc, pick):
+def test_contact_pick_rules_vs_oracle(plp, ctx, orc, pick, wpd):
     text = load_text("data/instances/BR2_050.txt")
     o_inst = orc.parse(text)[0]
     K = 48
-    eng = plp.Engine(ctx)
+    eng = plp.Engine(ctx, waves_per_decoder=wpd)
     eng.load([plp.Instance.from_text(text)], [0] * K, list(range(100, 100 + K)),
              plp.params(eps=0.1, theta=0.9), blocks=plp.block_params(pick=pick, topk=4))
     eng.run(1)
@@ -100,14 +107,15 @@ def test_contact_pick_rules_vs_oracle(plp, ctx, orc, pick):
         assert {x: got[x] for x in FIELDS} == {x: want[x] for x in FIELDS}, k
 
 
-def test_mixed_block_batch_vs_oracle(plp, ctx, orc):
+@pytest.mark.parametrize("wpd", WAVES)
+def test_mixed_block_batch_vs_oracle(plp, ctx, orc, wpd):
     """Uneven instances (3..100 types) x 3 seeds in one launch."""
     from plp_amd import synth
     texts = synth.mixed_batch(12, seed0=515_000)
     insts = [plp.Instance.from_text(t) for t in texts]
     di = [i for i in range(len(texts)) for _ in range(3)]
     ds = [7 + s for _ in range(len(texts)) for s in range(3)]
-    eng = plp.Engine(ctx)
+    eng = plp.Engine(ctx, waves_per_decoder=wpd)
     eng.load(insts, di, ds, plp.params(eps=0.2, theta=0.9), blocks=plp.block_params(kmax=3))
     eng.run(2)
     eng.sync()
@@ -118,8 +126,9 @@ def test_mixed_block_batch_vs_oracle(plp, ctx, orc):
         assert {x: got[x] for x in FIELDS} == {x: want[x] for x in FIELDS}, k
 
 
+@pytest.mark.parametrize("wpd", WAVES)
 @pytest.mark.parametrize("eps", [0.0, 1.0])
-def test_block_edge_instances_vs_oracle(plp, ctx, orc, eps):
+def test_block_edge_instances_vs_oracle(plp, ctx, orc, eps, wpd):
     """The decoder's degenerate instances (nothing fits, one item, types
     without an allowed orientation, an exact tiling) through the block engine,
     seeds 0, 1 and 2^64 - 1; eps 1.0 shuffles every plan and picks every shape
@@ -128,7 +137,7 @@ def test_block_edge_instances_vs_oracle(plp, ctx, orc, eps):
     seeds = [0, 1, 2**64 - 1]
     di = [i for i in range(len(names)) for _ in seeds]
     ds = [s for _ in names for s in seeds]
-    eng = plp.Engine(ctx)
+    eng = plp.Engine(ctx, waves_per_decoder=wpd)
     eng.load([plp.Instance.from_text(EDGE_TEXTS[n]) for n in names], di, ds,
              plp.params(eps=eps, theta=0.9), blocks=plp.block_params())
     eng.run(2)
@@ -157,8 +166,8 @@ def test_block_capacity_escalation(plp, ctx, orc):
         assert {x: got[x] for x in FIELDS} == {x: want[x] for x in FIELDS}, k
 
 
-@pytest.mark.parametrize("box_cap", [8, 24, 64])
-def test_block_item_capacity_rerun(plp, ctx, box_cap):
+@pytest.mark.parametrize("box_cap,wpd", [(8, 4), (24, 4), (64, 4), (24, 1)])
+def test_block_item_capacity_rerun(plp, ctx, box_cap, wpd):
     """Room for fewer items than the placed blocks hold: decoders stop when a
     block would pass it and are rerun with room for every item; every result
     equals the pinned digests of the 20-type instance x 1024 x 4 passes, run
@@ -169,7 +178,7 @@ def test_block_item_capacity_rerun(plp, ctx, box_cap):
     with open(golden_path("fullsize_digests.json")) as f:
         pin = json.load(f)["workloads"]["blocks_twenty_1024x4"]["per_decoder"]
     texts, dec_inst, seeds, passes = synth.workload("blocks_twenty_1024x4")
-    eng = plp.Engine(ctx, box_cap=box_cap, use_graph=True)
+    eng = plp.Engine(ctx, box_cap=box_cap, use_graph=True, waves_per_decoder=wpd)
     eng.load([plp.Instance.from_text(t) for t in texts], dec_inst, seeds,
              plp.params(eps=0.01, theta=0.9), blocks=plp.block_params())
     for _ in range(2):

@@ -114,8 +114,9 @@ def _check_all(plp, ctx, insts, dec_inst, rows, pl, theta, per_box_support=True)
 
 
 def _run(plp, ctx, texts, dec_inst, seeds, passes, blocks=None, count_work=False,
-         verdicts=0):
-    eng = plp.Engine(ctx, use_graph=not count_work, count_work=count_work, verdicts=verdicts)
+         verdicts=0, waves=0):
+    eng = plp.Engine(ctx, use_graph=not count_work, count_work=count_work, verdicts=verdicts,
+                     waves_per_decoder=waves)
     eng.load([plp.Instance.from_text(tx) for tx in texts], dec_inst, seeds,
              plp.params(eps=EPS, theta=THETA), blocks=blocks)
     eng.run(passes)
@@ -239,3 +240,6 @@ def test_block_engine_1024_decoders_4_passes(plp, ctx, orc):
     _check_all(plp, ctx, [inst], [0] * K, rows, pl, THETA, per_box_support=False)
     _compare_pinned(plp, "blocks_twenty_1024x4", rows, pl)
     _compare(plp, rows, pl, _oracle_all(orc, [inst], [0] * K, seeds, P, block=True))
+    # one wave per decoder, the autotuned shape (several plan tests a pass)
+    rows1, pl1 = _run(plp, ctx, [text], [0] * K, seeds, P, blocks=plp.block_params(), waves=1)
+    _compare_pinned(plp, "blocks_twenty_1024x4", rows1, pl1)
