@@ -21,8 +21,9 @@
 //
 // One block of NW waves per (instance, seed), persistent over a work queue
 // like decode_kernel.  Every wave tests 64-point chunks c = wave, wave + NW,
-// ...; wave 0 runs the plan, the picks, the extreme-point merge and the pass
-// end.  The extreme-point list keeps only points where a box of the
+// ...; with one wave and at most 32 live points, the next 2 or 4 tests of the
+// plan share one pass over the list instead.  Wave 0 runs the plan, the
+// picks, the extreme-point merge and the pass end.  The extreme-point list keeps only points where a box of the
 // remaining types' smallest dimensions fits (every later block contains one
 // such box), so the feasible points, and so every pick, are the reference's.
 #include <hip/hip_runtime.h>
