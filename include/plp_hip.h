@@ -144,7 +144,9 @@ typedef struct {
 } plp_params;
 
 typedef struct {
-    int32_t waves_per_decoder;   /* wavefronts per decoder block: 1, 2, 4, 8; 0 = automatic (4) */
+    int32_t waves_per_decoder;   /* wavefronts per decoder block: 1, 2, 4, 8; 0 = automatic
+                                    (4 for the item decoder, 1 for the block engine, or what
+                                    plp_engine_autotune chose) */
     int32_t ep_cap;              /* extreme-point capacity per decoder, 0 = automatic */
     int32_t blocks_per_cu;       /* persistent blocks per CU, 0 = as many as fit */
     int32_t use_graph;           /* capture the launch sequence in a hipGraph */

@@ -42,6 +42,7 @@ struct plp_engine {
     int full_box = 8;   // every item of the largest instance, rounded up to 8
     int step_cap = 8;   // step-table entries in LDS
     int ep_cap = 0, lds = 0, grid = 0, occ = 0, wpd = 1;
+    bool wpd_tuned = false;   // wpd chosen by plp_engine_autotune (kept across loads)
     int mode = plpk::kNarrow;   // widths of keys, support sums and cache entries (StateMode)
     int verd = plpk::kPointState;   // how the item decoder reaches its verdicts (VerdictKind)
     bool blocks = false;   // block-building decoder (plp_engine_load_blocks)
@@ -834,6 +835,10 @@ int load_impl(plp_engine* g, const plp_instance_desc* insts, int32_t n_inst,
     g->params = *params;
     g->blocks = bp != nullptr;
     if (bp) g->bp = *bp;
+    // automatic waves per decoder: one for the block engine (its chain is
+    // the latency of one wave, and one wave tests several plan entries at
+    // once on short lists), four for the item decoder
+    if (g->cfg.waves_per_decoder == 0 && !g->wpd_tuned) g->wpd = g->blocks ? 1 : 4;
     g->max_types = max_types;
     g->use_rng = bp ? (params->eps > 0.0 ? 1 : 0)
                     : ((params->first_fit || params->eps == 0.0) ? 0 : 1);
@@ -1307,6 +1312,7 @@ extern "C" int plp_engine_autotune(plp_engine* g, int32_t n_passes, plp_engine_c
     g->cfg.box_cap = bcfg.box_cap;
     g->box_cap = bcfg.box_cap > 0 ? bcfg.box_cap : g->box_cap;
     g->wpd = bcfg.waves_per_decoder > 0 ? bcfg.waves_per_decoder : saved_wpd;
+    g->wpd_tuned = true;
     g->verd = best_verd;
     rc = shape(g, bcfg.ep_cap, bcfg.blocks_per_cu, g->n_dec);
     if (rc != PLP_OK) return rc;
