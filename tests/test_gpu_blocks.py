@@ -160,6 +160,8 @@ def test_block_capacity_escalation(plp, ctx, orc):
     eng.sync()
     rows, pl = eng.fetch()
     assert eng.describe()["ep_cap"] >= max(r.ep_need for r in rows) > 16
+    # waves per decoder left automatic: one for the block engine
+    assert eng.describe()["waves_per_decoder"] == 1
     for k in range(4):
         want = orc.block_run(orc.parse(text)[0], seed=k + 1, passes=2, eps=0.3, theta=0.9)
         got = plp.to_json_dict(rows[k], pl[k])
